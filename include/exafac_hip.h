@@ -1,0 +1,177 @@
+/*
+ * exafac_hip.h -- C-ABI of libexafac_hip.so, the MI355X (gfx950) randomized
+ * CP-ALS inner loop of rdist_tensor (CP-ARLS-LEV and STS-CP) for one GPU that
+ * holds the whole tensor.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (EX_OK = 0).  No C++ type or exception crosses this boundary.  The message
+ * of the last failure on the calling thread is returned by ex_last_error().
+ * Host arrays are row-major.  Calls on one context are serialised on that
+ * context's HIP stream; objects are not thread-safe.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the rdist_tensor source tree).  The pybind11 module
+ * rdist_tensor_amd/cpp_ext/py_module.cpp binds these to the reference's
+ * Python surface (exafac/cpp_ext/py_module.cpp:37-77).
+ */
+#ifndef EXAFAC_HIP_H
+#define EXAFAC_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ex_ctx ex_ctx;
+typedef struct ex_tensor ex_tensor;
+typedef struct ex_factors ex_factors;
+typedef struct ex_sampler ex_sampler;
+typedef struct ex_als ex_als;
+
+enum ex_status {
+    EX_OK = 0,
+    EX_ERR_ARG = 1,          /* bad argument (shape, range, null) */
+    EX_ERR_HIP = 2,          /* HIP runtime error */
+    EX_ERR_STATE = 3,        /* call out of order */
+    EX_ERR_UNSUPPORTED = 4,  /* outside the supported envelope (e.g. R > 64) */
+    EX_ERR_NOMEM = 5         /* device allocation failed */
+};
+
+enum ex_precision { EX_FP64 = 0, EX_MIXED = 1 };
+enum ex_sampler_kind { EX_SAMPLER_CP_ARLS_LEV = 0, EX_SAMPLER_STS_CP = 1 };
+enum ex_distribution { EX_DIST_ACCUMULATOR_STATIONARY = 0, EX_DIST_TENSOR_STATIONARY = 1 };
+enum ex_preprocessing { EX_PRE_NONE = 0, EX_PRE_LOG_COUNT = 1, EX_PRE_ONES = 2 };
+
+/* Maximum decomposition rank supported by the kernels. */
+#define EX_MAX_RANK 64
+
+/* ---- errors / context ------------------------------------------------ */
+/* Message of the last failed call on this thread ("" if none). */
+const char* ex_last_error(void);
+/* One context per device: a HIP stream plus scratch.  Replaces the MPI world
+ * set up by mpi4py + Grid (grid.hpp:34-108) -- one rank, one GPU. */
+int ex_ctx_create(int device, ex_ctx** out);
+int ex_ctx_destroy(ex_ctx* ctx);
+int ex_ctx_sync(ex_ctx* ctx);
+/* Device name (e.g. "AMD Instinct MI355X") into buf. */
+int ex_ctx_device_name(ex_ctx* ctx, char* buf, size_t len);
+/* Per-launch-group timing on the context stream (no reference counterpart:
+ * the reference's seven region timers, als_optimizer.hpp:21-27, are host
+ * wall clocks).  Enabling clears the totals.  ex_ctx_timing_json returns
+ * {"spans": {name: {"ms": total, "count": n}}, "counters": {...}} with the
+ * same buffer protocol as ex_als_stats_json. */
+int ex_ctx_timing(ex_ctx* ctx, int enable);
+int ex_ctx_timing_json(ex_ctx* ctx, char* buf, size_t* len);
+
+/* ---- sparse tensor ---------------------------------------------------- */
+/* SparseTensor(TensorGrid&, np.ndarray[uint32] (nnz,N), np.ndarray[float64]
+ * (nnz,), str preprocessing)  -- sparse_tensor.hpp:36-96.
+ * Copies idx (nnz x N, 0-based) and vals to the device, applies the
+ * preprocessing (log_count: x <- log(x+1); ones: x <- 1) and computes ||X||.
+ * The reference's random per-mode permutation is the identity here. */
+int ex_tensor_create(ex_ctx* ctx, int N, const int64_t* dims, int64_t nnz,
+                     const uint32_t* idx, const double* vals, int preprocessing,
+                     ex_tensor** out);
+int ex_tensor_destroy(ex_tensor* t);
+/* ||X|| after preprocessing (SparseTensor::tensor_norm, sparse_tensor.hpp:48-71). */
+int ex_tensor_norm(ex_tensor* t, double* norm);
+/* Builds the per-mode nonzero copies sorted by every coordinate except the
+ * target (SortIdxLookup ctor with reorder=true, sort_lookup.hpp:37-102, as
+ * AccumulatorStationary::initialize_ground_truth_for_als builds them,
+ * accumulator_stationary.hpp:38-137).  Idempotent. */
+int ex_tensor_build_lookups(ex_tensor* t);
+/* Sampled MTTKRP with the sample set held fixed (SortIdxLookup::csr_based_spmm,
+ * sort_lookup.hpp:238-375): Y (I_mode x R, overwritten) = sum over the
+ * nonzeros of each sampled fiber of x * design[c].  samples: Ju x N.
+ * *hits receives the number of nonzeros touched (nonzeros_iterated). */
+int ex_sampled_mttkrp(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samples,
+                      const double* design, int64_t R, double* Y, int64_t* hits);
+
+/* ---- factors (LowRankTensor) ----------------------------------------- */
+/* LowRankTensor(R, TensorGrid&) -- low_rank_tensor.hpp:16-25: N factor
+ * matrices I_k x R (FP64 on the device) and sigma = 1. */
+int ex_factors_create(ex_ctx* ctx, int N, const int64_t* dims, int64_t R, ex_factors** out);
+int ex_factors_destroy(ex_factors* f);
+/* Upload / download factor `mode` (I_mode x R). */
+int ex_factors_set(ex_factors* f, int mode, const double* host);
+int ex_factors_get(ex_factors* f, int mode, double* host);
+int ex_factors_get_sigma(ex_factors* f, double* sigma);
+int ex_factors_set_sigma(ex_factors* f, const double* sigma);
+/* DistMat1D::renormalize_columns (distmat.hpp:85-138) of one factor; sigma
+ * receives the column norms unless NULL (then the LowRankTensor's sigma is
+ * left unchanged, as in LowRankTensor::initialize_factors_*). */
+int ex_factors_renormalize(ex_factors* f, int mode, double* sigma);
+/* LowRankTensor::initialize_factors_deterministic (low_rank_tensor.hpp:27-33,
+ * distmat.hpp:140-151): U_k[i][r] = cos(i*R + r), renormalised, sigma = 1. */
+int ex_factors_init_deterministic(ex_factors* f);
+/* DistMat1D::compute_gram_matrix (distmat.hpp:73-83): G = U^T U (R x R). */
+int ex_factors_gram(ex_factors* f, int mode, double* G);
+
+/* ---- samplers ----------------------------------------------------------- */
+/* CP_ARLS_LEV(LowRankTensor&) -- cp_arls_lev.hpp:14-23, or
+ * STS_CP(LowRankTensor&) -- sts_cp.hpp:32-57.  Builds the per-mode state
+ * (leverage-score CDFs, or segment trees) from the current factors.
+ * Draws come from Philox4x32-10 keyed by `seed`. */
+int ex_sampler_create(ex_ctx* ctx, ex_factors* f, int kind, int precision, uint64_t seed,
+                      ex_sampler** out);
+int ex_sampler_destroy(ex_sampler* s);
+/* Sampler::update_sampler(j) -- sampler.hpp:33 (cp_arls_lev.hpp:25-34,
+ * sts_cp.hpp:63-65): rebuild mode j's state from factor j. */
+int ex_sampler_update(ex_sampler* s, int mode);
+/* Sampler::KRPDrawSamples(J, j, samples, weights, ...) -- sampler.hpp:34-38
+ * (cp_arls_lev.hpp:36-163, sts_cp.hpp:203-300).  samples: J x N (column
+ * `mode` is scratch: 0 for CP-ARLS-LEV, the last eigen-component index for
+ * STS-CP); weights: J.  Draws are taken from the generator at (sweep, mode)
+ * unless `draws_row` is given: an N x J array, row k holding the uniform
+ * draw of each sample for mode k (CP-ARLS-LEV row draw, STS-CP row-tree
+ * draw); `draws_eig` (N x J) gives the STS-CP eigen-tree draws. */
+int ex_sampler_draw(ex_sampler* s, int64_t J, int mode, uint64_t sweep,
+                    const double* draws_row, const double* draws_eig,
+                    uint32_t* samples, double* weights);
+/* STS-CP internals for parity checks: the (N+2) x R x R matrix stack built
+ * by STS_CP::computeM(mode) (sts_cp.hpp:80-188), slot k at [k*R*R]. */
+int ex_sts_compute_m(ex_sampler* s, int mode, double* M);
+/* CP-ARLS-LEV internals: leverage scores of factor `mode` (length I_mode). */
+int ex_arls_scores(ex_sampler* s, int mode, double* scores);
+
+/* ---- ALS optimizers ------------------------------------------------------ */
+/* AccumulatorStationary / TensorStationary(SparseTensor&, LowRankTensor&,
+ * Sampler&) -- accumulator_stationary.hpp:28-36, tensor_stationary.hpp:21-49.
+ * At one rank both distributions perform the same arithmetic. */
+int ex_als_create(ex_ctx* ctx, ex_tensor* t, ex_factors* f, ex_sampler* s,
+                  int distribution, int precision, ex_als** out);
+int ex_als_destroy(ex_als* a);
+/* ALS_Optimizer::initialize_ground_truth_for_als -- accumulator_stationary.hpp:38-137 */
+int ex_als_init_ground_truth(ex_als* a);
+/* AccumulatorStationary::execute_ALS_step(mode, J) -- accumulator_stationary.hpp:139-295.
+ * `sweep` keys the draws. */
+int ex_als_step(ex_als* a, int mode, int64_t J, uint64_t sweep);
+/* ALS_Optimizer::execute_ALS_rounds(rounds, J, epoch) -- als_optimizer.hpp:40-124 */
+int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch);
+/* ALS_Optimizer::compute_exact_fit -- als_optimizer.hpp:208-211,
+ * sparse_tensor.hpp:288-356 */
+int ex_als_fit(ex_als* a, double* fit);
+/* ALS_Optimizer::get_statistics_json -- als_optimizer.hpp:126-128.
+ * Writes at most *len bytes (NUL-terminated) and sets *len to the full
+ * length needed (including the NUL). */
+int ex_als_stats_json(ex_als* a, char* buf, size_t* len);
+
+/* ---- dense R x R kernels (parity hooks; host in/out) ----------------------- */
+/* compute_gram (common.h:363-403): G = A^T A, A is m x R. */
+int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G);
+/* LAPACKE_dsyev('V','U') replacement: eigenvalues ascending, eigenvectors in
+ * the columns of V (row-major V[u*R+v]); reads the upper triangle of A. */
+int ex_eigh(ex_ctx* ctx, const double* A, int64_t R, double* lam, double* V);
+/* compute_pinv_square (common.h:405-443) with absolute eigenvalue tolerance tol. */
+int ex_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol, double* out);
+/* deduplicate_design_matrix (als_optimizer.hpp:130-206): returns *Ju unique
+ * tuples (ignoring column `mode`) in ascending order with weight x count. */
+int ex_dedup(ex_ctx* ctx, const uint32_t* samples, const double* weights, int64_t J, int N,
+             int mode, const int64_t* dims, uint32_t* samples_dedup, double* weights_dedup,
+             int64_t* Ju);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

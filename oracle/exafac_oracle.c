@@ -63,6 +63,13 @@ static void philox(uint32_t ctr[4], uint32_t key[2]) {
     }
 }
 
+void orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    uint32_t k[2] = {key[0], key[1]};
+    philox(c, k);
+    for (int i = 0; i < 4; i++) out[i] = c[i];
+}
+
 /* generate_canonical<double,53> over a 32-bit engine takes two words:
  * sum = w0 + w1 * 2^32 (in double), ret = sum / 2^64, clamped below 1
  * (libstdc++ random.tcc:3345-3380). */

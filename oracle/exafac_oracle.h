@@ -18,6 +18,7 @@ extern "C" {
 #endif
 
 /* ---- draws: Philox4x32-10 + generate_canonical<double,53> mapping ---- */
+void   orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double orc_uniform(uint64_t seed, uint64_t stream, uint64_t index);
 void   orc_uniform_fill(uint64_t seed, uint64_t stream, uint64_t first, int64_t count, double* out);
 uint64_t orc_stream_id(uint64_t sweep, uint32_t target_mode, uint32_t draw_mode, uint32_t kind);

@@ -49,6 +49,7 @@ def lib():
             build()
         L = C.CDLL(so)
         sig = {
+            "orc_philox4x32": (None, [_u32, _u32, _u32]),
             "orc_uniform": (C.c_double, [C.c_uint64, C.c_uint64, C.c_uint64]),
             "orc_uniform_fill": (None, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_int64, _d]),
             "orc_stream_id": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32]),
@@ -118,6 +119,14 @@ def _ptr_array(arrs):
 def uniform(seed, stream, first, count):
     out = np.empty(count, np.float64)
     lib().orc_uniform_fill(seed, stream, first, count, _p(out))
+    return out
+
+
+def philox4x32(ctr, key):
+    c = np.asarray(ctr, np.uint32)
+    k = np.asarray(key, np.uint32)
+    out = np.empty(4, np.uint32)
+    lib().orc_philox4x32(_p(c, _u32), _p(k, _u32), _p(out, _u32))
     return out
 
 

@@ -1,0 +1,323 @@
+"""ctypes binding of the C-ABI in include/exafac_hip.h (libexafac_hip.so).
+
+This is the binding a Python caller (or a test) uses to reach the HIP library
+without pybind11.  Raises RuntimeError if the library is missing: there is no
+CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libexafac_hip.so")
+
+EX_OK, EX_ERR_ARG, EX_ERR_HIP, EX_ERR_STATE, EX_ERR_UNSUPPORTED, EX_ERR_NOMEM = range(6)
+EX_FP64, EX_MIXED = 0, 1
+EX_SAMPLER_CP_ARLS_LEV, EX_SAMPLER_STS_CP = 0, 1
+EX_DIST_ACCUMULATOR_STATIONARY, EX_DIST_TENSOR_STATIONARY = 0, 1
+EX_PRE_NONE, EX_PRE_LOG_COUNT, EX_PRE_ONES = 0, 1, 2
+
+_vp = C.c_void_p
+_d = C.POINTER(C.c_double)
+_u32 = C.POINTER(C.c_uint32)
+_i64 = C.POINTER(C.c_int64)
+_pp = C.POINTER(C.c_void_p)
+
+# name -> argtypes (all return int status, except ex_last_error)
+SIGNATURES = {
+    "ex_ctx_create": [C.c_int, _pp],
+    "ex_ctx_destroy": [_vp],
+    "ex_ctx_sync": [_vp],
+    "ex_ctx_device_name": [_vp, C.c_char_p, C.c_size_t],
+    "ex_ctx_timing": [_vp, C.c_int],
+    "ex_ctx_timing_json": [_vp, C.c_char_p, C.POINTER(C.c_size_t)],
+    "ex_tensor_create": [_vp, C.c_int, _i64, C.c_int64, _u32, _d, C.c_int, _pp],
+    "ex_tensor_destroy": [_vp],
+    "ex_tensor_norm": [_vp, _d],
+    "ex_tensor_build_lookups": [_vp],
+    "ex_sampled_mttkrp": [_vp, C.c_int, C.c_int64, _u32, _d, C.c_int64, _d, _i64],
+    "ex_factors_create": [_vp, C.c_int, _i64, C.c_int64, _pp],
+    "ex_factors_destroy": [_vp],
+    "ex_factors_set": [_vp, C.c_int, _d],
+    "ex_factors_get": [_vp, C.c_int, _d],
+    "ex_factors_get_sigma": [_vp, _d],
+    "ex_factors_set_sigma": [_vp, _d],
+    "ex_factors_renormalize": [_vp, C.c_int, _d],
+    "ex_factors_init_deterministic": [_vp],
+    "ex_factors_gram": [_vp, C.c_int, _d],
+    "ex_sampler_create": [_vp, _vp, C.c_int, C.c_int, C.c_uint64, _pp],
+    "ex_sampler_destroy": [_vp],
+    "ex_sampler_update": [_vp, C.c_int],
+    "ex_sampler_draw": [_vp, C.c_int64, C.c_int, C.c_uint64, _d, _d, _u32, _d],
+    "ex_sts_compute_m": [_vp, C.c_int, _d],
+    "ex_arls_scores": [_vp, C.c_int, _d],
+    "ex_als_create": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _pp],
+    "ex_als_destroy": [_vp],
+    "ex_als_init_ground_truth": [_vp],
+    "ex_als_step": [_vp, C.c_int, C.c_int64, C.c_uint64],
+    "ex_als_run": [_vp, C.c_int64, C.c_int64, C.c_int64],
+    "ex_als_fit": [_vp, _d],
+    "ex_als_stats_json": [_vp, C.c_char_p, C.POINTER(C.c_size_t)],
+    "ex_gram": [_vp, _d, C.c_int64, C.c_int64, _d],
+    "ex_eigh": [_vp, _d, C.c_int64, _d, _d],
+    "ex_pinv": [_vp, _d, C.c_int64, C.c_double, _d],
+    "ex_dedup": [_vp, _u32, _d, C.c_int64, C.c_int, C.c_int, _i64, _u32, _d, _i64],
+}
+
+_LIB = None
+
+
+class ExafacError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m rdist_tensor_amd.build` "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.ex_last_error.argtypes = []
+        L.ex_last_error.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != EX_OK:
+        raise ExafacError(rc, lib().ex_last_error().decode())
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        check(lib().ex_ctx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().ex_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def device_name(self):
+        buf = C.create_string_buffer(256)
+        check(lib().ex_ctx_device_name(self.h, buf, 256))
+        return buf.value.decode()
+
+    def sync(self):
+        check(lib().ex_ctx_sync(self.h))
+
+    def timing(self, enable=True):
+        check(lib().ex_ctx_timing(self.h, 1 if enable else 0))
+
+    def timing_json(self):
+        n = C.c_size_t(0)
+        check(lib().ex_ctx_timing_json(self.h, None, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(lib().ex_ctx_timing_json(self.h, buf, C.byref(n)))
+        return buf.value.decode()
+
+    # dense hooks
+    def gram(self, A):
+        A = f64(A)
+        m, R = A.shape
+        G = np.empty((R, R))
+        check(lib().ex_gram(self.h, _p(A), m, R, _p(G)))
+        return G
+
+    def eigh(self, A):
+        A = f64(A)
+        R = A.shape[0]
+        lam = np.empty(R)
+        V = np.empty((R, R))
+        check(lib().ex_eigh(self.h, _p(A), R, _p(lam), _p(V)))
+        return lam, V
+
+    def pinv(self, M, tol=1e-11):
+        M = f64(M)
+        R = M.shape[0]
+        out = np.empty((R, R))
+        check(lib().ex_pinv(self.h, _p(M), R, tol, _p(out)))
+        return out
+
+    def dedup(self, samples, weights, mode, dims):
+        samples = u32(samples)
+        weights = f64(weights)
+        J, N = samples.shape
+        dims = np.asarray(dims, np.int64)
+        sd = np.empty((max(J, 1), N), np.uint32)
+        wd = np.empty(max(J, 1))
+        ju = C.c_int64()
+        check(lib().ex_dedup(self.h, _p(samples, _u32), _p(weights), J, N, mode, _p(dims, _i64),
+                             _p(sd, _u32), _p(wd), C.byref(ju)))
+        return sd[:ju.value].copy(), wd[:ju.value].copy()
+
+
+class Tensor:
+    def __init__(self, ctx, idx, vals, dims, preprocessing=EX_PRE_NONE):
+        self.ctx = ctx
+        self.idx = u32(idx)
+        self.vals = f64(vals)
+        self.dims = np.asarray(dims, np.int64)
+        nnz, N = self.idx.shape
+        self.N = N
+        self.h = C.c_void_p()
+        check(lib().ex_tensor_create(ctx.h, N, _p(self.dims, _i64), nnz, _p(self.idx, _u32), _p(self.vals),
+                                     preprocessing, C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ex_tensor_destroy(self.h)
+
+    def norm(self):
+        v = C.c_double()
+        check(lib().ex_tensor_norm(self.h, C.byref(v)))
+        return v.value
+
+    def build_lookups(self):
+        check(lib().ex_tensor_build_lookups(self.h))
+
+    def sampled_mttkrp(self, mode, samples, design):
+        samples = u32(samples)
+        design = f64(design)
+        Ju = samples.shape[0]
+        R = design.shape[1]
+        Y = np.empty((int(self.dims[mode]), R))
+        hits = C.c_int64()
+        check(lib().ex_sampled_mttkrp(self.h, mode, Ju, _p(samples, _u32), _p(design), R, _p(Y), C.byref(hits)))
+        return Y, hits.value
+
+
+class Factors:
+    def __init__(self, ctx, dims, R):
+        self.ctx = ctx
+        self.dims = np.asarray(dims, np.int64)
+        self.N = len(dims)
+        self.R = R
+        self.h = C.c_void_p()
+        check(lib().ex_factors_create(ctx.h, self.N, _p(self.dims, _i64), R, C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ex_factors_destroy(self.h)
+
+    def set(self, mode, U):
+        U = f64(U)
+        assert U.shape == (self.dims[mode], self.R)
+        check(lib().ex_factors_set(self.h, mode, _p(U)))
+
+    def get(self, mode):
+        U = np.empty((int(self.dims[mode]), self.R))
+        check(lib().ex_factors_get(self.h, mode, _p(U)))
+        return U
+
+    def sigma(self):
+        s = np.empty(self.R)
+        check(lib().ex_factors_get_sigma(self.h, _p(s)))
+        return s
+
+    def set_sigma(self, s):
+        s = f64(s)
+        check(lib().ex_factors_set_sigma(self.h, _p(s)))
+
+    def gram(self, mode):
+        G = np.empty((self.R, self.R))
+        check(lib().ex_factors_gram(self.h, mode, _p(G)))
+        return G
+
+    def init_deterministic(self):
+        check(lib().ex_factors_init_deterministic(self.h))
+
+
+class Sampler:
+    def __init__(self, ctx, factors, kind, precision=EX_FP64, seed=0):
+        self.ctx = ctx
+        self.f = factors
+        self.kind = kind
+        self.h = C.c_void_p()
+        check(lib().ex_sampler_create(ctx.h, factors.h, kind, precision, seed, C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ex_sampler_destroy(self.h)
+
+    def update(self, mode):
+        check(lib().ex_sampler_update(self.h, mode))
+
+    def draw(self, J, mode, sweep=1, draws_row=None, draws_eig=None):
+        N = self.f.N
+        samples = np.empty((J, N), np.uint32)
+        w = np.empty(J)
+        dr = _p(f64(draws_row)) if draws_row is not None else None
+        de = _p(f64(draws_eig)) if draws_eig is not None else None
+        keep = (draws_row, draws_eig)  # noqa: F841 keep arrays alive during the call
+        dr_arr = f64(draws_row) if draws_row is not None else None
+        de_arr = f64(draws_eig) if draws_eig is not None else None
+        dr = _p(dr_arr) if dr_arr is not None else None
+        de = _p(de_arr) if de_arr is not None else None
+        check(lib().ex_sampler_draw(self.h, J, mode, sweep, dr, de, _p(samples, _u32), _p(w)))
+        return samples, w
+
+    def sts_m(self, mode):
+        N, R = self.f.N, self.f.R
+        M = np.empty((N + 2, R, R))
+        check(lib().ex_sts_compute_m(self.h, mode, _p(M)))
+        return M
+
+    def arls_scores(self, mode):
+        s = np.empty(int(self.f.dims[mode]))
+        check(lib().ex_arls_scores(self.h, mode, _p(s)))
+        return s
+
+
+class ALS:
+    def __init__(self, ctx, tensor, factors, sampler, distribution=EX_DIST_ACCUMULATOR_STATIONARY,
+                 precision=EX_FP64):
+        self.ctx = ctx
+        self.keep = (tensor, factors, sampler)
+        self.h = C.c_void_p()
+        check(lib().ex_als_create(ctx.h, tensor.h, factors.h, sampler.h, distribution, precision, C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ex_als_destroy(self.h)
+
+    def init(self):
+        check(lib().ex_als_init_ground_truth(self.h))
+
+    def step(self, mode, J, sweep):
+        check(lib().ex_als_step(self.h, mode, J, sweep))
+
+    def run(self, rounds, J, epoch=1):
+        check(lib().ex_als_run(self.h, rounds, J, epoch))
+
+    def fit(self):
+        v = C.c_double()
+        check(lib().ex_als_fit(self.h, C.byref(v)))
+        return v.value
+
+    def stats_json(self):
+        n = C.c_size_t(0)
+        check(lib().ex_als_stats_json(self.h, None, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        check(lib().ex_als_stats_json(self.h, buf, C.byref(n)))
+        return buf.value.decode()

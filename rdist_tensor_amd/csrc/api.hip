@@ -1,0 +1,978 @@
+// C-ABI of libexafac_hip.so (include/exafac_hip.h): object lifetimes, the
+// randomized ALS step orchestration and the statistics JSON.
+//
+// Reference flow: ALS_Optimizer::execute_ALS_rounds (als_optimizer.hpp:40-124)
+// over AccumulatorStationary::execute_ALS_step (accumulator_stationary.hpp:139-295).
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/exafac_hip.h"
+#include "common.hpp"
+
+using namespace exafac;
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return EX_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return EX_ERR_STATE;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return EX_ERR_STATE;
+    }
+}
+
+void h2d(hipStream_t s, void* d, const void* h, size_t bytes) {
+    if (bytes) EX_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+}
+void d2h(hipStream_t s, void* h, const void* d, size_t bytes) {
+    if (bytes) EX_HIP(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s));
+    EX_HIP(hipStreamSynchronize(s));
+}
+
+std::string json_num(double v) {
+    if (!std::isfinite(v)) return "null";
+    std::ostringstream o;
+    o.precision(17);
+    o << v;
+    return o.str();
+}
+
+// compute_dstat (common.h:613-675) at one rank: mean = min = max = x; the
+// reference's std divides 0 by (world_size - 1) = 0, giving NaN (JSON null).
+std::string dstat(double x) {
+    return "{\"max\": " + json_num(x) + ", \"mean\": " + json_num(x) + ", \"min\": " + json_num(x) +
+           ", \"std\": null}";
+}
+
+template <class T>
+std::string json_list(const std::vector<T>& v) {
+    std::string s = "[";
+    for (size_t i = 0; i < v.size(); i++) {
+        if (i) s += ", ";
+        if constexpr (std::is_floating_point<T>::value) s += json_num((double)v[i]);
+        else s += std::to_string(v[i]);
+    }
+    return s + "]";
+}
+
+}  // namespace
+
+struct ex_ctx {
+    Ctx c;
+    // opt-in per-launch-group timing (ex_ctx_timing): events are recorded
+    // around named launch groups on the context's stream and resolved after
+    // the stream is synchronised
+    bool timing = false;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<std::string, std::pair<double, int64_t>>> totals;
+    std::vector<std::pair<std::string, double>> counters;
+    hipEvent_t get_event() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        EX_HIP(hipEventCreate(&e));
+        return e;
+    }
+    void add_counter(const std::string& n, double v) {
+        for (auto& c2 : counters)
+            if (c2.first == n) { c2.second += v; return; }
+        counters.push_back({n, v});
+    }
+    void resolve() {
+        for (auto& p : pending) {
+            float ms = 0.f;
+            EX_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+            bool found = false;
+            for (auto& t : totals)
+                if (t.first == p.first) { t.second.first += ms; t.second.second += 1; found = true; break; }
+            if (!found) totals.push_back({p.first, {ms, 1}});
+            pool.push_back(p.second.first);
+            pool.push_back(p.second.second);
+        }
+        pending.clear();
+    }
+};
+
+// Marketing name when the runtime reports one, else the ISA name.
+std::string device_label(const ex_ctx* c) {
+    std::string n = c->c.prop.name;
+    std::string arch = c->c.prop.gcnArchName;
+    if (n.empty()) return arch;
+    return n + " (" + arch + ")";
+}
+
+// Records a named interval on the context stream when timing is enabled.
+struct Span {
+    ex_ctx* c;
+    std::string name;
+    hipEvent_t a = nullptr;
+    Span(ex_ctx* ctx, const char* n) : c(ctx), name(n) {
+        if (c->timing) {
+            a = c->get_event();
+            EX_HIP(hipEventRecord(a, c->c.stream));
+        }
+    }
+    void end() {
+        if (c->timing && a) {
+            hipEvent_t b = c->get_event();
+            EX_HIP(hipEventRecord(b, c->c.stream));
+            c->pending.push_back({name, {a, b}});
+            a = nullptr;
+        }
+    }
+};
+
+struct ex_tensor {
+    ex_ctx* ctx = nullptr;
+    int N = 0;
+    int64_t dims[kMaxModes] = {};
+    int64_t nnz = 0;
+    DevBuf coords;  // u32 nnz x N
+    DevBuf vals;    // f64 nnz
+    double norm = 0.0;
+    bool built = false;
+    ModeLookup lk[kMaxModes];
+    DevBuf tmp[4];
+};
+
+struct ex_factors {
+    ex_ctx* ctx = nullptr;
+    int N = 0, R = 0;
+    int64_t dims[kMaxModes] = {};
+    DevBuf U[kMaxModes];
+    DevBuf sigma;
+    DevBuf part, gram;
+    const double* ptrs[kMaxModes] = {};
+};
+
+struct ex_sampler {
+    ex_ctx* ctx = nullptr;
+    ex_factors* f = nullptr;
+    int kind = 0, precision = 0;
+    uint64_t seed = 0;
+    // CP-ARLS-LEV
+    DevBuf scores[kMaxModes], cdf[kMaxModes], sums, gtmp, gpinv;
+    // STS-CP
+    TreeGeom g[kMaxModes];
+    TreeGeom ge;
+    DevBuf nodes[kMaxModes];
+    DevBuf grams, M, lam, eig, pinvp;
+    DevBuf part;
+    // draw outputs and injected draws
+    DevBuf samples, weights, draws_row, draws_eig;
+};
+
+struct ex_als {
+    ex_ctx* ctx = nullptr;
+    ex_tensor* t = nullptr;
+    ex_factors* f = nullptr;
+    ex_sampler* s = nullptr;
+    int dist = 0, precision = 0;
+    bool initialized = false;
+    DevBuf sd, wd, kd, dg, dm, G, Gi, Y, scal;
+    DevBuf tmp[4];
+    DevBuf part;
+    double t_region[7] = {};
+    double nonzeros = 0.0;
+    std::string stats = "{}";
+};
+
+namespace {
+
+void sampler_update_impl(ex_sampler* s, int j) {
+    ex_factors* f = s->f;
+    hipStream_t st = s->ctx->c.stream;
+    const int R = f->R;
+    const int64_t I = f->dims[j];
+    const double* U = f->U[j].as<double>();
+    Span sp(s->ctx, s->kind == EX_SAMPLER_CP_ARLS_LEV ? "arls_update" : "tree_build");
+    if (s->kind == EX_SAMPLER_CP_ARLS_LEV) {
+        // cp_arls_lev.hpp:25-34: G = U^T U, pinv (tol 1e-11), l_i = u_i^T G+ u_i
+        s->gtmp.ensure((size_t)R * R * 8);
+        s->gpinv.ensure((size_t)R * R * 8);
+        k::gram(st, U, I, R, s->gtmp.as<double>(), s->part);
+        k::pinv(st, s->gtmp.as<double>(), R, 1e-11, s->gpinv.as<double>());
+        s->scores[j].ensure((size_t)I * 8);
+        s->cdf[j].ensure((size_t)I * 8);
+        k::dagat(st, U, I, R, s->gpinv.as<double>(), s->scores[j].as<double>());
+        k::arls_cdf(st, s->scores[j].as<double>(), I, s->cdf[j].as<double>(), s->sums.as<double>() + j, s->part);
+    } else {
+        // sts_cp.hpp:63-65 -> ExactLeverageTree::build_tree: the local tree and
+        // ancestor_grams[0] = compute_gram(U_j)
+        k::tree_build(st, U, s->g[j], R, s->nodes[j].as<double>());
+        k::gram(st, U, I, R, s->grams.as<double>() + (int64_t)j * R * R, s->part);
+    }
+    sp.end();
+}
+
+void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const double* draws_row_dev,
+                       const double* draws_eig_dev, uint32_t* samples, double* weights) {
+    ex_factors* f = s->f;
+    hipStream_t st = s->ctx->c.stream;
+    const int N = f->N, R = f->R;
+    if (s->kind == EX_SAMPLER_CP_ARLS_LEV) {
+        const double* cdfs[kMaxModes];
+        const double* sc[kMaxModes];
+        for (int k2 = 0; k2 < N; k2++) {
+            cdfs[k2] = s->cdf[k2].as<double>();
+            sc[k2] = s->scores[k2].as<double>();
+        }
+        Span sp(s->ctx, "arls_draw");
+        k::arls_draw(st, N, j, J, cdfs, sc, s->sums.as<double>(), f->dims, s->seed, sweep, draws_row_dev, samples,
+                     weights);
+        sp.end();
+    } else {
+        Span sm(s->ctx, "sts_compute_m");
+        k::sts_compute_m(st, N, j, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
+        k::eig_tree_build(st, N, j, R, s->M.as<double>(), s->grams.as<double>(), s->eig.as<double>());
+        k::pack_upper(st, s->M.as<double>(), R, s->pinvp.as<double>());
+        sm.end();
+        k::StsModeArgs m[kMaxModes];
+        const int S = tri_stride(R);
+        for (int k2 = 0; k2 < N; k2++) {
+            m[k2].rows = f->U[k2].as<double>();
+            m[k2].nodes = s->nodes[k2].as<double>();
+            m[k2].eig = s->eig.as<double>() + (int64_t)k2 * s->ge.node_count * S;
+            int off = (k2 + 1 == j) ? k2 + 2 : k2 + 1;
+            m[k2].w = s->M.as<double>() + (int64_t)off * R * R;
+            m[k2].g = s->g[k2];
+            m[k2].ge = s->ge;
+        }
+        Span sd(s->ctx, "sts_draw");
+        k::sts_draw(st, N, j, R, J, m, s->pinvp.as<double>(), s->seed, sweep, draws_row_dev, draws_eig_dev, samples,
+                    weights);
+        sd.end();
+    }
+}
+
+void tensor_build_lookups(ex_tensor* t) {
+    if (t->built) return;
+    hipStream_t st = t->ctx->c.stream;
+    const int64_t nnz = t->nnz;
+    for (int mode = 0; mode < t->N; mode++) {
+        ModeLookup& L = t->lk[mode];
+        L.mode = mode;
+        L.kl = make_key_layout(t->N, mode, t->dims);
+        EX_CHECK(L.kl.bits <= 64, EX_ERR_UNSUPPORTED,
+                 "coordinates of the non-target modes need " + std::to_string(L.kl.bits) + " > 64 key bits");
+        size_t sort_bytes = k::sort_pairs_u64_u32(st, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nnz, L.kl.bits);
+        t->tmp[0].ensure((size_t)nnz * 16 + 512);
+        t->tmp[1].ensure((size_t)nnz * 8 + 512);
+        t->tmp[2].ensure(sort_bytes + 256);
+        uint64_t* kin = t->tmp[0].as<uint64_t>();
+        uint64_t* kout = kin + nnz;
+        uint32_t* iin = t->tmp[1].as<uint32_t>();
+        uint32_t* iout = iin + nnz;
+        k::keys_from_coords(st, t->coords.as<uint32_t>(), nnz, L.kl, kin, iin);
+        if (nnz) k::sort_pairs_u64_u32(st, t->tmp[2].p, t->tmp[2].bytes, kin, kout, iin, iout, nnz, L.kl.bits);
+        L.rowj.ensure((size_t)nnz * 4);
+        L.val.ensure((size_t)nnz * 8);
+        k::gather_lookup(st, iout, t->coords.as<uint32_t>(), t->vals.as<double>(), nnz, t->N, mode,
+                         L.rowj.as<uint32_t>(), L.val.as<double>());
+        L.nfib = k::fiber_index(st, kout, nnz, L.fkey, L.fptr, t->tmp[3]);
+    }
+    EX_HIP(hipStreamSynchronize(st));
+    for (auto& b : t->tmp) b.release();
+    t->built = true;
+}
+
+struct EventTimer {
+    hipEvent_t ev[8];
+    int n = 0;
+    explicit EventTimer(int count) : n(count) {
+        for (int i = 0; i < n; i++) EX_HIP(hipEventCreate(&ev[i]));
+    }
+    ~EventTimer() {
+        for (int i = 0; i < n; i++) (void)hipEventDestroy(ev[i]);
+    }
+    void mark(int i, hipStream_t s) { EX_HIP(hipEventRecord(ev[i], s)); }
+    double secs(int a, int b) {
+        float ms = 0.f;
+        EX_HIP(hipEventElapsedTime(&ms, ev[a], ev[b]));
+        return ms * 1e-3;
+    }
+};
+
+void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
+    ex_factors* f = a->f;
+    ex_sampler* s = a->s;
+    hipStream_t st = a->ctx->c.stream;
+    const int N = f->N, R = f->R;
+    const int64_t Ij = f->dims[j];
+    EX_CHECK(J > 0 && J < (int64_t)1 << 31, EX_ERR_ARG, "sample count J must be in [1, 2^31)");
+    EventTimer ev(6);
+    ev.mark(0, st);
+    // Region 1: draw samples (accumulator_stationary.hpp:148-153)
+    s->samples.ensure((size_t)J * N * 4);
+    s->weights.ensure((size_t)J * 8);
+    sampler_draw_impl(s, J, j, sweep, nullptr, nullptr, s->samples.as<uint32_t>(), s->weights.as<double>());
+    ev.mark(1, st);
+    // Region 3: dedup, design, Gram, pinv (:176-250)
+    a->sd.ensure((size_t)J * N * 4);
+    a->wd.ensure((size_t)J * 8);
+    a->kd.ensure((size_t)J * 8);
+    const KeyLayout kl = a->t->lk[j].kl;
+    Span s_dd(a->ctx, "dedup");
+    const int64_t Ju = k::dedup(st, s->samples.as<uint32_t>(), s->weights.as<double>(), J, N, kl,
+                                a->sd.as<uint32_t>(), a->wd.as<double>(), a->kd.as<uint64_t>(), a->tmp);
+    a->dg.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
+    a->dm.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
+    a->G.ensure((size_t)R * R * 8);
+    a->Gi.ensure((size_t)R * R * 8);
+    s_dd.end();
+    Span s_de(a->ctx, "design_gather");
+    k::design(st, a->sd.as<uint32_t>(), a->wd.as<double>(), Ju, N, j, R, f->ptrs, a->dg.as<double>(),
+              a->dm.as<double>());
+    s_de.end();
+    Span s_dp(a->ctx, "design_gram_pinv");
+    k::gram(st, a->dg.as<double>(), Ju, R, a->G.as<double>(), a->part);
+    k::pinv(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>());
+    s_dp.end();
+    ev.mark(2, st);
+    // Region 4: sampled MTTKRP (:252-274)
+    a->Y.ensure((size_t)Ij * R * 8);
+    Span s_mt(a->ctx, "sampled_mttkrp");
+    int64_t Tn = 0;
+    hipEvent_t seg_ev[2] = {nullptr, nullptr};
+    if (a->ctx->timing) {
+        seg_ev[0] = a->ctx->get_event();
+        seg_ev[1] = a->ctx->get_event();
+    }
+    const int64_t H = k::sampled_mttkrp(st, a->t->lk[j], a->kd.as<uint64_t>(), Ju, a->dm.as<double>(), R,
+                                        a->Y.as<double>(), Ij, a->tmp, a->ctx->timing ? &Tn : nullptr,
+                                        a->ctx->timing ? seg_ev : nullptr);
+    s_mt.end();
+    if (a->ctx->timing) {
+        if (H > 0) a->ctx->pending.push_back({"seg_reduce", {seg_ev[0], seg_ev[1]}});
+        else { a->ctx->pool.push_back(seg_ev[0]); a->ctx->pool.push_back(seg_ev[1]); }
+    }
+    a->nonzeros += (double)H;
+    if (a->ctx->timing) {
+        a->ctx->add_counter("J_u", (double)Ju);
+        a->ctx->add_counter("H", (double)H);
+        a->ctx->add_counter("T", (double)Tn);
+        a->ctx->add_counter("mode_steps", 1.0);
+    }
+    ev.mark(3, st);
+    // Region 6: U_j = Y G+, renormalise (:280-289)
+    double* Uj = f->U[j].as<double>();
+    Span s_fu(a->ctx, "factor_update");
+    k::dsymm(st, a->Gi.as<double>(), a->Y.as<double>(), Ij, R, Uj);
+    k::col_sumsq(st, Uj, Ij, R, f->sigma.as<double>(), a->part);
+    k::scale_cols(st, Uj, Ij, R, f->sigma.as<double>());
+    s_fu.end();
+    ev.mark(4, st);
+    // Region 7: sampler update (:291-294)
+    sampler_update_impl(s, j);
+    ev.mark(5, st);
+    EX_HIP(hipEventSynchronize(ev.ev[5]));
+    a->t_region[0] += ev.secs(0, 1);
+    a->t_region[2] += ev.secs(1, 2);
+    a->t_region[3] += ev.secs(2, 3);
+    a->t_region[5] += ev.secs(3, 4);
+    a->t_region[6] += ev.secs(4, 5);
+    if (a->ctx->timing) a->ctx->resolve();
+}
+
+double als_fit_impl(ex_als* a) {
+    ex_factors* f = a->f;
+    hipStream_t st = a->ctx->c.stream;
+    const int N = f->N, R = f->R;
+    a->scal.ensure(64);
+    a->G.ensure((size_t)N * R * R * 8 > (size_t)R * R * 8 ? (size_t)N * R * R * 8 : (size_t)R * R * 8);
+    DevBuf grams;
+    grams.ensure((size_t)N * R * R * 8);
+    for (int k2 = 0; k2 < N; k2++)
+        k::gram(st, f->U[k2].as<double>(), f->dims[k2], R, grams.as<double>() + (int64_t)k2 * R * R, a->part);
+    double* d = a->scal.as<double>();
+    k::chain_had_norm(st, grams.as<double>(), N, R, f->sigma.as<double>(), d);
+    DevBuf fpart;
+    k::fit_bmb(st, a->t->lk[0], N, f->ptrs, f->sigma.as<double>(), R, d + 1, fpart);
+    double h[2];
+    d2h(st, h, d, sizeof(h));
+    return 1.0 - std::sqrt(h[0] + h[1]) / a->t->norm;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ex_last_error(void) { return g_last_error.c_str(); }
+
+int ex_ctx_create(int device, ex_ctx** out) {
+    return guarded([&] {
+        EX_CHECK(out, EX_ERR_ARG, "null output pointer");
+        int n = 0;
+        EX_HIP(hipGetDeviceCount(&n));
+        EX_CHECK(device >= 0 && device < n, EX_ERR_ARG, "device index out of range");
+        auto c = std::make_unique<ex_ctx>();
+        c->c.device = device;
+        EX_HIP(hipSetDevice(device));
+        EX_HIP(hipGetDeviceProperties(&c->c.prop, device));
+        EX_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        *out = c.release();
+    });
+}
+
+int ex_ctx_destroy(ex_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipStreamSynchronize(ctx->c.stream);
+        (void)hipStreamDestroy(ctx->c.stream);
+        delete ctx;
+    });
+}
+
+int ex_ctx_sync(ex_ctx* ctx) {
+    return guarded([&] {
+        EX_CHECK(ctx, EX_ERR_ARG, "null context");
+        EX_HIP(hipStreamSynchronize(ctx->c.stream));
+    });
+}
+
+int ex_ctx_timing(ex_ctx* ctx, int enable) {
+    return guarded([&] {
+        EX_CHECK(ctx, EX_ERR_ARG, "null context");
+        EX_HIP(hipStreamSynchronize(ctx->c.stream));
+        ctx->resolve();
+        ctx->timing = enable != 0;
+        ctx->totals.clear();
+        ctx->counters.clear();
+    });
+}
+
+int ex_ctx_timing_json(ex_ctx* ctx, char* buf, size_t* len) {
+    return guarded([&] {
+        EX_CHECK(ctx && len, EX_ERR_ARG, "null argument");
+        EX_HIP(hipStreamSynchronize(ctx->c.stream));
+        ctx->resolve();
+        std::string js = "{\"spans\": {";
+        for (size_t i = 0; i < ctx->totals.size(); i++) {
+            if (i) js += ", ";
+            js += "\"" + ctx->totals[i].first + "\": {\"ms\": " + json_num(ctx->totals[i].second.first) +
+                  ", \"count\": " + std::to_string(ctx->totals[i].second.second) + "}";
+        }
+        js += "}, \"counters\": {";
+        for (size_t i = 0; i < ctx->counters.size(); i++) {
+            if (i) js += ", ";
+            js += "\"" + ctx->counters[i].first + "\": " + json_num(ctx->counters[i].second);
+        }
+        js += "}}";
+        size_t need = js.size() + 1;
+        if (buf && *len > 0) {
+            size_t n = *len < need ? *len - 1 : need - 1;
+            std::memcpy(buf, js.data(), n);
+            buf[n] = 0;
+        }
+        *len = need;
+    });
+}
+
+int ex_ctx_device_name(ex_ctx* ctx, char* buf, size_t len) {
+    return guarded([&] {
+        EX_CHECK(ctx && buf && len > 0, EX_ERR_ARG, "bad arguments");
+        std::strncpy(buf, device_label(ctx).c_str(), len - 1);
+        buf[len - 1] = 0;
+    });
+}
+
+// ---------------------------------------------------------------- tensor
+int ex_tensor_create(ex_ctx* ctx, int N, const int64_t* dims, int64_t nnz, const uint32_t* idx, const double* vals,
+                     int preprocessing, ex_tensor** out) {
+    return guarded([&] {
+        EX_CHECK(ctx && dims && out, EX_ERR_ARG, "null argument");
+        EX_CHECK(N >= 2 && N <= kMaxModes, EX_ERR_UNSUPPORTED, "mode count must be in [2, 8]");
+        EX_CHECK(nnz >= 0 && nnz < ((int64_t)1 << 32), EX_ERR_UNSUPPORTED, "nnz must be below 2^32");
+        EX_CHECK(nnz == 0 || (idx && vals), EX_ERR_ARG, "null index or value array");
+        EX_CHECK(preprocessing >= 0 && preprocessing <= 2, EX_ERR_ARG, "unknown preprocessing");
+        for (int k2 = 0; k2 < N; k2++)
+            EX_CHECK(dims[k2] >= 1 && dims[k2] <= 0xFFFFFFFFLL, EX_ERR_ARG, "mode size out of range");
+        for (int64_t i = 0; i < nnz; i++)
+            for (int k2 = 0; k2 < N; k2++)
+                EX_CHECK(idx[i * N + k2] < (uint64_t)dims[k2], EX_ERR_ARG,
+                         "coordinate out of range at nonzero " + std::to_string(i) + ", mode " + std::to_string(k2));
+        auto t = std::make_unique<ex_tensor>();
+        t->ctx = ctx;
+        t->N = N;
+        for (int k2 = 0; k2 < N; k2++) t->dims[k2] = dims[k2];
+        t->nnz = nnz;
+        hipStream_t st = ctx->c.stream;
+        t->coords.ensure((size_t)nnz * N * 4);
+        t->vals.ensure((size_t)nnz * 8);
+        h2d(st, t->coords.p, idx, (size_t)nnz * N * 4);
+        h2d(st, t->vals.p, vals, (size_t)nnz * 8);
+        k::preprocess(st, t->vals.as<double>(), nnz, preprocessing);
+        DevBuf part, out1;
+        out1.ensure(8);
+        k::sumsq(st, t->vals.as<double>(), nnz, out1.as<double>(), part);
+        double ss = 0;
+        d2h(st, &ss, out1.p, 8);
+        t->norm = std::sqrt(ss);
+        *out = t.release();
+    });
+}
+
+int ex_tensor_destroy(ex_tensor* t) {
+    return guarded([&] { delete t; });
+}
+
+int ex_tensor_norm(ex_tensor* t, double* norm) {
+    return guarded([&] {
+        EX_CHECK(t && norm, EX_ERR_ARG, "null argument");
+        *norm = t->norm;
+    });
+}
+
+int ex_tensor_build_lookups(ex_tensor* t) {
+    return guarded([&] {
+        EX_CHECK(t, EX_ERR_ARG, "null tensor");
+        tensor_build_lookups(t);
+    });
+}
+
+int ex_sampled_mttkrp(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samples, const double* design, int64_t R,
+                      double* Y, int64_t* hits) {
+    return guarded([&] {
+        EX_CHECK(t && Y, EX_ERR_ARG, "null argument");
+        EX_CHECK(mode >= 0 && mode < t->N, EX_ERR_ARG, "mode out of range");
+        EX_CHECK(R >= 1 && R <= kMaxRank, EX_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+        EX_CHECK(Ju >= 0 && (Ju == 0 || (samples && design)), EX_ERR_ARG, "bad sample arrays");
+        tensor_build_lookups(t);
+        hipStream_t st = t->ctx->c.stream;
+        const int N = t->N;
+        for (int64_t c = 0; c < Ju; c++)
+            for (int k2 = 0; k2 < N; k2++)
+                EX_CHECK(k2 == mode || samples[c * N + k2] < (uint64_t)t->dims[k2], EX_ERR_ARG, "sample index out of range");
+        DevBuf ds, dd, dk, dy;
+        DevBuf tmp[4];
+        ds.ensure((size_t)(Ju > 0 ? Ju : 1) * N * 4);
+        dd.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
+        dk.ensure((size_t)(Ju > 0 ? Ju : 1) * 8);
+        const int64_t Ij = t->dims[mode];
+        dy.ensure((size_t)Ij * R * 8);
+        h2d(st, ds.p, samples, (size_t)Ju * N * 4);
+        h2d(st, dd.p, design, (size_t)Ju * R * 8);
+        k::keys_from_coords(st, ds.as<uint32_t>(), Ju, t->lk[mode].kl, dk.as<uint64_t>(), nullptr);
+        int64_t H = k::sampled_mttkrp(st, t->lk[mode], dk.as<uint64_t>(), Ju, dd.as<double>(), (int)R, dy.as<double>(),
+                                      Ij, tmp);
+        d2h(st, Y, dy.p, (size_t)Ij * R * 8);
+        if (hits) *hits = H;
+    });
+}
+
+// ---------------------------------------------------------------- factors
+int ex_factors_create(ex_ctx* ctx, int N, const int64_t* dims, int64_t R, ex_factors** out) {
+    return guarded([&] {
+        EX_CHECK(ctx && dims && out, EX_ERR_ARG, "null argument");
+        EX_CHECK(N >= 2 && N <= kMaxModes, EX_ERR_UNSUPPORTED, "mode count must be in [2, 8]");
+        EX_CHECK(R >= 1 && R <= kMaxRank, EX_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+        auto f = std::make_unique<ex_factors>();
+        f->ctx = ctx;
+        f->N = N;
+        f->R = (int)R;
+        hipStream_t st = ctx->c.stream;
+        for (int k2 = 0; k2 < N; k2++) {
+            EX_CHECK(dims[k2] >= 1, EX_ERR_ARG, "mode size must be positive");
+            f->dims[k2] = dims[k2];
+            f->U[k2].ensure((size_t)dims[k2] * R * 8);
+            EX_HIP(hipMemsetAsync(f->U[k2].p, 0, (size_t)dims[k2] * R * 8, st));
+            f->ptrs[k2] = f->U[k2].as<double>();
+        }
+        f->sigma.ensure((size_t)R * 8);
+        std::vector<double> ones(R, 1.0);
+        h2d(st, f->sigma.p, ones.data(), (size_t)R * 8);
+        EX_HIP(hipStreamSynchronize(st));
+        *out = f.release();
+    });
+}
+
+int ex_factors_destroy(ex_factors* f) {
+    return guarded([&] { delete f; });
+}
+
+int ex_factors_set(ex_factors* f, int mode, const double* host) {
+    return guarded([&] {
+        EX_CHECK(f && host, EX_ERR_ARG, "null argument");
+        EX_CHECK(mode >= 0 && mode < f->N, EX_ERR_ARG, "mode out of range");
+        h2d(f->ctx->c.stream, f->U[mode].p, host, (size_t)f->dims[mode] * f->R * 8);
+        EX_HIP(hipStreamSynchronize(f->ctx->c.stream));
+    });
+}
+
+int ex_factors_get(ex_factors* f, int mode, double* host) {
+    return guarded([&] {
+        EX_CHECK(f && host, EX_ERR_ARG, "null argument");
+        EX_CHECK(mode >= 0 && mode < f->N, EX_ERR_ARG, "mode out of range");
+        d2h(f->ctx->c.stream, host, f->U[mode].p, (size_t)f->dims[mode] * f->R * 8);
+    });
+}
+
+int ex_factors_get_sigma(ex_factors* f, double* sigma) {
+    return guarded([&] {
+        EX_CHECK(f && sigma, EX_ERR_ARG, "null argument");
+        d2h(f->ctx->c.stream, sigma, f->sigma.p, (size_t)f->R * 8);
+    });
+}
+
+int ex_factors_set_sigma(ex_factors* f, const double* sigma) {
+    return guarded([&] {
+        EX_CHECK(f && sigma, EX_ERR_ARG, "null argument");
+        h2d(f->ctx->c.stream, f->sigma.p, sigma, (size_t)f->R * 8);
+        EX_HIP(hipStreamSynchronize(f->ctx->c.stream));
+    });
+}
+
+int ex_factors_renormalize(ex_factors* f, int mode, double* sigma) {
+    return guarded([&] {
+        EX_CHECK(f, EX_ERR_ARG, "null argument");
+        EX_CHECK(mode >= 0 && mode < f->N, EX_ERR_ARG, "mode out of range");
+        hipStream_t st = f->ctx->c.stream;
+        DevBuf sg;
+        sg.ensure((size_t)f->R * 8);
+        k::col_sumsq(st, f->U[mode].as<double>(), f->dims[mode], f->R, sg.as<double>(), f->part);
+        k::scale_cols(st, f->U[mode].as<double>(), f->dims[mode], f->R, sg.as<double>());
+        if (sigma) d2h(st, sigma, sg.p, (size_t)f->R * 8);
+        EX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int ex_factors_init_deterministic(ex_factors* f) {
+    return guarded([&] {
+        EX_CHECK(f, EX_ERR_ARG, "null argument");
+        hipStream_t st = f->ctx->c.stream;
+        DevBuf sg;
+        sg.ensure((size_t)f->R * 8);
+        for (int k2 = 0; k2 < f->N; k2++) {
+            k::cos_init(st, f->U[k2].as<double>(), f->dims[k2], f->R);
+            k::col_sumsq(st, f->U[k2].as<double>(), f->dims[k2], f->R, sg.as<double>(), f->part);
+            k::scale_cols(st, f->U[k2].as<double>(), f->dims[k2], f->R, sg.as<double>());
+        }
+        std::vector<double> ones(f->R, 1.0);
+        h2d(st, f->sigma.p, ones.data(), (size_t)f->R * 8);
+        EX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int ex_factors_gram(ex_factors* f, int mode, double* G) {
+    return guarded([&] {
+        EX_CHECK(f && G, EX_ERR_ARG, "null argument");
+        EX_CHECK(mode >= 0 && mode < f->N, EX_ERR_ARG, "mode out of range");
+        hipStream_t st = f->ctx->c.stream;
+        f->gram.ensure((size_t)f->R * f->R * 8);
+        k::gram(st, f->U[mode].as<double>(), f->dims[mode], f->R, f->gram.as<double>(), f->part);
+        d2h(st, G, f->gram.p, (size_t)f->R * f->R * 8);
+    });
+}
+
+// ---------------------------------------------------------------- samplers
+int ex_sampler_create(ex_ctx* ctx, ex_factors* f, int kind, int precision, uint64_t seed, ex_sampler** out) {
+    return guarded([&] {
+        EX_CHECK(ctx && f && out, EX_ERR_ARG, "null argument");
+        EX_CHECK(kind == EX_SAMPLER_CP_ARLS_LEV || kind == EX_SAMPLER_STS_CP, EX_ERR_ARG, "unknown sampler kind");
+        EX_CHECK(precision == EX_FP64, EX_ERR_UNSUPPORTED, "only the FP64 sampler is available");
+        auto s = std::make_unique<ex_sampler>();
+        s->ctx = ctx;
+        s->f = f;
+        s->kind = kind;
+        s->precision = precision;
+        s->seed = seed;
+        const int N = f->N, R = f->R;
+        s->sums.ensure((size_t)N * 8);
+        s->part.ensure(4096);
+        if (kind == EX_SAMPLER_STS_CP) {
+            const int S = tri_stride(R);
+            s->ge.init(R, 1);
+            for (int k2 = 0; k2 < N; k2++) {
+                s->g[k2].init(f->dims[k2], R);
+                s->nodes[k2].ensure((size_t)s->g[k2].node_count * S * 8);
+            }
+            s->grams.ensure((size_t)N * R * R * 8);
+            s->M.ensure((size_t)(N + 2) * R * R * 8);
+            s->lam.ensure((size_t)(N + 1) * R * 8);
+            s->eig.ensure((size_t)N * s->ge.node_count * S * 8);
+            s->pinvp.ensure((size_t)S * 8);
+        }
+        for (int k2 = 0; k2 < N; k2++) sampler_update_impl(s.get(), k2);
+        EX_HIP(hipStreamSynchronize(ctx->c.stream));
+        *out = s.release();
+    });
+}
+
+int ex_sampler_destroy(ex_sampler* s) {
+    return guarded([&] { delete s; });
+}
+
+int ex_sampler_update(ex_sampler* s, int mode) {
+    return guarded([&] {
+        EX_CHECK(s, EX_ERR_ARG, "null sampler");
+        EX_CHECK(mode >= 0 && mode < s->f->N, EX_ERR_ARG, "mode out of range");
+        sampler_update_impl(s, mode);
+        EX_HIP(hipStreamSynchronize(s->ctx->c.stream));
+    });
+}
+
+int ex_sampler_draw(ex_sampler* s, int64_t J, int mode, uint64_t sweep, const double* draws_row,
+                    const double* draws_eig, uint32_t* samples, double* weights) {
+    return guarded([&] {
+        EX_CHECK(s && samples && weights, EX_ERR_ARG, "null argument");
+        EX_CHECK(mode >= 0 && mode < s->f->N, EX_ERR_ARG, "mode out of range");
+        EX_CHECK(J > 0 && J < (int64_t)1 << 31, EX_ERR_ARG, "sample count J must be in [1, 2^31)");
+        const int N = s->f->N;
+        hipStream_t st = s->ctx->c.stream;
+        const double* dr = nullptr;
+        const double* de = nullptr;
+        if (draws_row) {
+            s->draws_row.ensure((size_t)N * J * 8);
+            h2d(st, s->draws_row.p, draws_row, (size_t)N * J * 8);
+            dr = s->draws_row.as<double>();
+        }
+        if (draws_eig) {
+            s->draws_eig.ensure((size_t)N * J * 8);
+            h2d(st, s->draws_eig.p, draws_eig, (size_t)N * J * 8);
+            de = s->draws_eig.as<double>();
+        }
+        EX_CHECK(s->kind == EX_SAMPLER_CP_ARLS_LEV || ((dr == nullptr) == (de == nullptr)), EX_ERR_ARG,
+                 "STS-CP needs both draw arrays or neither");
+        s->samples.ensure((size_t)J * N * 4);
+        s->weights.ensure((size_t)J * 8);
+        sampler_draw_impl(s, J, mode, sweep, dr, de, s->samples.as<uint32_t>(), s->weights.as<double>());
+        EX_HIP(hipMemcpyAsync(samples, s->samples.p, (size_t)J * N * 4, hipMemcpyDeviceToHost, st));
+        d2h(st, weights, s->weights.p, (size_t)J * 8);
+    });
+}
+
+int ex_sts_compute_m(ex_sampler* s, int mode, double* M) {
+    return guarded([&] {
+        EX_CHECK(s && M, EX_ERR_ARG, "null argument");
+        EX_CHECK(s->kind == EX_SAMPLER_STS_CP, EX_ERR_STATE, "not an STS-CP sampler");
+        EX_CHECK(mode >= 0 && mode < s->f->N, EX_ERR_ARG, "mode out of range");
+        const int N = s->f->N, R = s->f->R;
+        hipStream_t st = s->ctx->c.stream;
+        k::sts_compute_m(st, N, mode, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
+        d2h(st, M, s->M.p, (size_t)(N + 2) * R * R * 8);
+    });
+}
+
+int ex_arls_scores(ex_sampler* s, int mode, double* scores) {
+    return guarded([&] {
+        EX_CHECK(s && scores, EX_ERR_ARG, "null argument");
+        EX_CHECK(s->kind == EX_SAMPLER_CP_ARLS_LEV, EX_ERR_STATE, "not a CP-ARLS-LEV sampler");
+        EX_CHECK(mode >= 0 && mode < s->f->N, EX_ERR_ARG, "mode out of range");
+        d2h(s->ctx->c.stream, scores, s->scores[mode].p, (size_t)s->f->dims[mode] * 8);
+    });
+}
+
+// ---------------------------------------------------------------- ALS
+int ex_als_create(ex_ctx* ctx, ex_tensor* t, ex_factors* f, ex_sampler* s, int distribution, int precision,
+                  ex_als** out) {
+    return guarded([&] {
+        EX_CHECK(ctx && t && f && s && out, EX_ERR_ARG, "null argument");
+        EX_CHECK(t->N == f->N, EX_ERR_ARG, "tensor and factor mode counts differ");
+        for (int k2 = 0; k2 < t->N; k2++) EX_CHECK(t->dims[k2] == f->dims[k2], EX_ERR_ARG, "mode sizes differ");
+        EX_CHECK(s->f == f, EX_ERR_ARG, "sampler was built on another LowRankTensor");
+        EX_CHECK(distribution == EX_DIST_ACCUMULATOR_STATIONARY || distribution == EX_DIST_TENSOR_STATIONARY,
+                 EX_ERR_ARG, "unknown data distribution");
+        EX_CHECK(precision == s->precision, EX_ERR_ARG, "optimizer and sampler precisions differ");
+        auto a = std::make_unique<ex_als>();
+        a->ctx = ctx;
+        a->t = t;
+        a->f = f;
+        a->s = s;
+        a->dist = distribution;
+        a->precision = precision;
+        *out = a.release();
+    });
+}
+
+int ex_als_destroy(ex_als* a) {
+    return guarded([&] { delete a; });
+}
+
+int ex_als_init_ground_truth(ex_als* a) {
+    return guarded([&] {
+        EX_CHECK(a, EX_ERR_ARG, "null optimizer");
+        tensor_build_lookups(a->t);
+        a->initialized = true;
+    });
+}
+
+int ex_als_step(ex_als* a, int mode, int64_t J, uint64_t sweep) {
+    return guarded([&] {
+        EX_CHECK(a, EX_ERR_ARG, "null optimizer");
+        EX_CHECK(a->initialized, EX_ERR_STATE, "initialize_ground_truth_for_als() has not been called");
+        EX_CHECK(mode >= 0 && mode < a->f->N, EX_ERR_ARG, "mode out of range");
+        als_step_impl(a, mode, J, sweep);
+    });
+}
+
+int ex_als_fit(ex_als* a, double* fit) {
+    return guarded([&] {
+        EX_CHECK(a && fit, EX_ERR_ARG, "null argument");
+        EX_CHECK(a->initialized, EX_ERR_STATE, "initialize_ground_truth_for_als() has not been called");
+        *fit = als_fit_impl(a);
+    });
+}
+
+int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
+    return guarded([&] {
+        EX_CHECK(a, EX_ERR_ARG, "null optimizer");
+        EX_CHECK(a->initialized, EX_ERR_STATE, "initialize_ground_truth_for_als() has not been called");
+        EX_CHECK(rounds >= 0, EX_ERR_ARG, "negative round count");
+        EX_CHECK(epoch >= 1, EX_ERR_ARG, "epoch_interval must be positive");
+        hipStream_t st = a->ctx->c.stream;
+        for (double& x : a->t_region) x = 0.0;
+        a->nonzeros = 0.0;
+        double als_total = 0.0, fit_total = 0.0;
+        std::vector<int64_t> rl{0};
+        std::vector<double> fits{0.0}, als_times{0.0}, fit_times{0.0};
+        for (int64_t round = 1; round <= rounds; round++) {
+            EX_HIP(hipStreamSynchronize(st));
+            auto t0 = std::chrono::steady_clock::now();
+            for (int j = 0; j < a->f->N; j++) als_step_impl(a, j, J, (uint64_t)round);
+            EX_HIP(hipStreamSynchronize(st));
+            als_total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (round % epoch == 0) {
+                auto t1 = std::chrono::steady_clock::now();
+                double fit = als_fit_impl(a);
+                fit_total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+                rl.push_back(round);
+                fits.push_back(fit);
+                als_times.push_back(als_total);
+                fit_times.push_back(fit_total);
+            }
+        }
+        std::string js = "{";
+        js += "\"als_total_time\": " + json_num(als_total);
+        js += ", \"als_times\": " + json_list(als_times);
+        js += ", \"avg_nonzeros_iterated\": " + dstat(a->nonzeros);
+        js += ", \"dense_reduce_time\": " + dstat(a->t_region[4]);
+        js += ", \"design_matrix_prep_time\": " + dstat(a->t_region[2]);
+        js += ", \"fit_computation_time\": " + json_num(fit_total);
+        js += ", \"fit_computation_times\": " + json_list(fit_times);
+        js += ", \"fits\": " + json_list(fits);
+        js += ", \"leverage_sampling_time\": " + dstat(a->t_region[0]);
+        js += ", \"num_rounds\": " + std::to_string(rounds);
+        js += ", \"postprocessing_time\": " + dstat(a->t_region[5]);
+        js += ", \"precision\": \"" + std::string(a->precision == EX_FP64 ? "fp64" : "mixed") + "\"";
+        js += ", \"row_gather_time\": " + dstat(a->t_region[1]);
+        js += ", \"rounds\": " + json_list(rl);
+        js += ", \"sampler\": \"" + std::string(a->s->kind == EX_SAMPLER_STS_CP ? "sts_cp" : "cp_arls_lev") + "\"";
+        js += ", \"sampler_update_time\": " + dstat(a->t_region[6]);
+        js += ", \"spmm_time\": " + dstat(a->t_region[3]);
+        js += ", \"sum_nonzeros_iterated\": " + json_num(a->nonzeros);
+        js += ", \"device\": \"" + device_label(a->ctx) + "\"";
+        js += "}";
+        a->stats = js;
+    });
+}
+
+int ex_als_stats_json(ex_als* a, char* buf, size_t* len) {
+    return guarded([&] {
+        EX_CHECK(a && len, EX_ERR_ARG, "null argument");
+        size_t need = a->stats.size() + 1;
+        if (buf && *len > 0) {
+            size_t n = *len < need ? *len - 1 : need - 1;
+            std::memcpy(buf, a->stats.data(), n);
+            buf[n] = 0;
+        }
+        *len = need;
+    });
+}
+
+// ---------------------------------------------------------------- dense hooks
+int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G) {
+    return guarded([&] {
+        EX_CHECK(ctx && G && (m == 0 || A), EX_ERR_ARG, "null argument");
+        EX_CHECK(R >= 1 && R <= kMaxRank, EX_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+        EX_CHECK(m >= 0, EX_ERR_ARG, "negative row count");
+        hipStream_t st = ctx->c.stream;
+        DevBuf dA, dG, part;
+        dA.ensure((size_t)(m > 0 ? m : 1) * R * 8);
+        dG.ensure((size_t)R * R * 8);
+        h2d(st, dA.p, A, (size_t)m * R * 8);
+        k::gram(st, dA.as<double>(), m, (int)R, dG.as<double>(), part);
+        d2h(st, G, dG.p, (size_t)R * R * 8);
+    });
+}
+
+int ex_eigh(ex_ctx* ctx, const double* A, int64_t R, double* lam, double* V) {
+    return guarded([&] {
+        EX_CHECK(ctx && A && lam && V, EX_ERR_ARG, "null argument");
+        EX_CHECK(R >= 1 && R <= kMaxRank, EX_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+        hipStream_t st = ctx->c.stream;
+        DevBuf dA, dl, dV;
+        dA.ensure((size_t)R * R * 8);
+        dl.ensure((size_t)R * 8);
+        dV.ensure((size_t)R * R * 8);
+        h2d(st, dA.p, A, (size_t)R * R * 8);
+        k::eigh(st, dA.as<double>(), (int)R, dl.as<double>(), dV.as<double>());
+        EX_HIP(hipMemcpyAsync(lam, dl.p, (size_t)R * 8, hipMemcpyDeviceToHost, st));
+        d2h(st, V, dV.p, (size_t)R * R * 8);
+    });
+}
+
+int ex_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol, double* out) {
+    return guarded([&] {
+        EX_CHECK(ctx && M && out, EX_ERR_ARG, "null argument");
+        EX_CHECK(R >= 1 && R <= kMaxRank, EX_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+        hipStream_t st = ctx->c.stream;
+        DevBuf dM, dO;
+        dM.ensure((size_t)R * R * 8);
+        dO.ensure((size_t)R * R * 8);
+        h2d(st, dM.p, M, (size_t)R * R * 8);
+        k::pinv(st, dM.as<double>(), (int)R, tol, dO.as<double>());
+        d2h(st, out, dO.p, (size_t)R * R * 8);
+    });
+}
+
+int ex_dedup(ex_ctx* ctx, const uint32_t* samples, const double* weights, int64_t J, int N, int mode,
+             const int64_t* dims, uint32_t* samples_dedup, double* weights_dedup, int64_t* Ju) {
+    return guarded([&] {
+        EX_CHECK(ctx && dims && Ju && samples_dedup && weights_dedup, EX_ERR_ARG, "null argument");
+        EX_CHECK(N >= 2 && N <= kMaxModes && mode >= 0 && mode < N, EX_ERR_ARG, "bad mode arguments");
+        EX_CHECK(J >= 0 && J < (int64_t)1 << 31 && (J == 0 || (samples && weights)), EX_ERR_ARG, "bad sample arrays");
+        KeyLayout kl = make_key_layout(N, mode, dims);
+        EX_CHECK(kl.bits <= 64, EX_ERR_UNSUPPORTED, "tuple key exceeds 64 bits");
+        for (int64_t i = 0; i < J; i++)
+            for (int k2 = 0; k2 < N; k2++)
+                EX_CHECK(k2 == mode || samples[i * N + k2] < (uint64_t)dims[k2], EX_ERR_ARG, "sample index out of range");
+        hipStream_t st = ctx->c.stream;
+        DevBuf ds, dw, sd, wd, kd;
+        DevBuf tmp[4];
+        size_t JJ = (size_t)(J > 0 ? J : 1);
+        ds.ensure(JJ * N * 4);
+        dw.ensure(JJ * 8);
+        sd.ensure(JJ * N * 4);
+        wd.ensure(JJ * 8);
+        kd.ensure(JJ * 8);
+        h2d(st, ds.p, samples, (size_t)J * N * 4);
+        h2d(st, dw.p, weights, (size_t)J * 8);
+        int64_t nu = k::dedup(st, ds.as<uint32_t>(), dw.as<double>(), J, N, kl, sd.as<uint32_t>(), wd.as<double>(),
+                              kd.as<uint64_t>(), tmp);
+        EX_HIP(hipMemcpyAsync(samples_dedup, sd.p, (size_t)nu * N * 4, hipMemcpyDeviceToHost, st));
+        d2h(st, weights_dedup, wd.p, (size_t)nu * 8);
+        *Ju = nu;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,196 @@
+// Internal declarations shared by the HIP translation units of libexafac_hip.so.
+// Nothing here crosses the C-ABI (include/exafac_hip.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+#include <stdexcept>
+
+namespace exafac {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define EX_HIP(call)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (call);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            throw ::exafac::Error(2, std::string(#call) + ": " + hipGetErrorString(e_) \
+                                         + " (" __FILE__ ":" + std::to_string(__LINE__) + ")"); \
+    } while (0)
+
+#define EX_CHECK(cond, code, msg)                         \
+    do {                                                  \
+        if (!(cond)) throw ::exafac::Error(code, (msg));  \
+    } while (0)
+
+constexpr int kMaxRank = 64;
+constexpr int kMaxModes = 8;
+
+// ---------------------------------------------------------------- device memory
+// Owning device buffer (bytes), grown on demand; never shrinks.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr; bytes = 0;
+    }
+    void ensure(size_t n) {
+        if (n <= bytes) return;
+        // queued kernels may still read the old allocation: drain the device
+        // before freeing it (buffers only grow, so this is rare)
+        if (p) (void)hipDeviceSynchronize();
+        release();
+        size_t want = n < 256 ? 256 : n;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(5, "hipMalloc(" + std::to_string(want) + " bytes) failed: " + hipGetErrorString(e));
+        }
+        bytes = want;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Packed upper triangle of an R x R symmetric matrix, row-major over the
+// upper triangle: (a, b), a <= b, at a*R - a*(a-1)/2 + (b - a).
+__host__ __device__ inline int tri_size(int R) { return R * (R + 1) / 2; }
+__host__ __device__ inline int tri_index(int R, int a, int b) { return a * R - (a * (a - 1)) / 2 + (b - a); }
+// Node stride in elements: packed size rounded up to 16 elements (128 B in FP64).
+__host__ __device__ inline int tri_stride(int R) { return (tri_size(R) + 15) & ~15; }
+
+// Heap-ordered segment tree over ceil(n/F) leaves (PartitionTree geometry,
+// partition_tree.hpp:82-115).
+struct TreeGeom {
+    int64_t n = 0, F = 0;
+    uint32_t leaf_count = 0, node_count = 0, lfill_level = 0, lfill_count = 0;
+    uint32_t nodes_upto_lfill = 0, nodes_before_lfill = 0, complete_level_offset = 0;
+    void init(int64_t n_, int64_t F_) {
+        n = n_; F = F_;
+        leaf_count = (uint32_t)((n + F - 1) / F);
+        node_count = 2 * leaf_count - 1;
+        lfill_level = 0; lfill_count = 1;
+        while (lfill_count * 2 <= leaf_count) { lfill_level++; lfill_count *= 2; }
+        nodes_upto_lfill = lfill_count * 2 - 1;
+        nodes_before_lfill = lfill_count - 1;
+        complete_level_offset = nodes_before_lfill - (node_count - nodes_upto_lfill) / 2;
+    }
+    __host__ __device__ int64_t leaf_idx(int64_t c) const {
+        return c >= (int64_t)nodes_upto_lfill ? c - nodes_upto_lfill : c - complete_level_offset;
+    }
+    __host__ __device__ bool is_leaf(int64_t c) const { return 2 * c + 1 >= (int64_t)node_count; }
+};
+
+// Bit layout of the fiber key: coordinates of every mode except `skip`, most
+// significant first in ascending mode order.
+struct KeyLayout {
+    int N = 0;
+    int shift[kMaxModes];
+    int bits = 0;
+};
+KeyLayout make_key_layout(int N, int skip, const int64_t* dims);
+
+// Per-mode sorted nonzero copy (the sampled MTTKRP lookup).
+struct ModeLookup {
+    int mode = -1;
+    KeyLayout kl;
+    int64_t nfib = 0;
+    DevBuf fkey;   // u64[nfib] distinct fiber keys, ascending
+    DevBuf fptr;   // u32[nfib+1] offsets into rowj/val
+    DevBuf rowj;   // u32[nnz] target-mode coordinate
+    DevBuf val;    // f64[nnz]
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t prop;
+    DevBuf scratch[8];
+};
+
+// ---------------------------------------------------------------- kernels (launchers)
+// All launchers enqueue on `s` and never synchronise.
+namespace k {
+// dense
+void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part);
+void eigh(hipStream_t s, const double* A, int R, double* lam, double* V, int ncopies = 1);
+void pinv(hipStream_t s, const double* M, int R, double tol, double* out, int ncopies = 1);
+void dagat(hipStream_t s, const double* A, int64_t m, int R, const double* G, double* res);
+void dsymm(hipStream_t s, const double* G, const double* Y, int64_t m, int R, double* out);
+void col_sumsq(hipStream_t s, const double* U, int64_t m, int R, double* sigma, DevBuf& part);
+void scale_cols(hipStream_t s, double* U, int64_t m, int R, const double* sigma);
+void cos_init(hipStream_t s, double* U, int64_t m, int R);
+void chain_had_norm(hipStream_t s, const double* grams, int N, int R, const double* sigma, double* out);
+// preprocessing / norms
+void preprocess(hipStream_t s, double* vals, int64_t nnz, int kind);
+void sumsq(hipStream_t s, const double* v, int64_t n, double* out, DevBuf& part);
+void plain_sum(hipStream_t s, const double* v, int64_t n, double* out, DevBuf& part);
+// ARLS
+void arls_cdf(hipStream_t s, const double* scores, int64_t I, double* cdf, double* sum, DevBuf& part);
+void arls_draw(hipStream_t s, int N, int j, int64_t J, const double* const* cdfs, const double* const* scores,
+               const double* sums, const int64_t* dims, uint64_t seed, uint64_t sweep,
+               const double* draws, uint32_t* samples, double* weights);
+// STS
+void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double* nodes);
+void sts_compute_m(hipStream_t s, int N, int j, int R, const double* grams, double* M, double* lam);
+void eig_tree_build(hipStream_t s, int N, int j, int R, const double* M, const double* grams, double* eig_nodes);
+struct StsModeArgs {
+    const double* rows;    // factor U_k (I_k x R)
+    const double* nodes;   // packed row-tree nodes
+    const double* eig;     // packed eigen-tree nodes (2R-1)
+    const double* w;       // R x R scaled eigenvectors, rows = components
+    TreeGeom g;
+    TreeGeom ge;
+};
+void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* modes,
+              const double* pinv_packed, uint64_t seed, uint64_t sweep,
+              const double* draws_row, const double* draws_eig, uint32_t* samples, double* weights);
+void pack_upper(hipStream_t s, const double* M, int R, double* packed);
+// sparse
+void keys_from_coords(hipStream_t s, const uint32_t* coords, int64_t n, const KeyLayout& kl, uint64_t* keys,
+                      uint32_t* iota);
+void gather_lookup(hipStream_t s, const uint32_t* perm, const uint32_t* coords, const double* vals, int64_t nnz,
+                   int N, int mode, uint32_t* rowj, double* val);
+int64_t fiber_index(hipStream_t s, const uint64_t* sorted_keys, int64_t nnz, DevBuf& fkey, DevBuf& fptr,
+                    DevBuf& tmp);
+int64_t dedup(hipStream_t s, const uint32_t* samples, const double* weights, int64_t J, int N, const KeyLayout& kl,
+              uint32_t* sd, double* wd, uint64_t* kd, DevBuf* tmp);
+void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
+            const double* const* U, double* d_gram, double* d_mttkrp);
+int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const uint64_t* kd, int64_t Ju, const double* design,
+                       int R, double* Y, int64_t I_j, DevBuf* tmp, int64_t* T_out = nullptr,
+                       hipEvent_t* seg_events = nullptr);
+void fit_bmb(hipStream_t s, const ModeLookup& L0, int N, const double* const* U, const double* sigma, int R,
+             double* out, DevBuf& part);
+// sort / scan primitives (rocPRIM)
+size_t sort_pairs_u64_u32(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* kin, uint64_t* kout,
+                          const uint32_t* vin, uint32_t* vout, int64_t n, int bits);
+size_t sort_pairs_u32_u64(hipStream_t s, void* temp, size_t temp_bytes, const uint32_t* kin, uint32_t* kout,
+                          const uint64_t* vin, uint64_t* vout, int64_t n, int bits);
+size_t exclusive_scan_u32(hipStream_t s, void* temp, size_t temp_bytes, const uint32_t* in, uint32_t* out, int64_t n);
+size_t exclusive_scan_u64(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* in, uint64_t* out, int64_t n);
+}  // namespace k
+
+inline unsigned grid_for(int64_t work, int per_block, int64_t cap = 1 << 20) {
+    int64_t g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+}  // namespace exafac

@@ -1,0 +1,724 @@
+// Dense R x R and tall-skinny kernels (FP64): Gram, Jacobi eigensolver,
+// pseudo-inverse, DAGAT, dsymm, column renormalisation.
+//
+// Reference semantics: common.h:363-443 (compute_gram, compute_pinv_square),
+// common.h:696-746 (parallel_dsymm, compute_DAGAT), distmat.hpp:85-151.
+#include "common.hpp"
+#include "device.hpp"
+
+namespace exafac {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kGramRows = 64;      // rows staged per block iteration
+constexpr int kGramMaxBlocks = 512;
+
+// ---------------------------------------------------------------- Gram
+// Stage 1: block b accumulates rows of chunks b, b+B, ... into part[b].
+__global__ __launch_bounds__(kThreads) void gram_partial(const double* __restrict__ A, int64_t m, int R,
+                                                         double* __restrict__ part) {
+    __shared__ double tile[kGramRows * kMaxRank];
+    const int R2 = R * R;
+    double acc[kMaxRank * kMaxRank / kThreads];
+    const int nown = (R2 + kThreads - 1) / kThreads;
+#pragma unroll
+    for (int i = 0; i < kMaxRank * kMaxRank / kThreads; i++) acc[i] = 0.0;
+    const int64_t nchunks = (m + kGramRows - 1) / kGramRows;
+    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        const int64_t r0 = ch * kGramRows;
+        const int rows = (int)((m - r0) < kGramRows ? (m - r0) : kGramRows);
+        __syncthreads();
+        for (int e = threadIdx.x; e < rows * R; e += kThreads) tile[e] = A[r0 * R + e];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kMaxRank * kMaxRank / kThreads; i++) {
+            if (i >= nown) break;
+            int o = threadIdx.x + i * kThreads;
+            if (o >= R2) break;
+            int a = o / R, b = o % R;
+            if (a > b) continue;
+            double s = acc[i];
+            for (int r = 0; r < rows; r++) s += tile[r * R + a] * tile[r * R + b];
+            acc[i] = s;
+        }
+    }
+    for (int i = 0; i < nown; i++) {
+        int o = threadIdx.x + i * kThreads;
+        if (o < R2) part[(int64_t)blockIdx.x * R2 + o] = acc[i];
+    }
+}
+
+// Stage 2: G[a][b] = sum of the partials in block order; lower mirrored.
+__global__ __launch_bounds__(kThreads) void gram_final(const double* __restrict__ part, int nparts, int R,
+                                                       double* __restrict__ G) {
+    const int R2 = R * R;
+    for (int o = threadIdx.x; o < R2; o += kThreads) {
+        int a = o / R, b = o % R;
+        int ua = a < b ? a : b, ub = a < b ? b : a;
+        int src = ua * R + ub;
+        double s = 0.0;
+        for (int p = 0; p < nparts; p++) s += part[(int64_t)p * R2 + src];
+        G[o] = s;
+    }
+}
+
+// ---------------------------------------------------------------- Jacobi
+// Round-robin ("circle method") pairing: position 0 fixed, the rest rotate.
+__device__ __forceinline__ void rr_pair(int n, int round, int i, int& a, int& b) {
+    int pa = i, pb = n - 1 - i;
+    a = pa == 0 ? 0 : ((pa - 1 + round) % (n - 1)) + 1;
+    b = pb == 0 ? 0 : ((pb - 1 + round) % (n - 1)) + 1;
+}
+
+struct JacobiShared {
+    double c[kMaxRank / 2], s[kMaxRank / 2], t[kMaxRank / 2];
+    double apq[kMaxRank / 2], app[kMaxRank / 2], aqq[kMaxRank / 2];
+    int p[kMaxRank / 2], q[kMaxRank / 2], rot[kMaxRank / 2];
+    int rank[kMaxRank];
+    int rotated;
+};
+
+__device__ __forceinline__ double symA(const double* A, int R, int a, int b) {
+    return a <= b ? A[tri_index(R, a, b)] : A[tri_index(R, b, a)];
+}
+
+// Jacobi runs on one workgroup of kJacobiThreads threads.
+constexpr int kJacobiThreads = 256;
+
+// New value of element (r, c), r < c, r in pair X, c in pair Y (X != Y):
+// rotation X applied to the rows, then rotation Y to the columns.  v[i][k] =
+// A[{p_X, q_X}[i]][{p_Y, q_Y}[k]]; ir / ic = position of r / c in its pair.
+__device__ __forceinline__ double rot_elem(const JacobiShared& sh, int X, int Y, int ir, int ic, double v00,
+                                           double v01, double v10, double v11) {
+#pragma clang fp contract(off)
+    double nr0, nr1;
+    if (sh.rot[X]) {
+        const double c = sh.c[X], s = sh.s[X];
+        if (ir == 0) { nr0 = c * v00 - s * v10; nr1 = c * v01 - s * v11; }
+        else { nr0 = s * v00 + c * v10; nr1 = s * v01 + c * v11; }
+    } else {
+        nr0 = ir == 0 ? v00 : v10;
+        nr1 = ir == 0 ? v01 : v11;
+    }
+    if (!sh.rot[Y]) return ic == 0 ? nr0 : nr1;
+    const double c = sh.c[Y], s = sh.s[Y];
+    return ic == 0 ? c * nr0 - s * nr1 : s * nr0 + c * nr1;
+}
+
+// Cyclic Jacobi on the packed upper triangle A (LDS) with eigenvectors V
+// (LDS, R x R row-major).  The same arithmetic as the oracle's orc_eigh:
+// rotation parameters from the pre-round matrix, each upper element rotated
+// rows-then-columns, the exact 2x2 diagonal, upper triangle authoritative.
+// Each thread owns NB fixed 2x2 blocks of pair indices (P <= Q) and NV fixed
+// (row, pair) items of V; the round-robin pair schedule is tabulated in LDS
+// once.  Afterwards sh.rank[v] is the ascending position of A[v][v] (ties by
+// index).
+template <int NB, int NV>
+__device__ void jacobi_block(double* A, double* V, int R, JacobiShared& sh, uint16_t* sched) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x;
+    const int n = R, ne = n + (n & 1), npair = ne / 2;
+    const int nblocks = npair * (npair + 1) / 2;
+    for (int e = tid; e < (ne - 1) * npair; e += kJacobiThreads) {
+        int a, b;
+        rr_pair(ne, e / npair, e % npair, a, b);
+        sched[e] = (uint16_t)((a < b ? a : b) | ((a < b ? b : a) << 8));
+    }
+    int bP[NB], bQ[NB];
+#pragma unroll
+    for (int t = 0; t < NB; t++) {
+        int bi = tid + kJacobiThreads * t, P = 0;
+        bP[t] = -1;
+        bQ[t] = -1;
+        if (bi < nblocks) {
+            while (bi >= npair - P) { bi -= npair - P; P++; }
+            bP[t] = P;
+            bQ[t] = P + bi;
+        }
+    }
+    int vk[NV], vi[NV];
+#pragma unroll
+    for (int u = 0; u < NV; u++) {
+        int it = tid + kJacobiThreads * u;
+        vk[u] = it < n * npair ? it / npair : -1;
+        vi[u] = it < n * npair ? it % npair : 0;
+    }
+    for (int e = tid; e < n * n; e += kJacobiThreads) V[e] = (e / n == e % n) ? 1.0 : 0.0;
+    __syncthreads();
+    for (int sweep = 0; sweep < 40; sweep++) {
+        if (tid == 0) sh.rotated = 0;
+        __syncthreads();
+        for (int round = 0; round < ne - 1; round++) {
+            if (tid < npair) {
+                const uint16_t pq = sched[round * npair + tid];
+                const int p = pq & 0xFF, q = pq >> 8;
+                sh.p[tid] = p;
+                sh.q[tid] = q;
+                int r = 0;
+                if (q < n) {
+                    double apq = A[tri_index(R, p, q)], app = A[tri_index(R, p, p)], aqq = A[tri_index(R, q, q)];
+                    double thr = 2.220446049250313e-16 * sqrt(fabs(app) * fabs(aqq));
+                    if (apq != 0.0 && fabs(apq) > thr) {
+                        double tau = (aqq - app) / (2.0 * apq);
+                        double rt = sqrt(1.0 + tau * tau);
+                        double t = tau >= 0.0 ? 1.0 / (tau + rt) : -1.0 / (-tau + rt);
+                        double c = 1.0 / sqrt(1.0 + t * t);
+                        sh.c[tid] = c;
+                        sh.s[tid] = t * c;
+                        sh.t[tid] = t;
+                        sh.apq[tid] = apq;
+                        sh.app[tid] = app;
+                        sh.aqq[tid] = aqq;
+                        r = 1;
+                        sh.rotated = 1;
+                    }
+                }
+                sh.rot[tid] = r;
+            }
+            __syncthreads();
+            double nv[NB][4];
+            int ni[NB][4];
+#pragma unroll
+            for (int t = 0; t < NB; t++) {
+#pragma unroll
+                for (int u = 0; u < 4; u++) ni[t][u] = -1;
+                const int P = bP[t], Q = bQ[t];
+                if (P < 0) continue;
+                const int rP = sh.rot[P], rQ = sh.rot[Q];
+                if (!rP && !rQ) continue;
+                const int p1 = sh.p[P], q1 = sh.q[P];
+                if (P == Q) {
+                    const double tt = sh.t[P], apq = sh.apq[P];
+                    ni[t][0] = tri_index(R, p1, p1);
+                    nv[t][0] = sh.app[P] - tt * apq;
+                    ni[t][1] = tri_index(R, q1, q1);
+                    nv[t][1] = sh.aqq[P] + tt * apq;
+                    ni[t][2] = tri_index(R, p1, q1);
+                    nv[t][2] = 0.0;
+                    continue;
+                }
+                const int p2 = sh.p[Q], q2 = sh.q[Q];
+                const bool vq1 = q1 < n, vq2 = q2 < n;
+                const double v00 = symA(A, R, p1, p2);
+                const double v01 = vq2 ? symA(A, R, p1, q2) : 0.0;
+                const double v10 = vq1 ? symA(A, R, q1, p2) : 0.0;
+                const double v11 = (vq1 && vq2) ? symA(A, R, q1, q2) : 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int ir = u >> 1, ic = u & 1;
+                    const int ra = ir ? q1 : p1, cb = ic ? q2 : p2;
+                    if (ra >= n || cb >= n) continue;
+                    double val;
+                    if (ra < cb) val = rot_elem(sh, P, Q, ir, ic, v00, v01, v10, v11);
+                    else val = rot_elem(sh, Q, P, ic, ir, v00, v10, v01, v11);
+                    ni[t][u] = ra < cb ? tri_index(R, ra, cb) : tri_index(R, cb, ra);
+                    nv[t][u] = val;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < NV; u++) {
+                const int kk = vk[u], i = vi[u];
+                if (kk < 0 || !sh.rot[i]) continue;
+                const int p = sh.p[i], q = sh.q[i];
+                const double c = sh.c[i], s = sh.s[i];
+                const double vp = V[kk * n + p], vq = V[kk * n + q];
+                V[kk * n + p] = c * vp - s * vq;
+                V[kk * n + q] = s * vp + c * vq;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < NB; t++)
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (ni[t][u] >= 0) A[ni[t][u]] = nv[t][u];
+            __syncthreads();
+        }
+        const int r = sh.rotated;
+        __syncthreads();
+        if (!r) break;
+    }
+    if (tid < n) {
+        const double lv = A[tri_index(R, tid, tid)];
+        int rk = 0;
+        for (int u = 0; u < n; u++) {
+            const double lu = A[tri_index(R, u, u)];
+            if (lu < lv || (lu == lv && u < tid)) rk++;
+        }
+        sh.rank[tid] = rk;
+    }
+    __syncthreads();
+}
+
+// Load an R x R matrix's upper triangle (row-major source) into packed LDS.
+__device__ void load_upper(const double* M, int R, double* A) {
+    const int P = tri_size(R);
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int a = 0;
+        while (a + 1 < R && tri_index(R, a + 1, a + 1) <= e) a++;
+        int b = a + (e - tri_index(R, a, a));
+        A[e] = M[a * R + b];
+    }
+    __syncthreads();
+}
+
+struct EighShared {
+    double A[kMaxRank * (kMaxRank + 1) / 2];
+    double V[kMaxRank * kMaxRank];
+    JacobiShared j;
+    uint16_t sched[(kMaxRank - 1) * (kMaxRank / 2)];
+};
+
+// Eigendecomposition of blockIdx.x-th matrix: lam ascending, V columns.
+template <int NB, int NV>
+__global__ __launch_bounds__(kJacobiThreads) void eigh_kernel(const double* __restrict__ Min, int R,
+                                                        double* __restrict__ lam, double* __restrict__ Vout) {
+    __shared__ EighShared sh;
+    const int64_t R2 = (int64_t)R * R;
+    load_upper(Min + blockIdx.x * R2, R, sh.A);
+    jacobi_block<NB, NV>(sh.A, sh.V, R, sh.j, sh.sched);
+    for (int o = threadIdx.x; o < R * R; o += blockDim.x) {
+        int u = o / R, v = o % R;
+        Vout[blockIdx.x * R2 + u * R + sh.j.rank[v]] = sh.V[u * R + v];
+    }
+    if (threadIdx.x < R) lam[blockIdx.x * R + sh.j.rank[threadIdx.x]] = sh.A[tri_index(R, threadIdx.x, threadIdx.x)];
+}
+
+// compute_pinv_square: W = V diag(1/sqrt(lam)) for lam > tol else 0 (columns in
+// ascending eigenvalue order), out = W W^T summed over ascending columns.
+__device__ void pinv_from_jacobi(EighShared& sh, int R, double tol, double* __restrict__ out) {
+#pragma clang fp contract(off)
+    // reuse V in place as W with sorted columns: build into A-sized temp? Use a
+    // column-scale factor per original column, then sum in sorted order.
+    __shared__ double scale[kMaxRank];
+    __shared__ int ord[kMaxRank];
+    if (threadIdx.x < R) {
+        double lv = sh.A[tri_index(R, threadIdx.x, threadIdx.x)];
+        scale[threadIdx.x] = lv > tol ? sqrt(lv) : 0.0;
+        ord[sh.j.rank[threadIdx.x]] = threadIdx.x;
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < R * R; o += blockDim.x) {
+        int a = o / R, b = o % R;
+        if (a > b) continue;
+        double acc = 0.0;
+        for (int cc = 0; cc < R; cc++) {
+            int v = ord[cc];
+            double sq = scale[v];
+            double wa = sq > 0.0 ? sh.V[a * R + v] / sq : 0.0;
+            double wb = sq > 0.0 ? sh.V[b * R + v] / sq : 0.0;
+            acc += wa * wb;
+        }
+        out[a * R + b] = acc;
+        out[b * R + a] = acc;
+    }
+}
+
+template <int NB, int NV>
+__global__ __launch_bounds__(kJacobiThreads) void pinv_kernel(const double* __restrict__ Min, int R, double tol,
+                                                        double* __restrict__ out) {
+    __shared__ EighShared sh;
+    const int64_t R2 = (int64_t)R * R;
+    load_upper(Min + blockIdx.x * R2, R, sh.A);
+    jacobi_block<NB, NV>(sh.A, sh.V, R, sh.j, sh.sched);
+    pinv_from_jacobi(sh, R, tol, out + blockIdx.x * R2);
+}
+
+// ---------------------------------------------------------------- STS computeM
+// sts_cp.hpp:80-188 for target mode j, one workgroup.  grams: N x R x R
+// (full symmetric), M: (N+2) x R x R, lam: (N+1) x R.
+template <int NB, int NV>
+__global__ __launch_bounds__(kJacobiThreads) void sts_compute_m_kernel(int N, int j, int R, const double* __restrict__ grams,
+                                                                 double* __restrict__ M, double* __restrict__ lam) {
+#pragma clang fp contract(off)
+    __shared__ EighShared sh;
+    __shared__ int ord[kMaxRank];
+    const int R2 = R * R;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int i = tid; i < R2; i += nt) M[N * R2 + i] = 1.0;
+    __syncthreads();
+    int last = N;
+    for (int k = N - 1; k >= 0; k--) {
+        if (k == j) continue;
+        for (int i = tid; i < R2; i += nt) M[k * R2 + i] = grams[k * R2 + i] * M[last * R2 + i];
+        __syncthreads();
+        last = k;
+    }
+    if (j == 0) {
+        for (int i = tid; i < R2; i += nt) M[i] = M[R2 + i];
+        __syncthreads();
+    }
+    for (int i = tid; i < R2; i += nt) M[(N + 1) * R2 + i] = M[i];
+    __syncthreads();
+    // eigendecomposition of the Hadamard product; slot N = V diag(lam^-1/2), columns reversed
+    load_upper(M, R, sh.A);
+    jacobi_block<NB, NV>(sh.A, sh.V, R, sh.j, sh.sched);
+    if (tid < R) ord[sh.j.rank[tid]] = tid;
+    __syncthreads();
+    if (tid < R) lam[sh.j.rank[tid]] = sh.A[tri_index(R, tid, tid)];
+    for (int o = tid; o < R2; o += nt) {
+        int u = o / R, vs = o % R;          // vs = sorted column
+        int v = ord[vs];
+        double lv = sh.A[tri_index(R, v, v)];
+        M[N * R2 + u * R + R - 1 - vs] = lv > 1e-8 ? sh.V[u * R + v] / sqrt(lv) : 0.0;
+    }
+    __syncthreads();
+    // slot 0 = W W^T (upper computed, mirrored)
+    for (int o = tid; o < R2; o += nt) {
+        int a = o / R, b = o % R;
+        if (a > b) continue;
+        double acc = 0.0;
+        for (int c = 0; c < R; c++) acc += M[N * R2 + a * R + c] * M[N * R2 + b * R + c];
+        M[a * R + b] = acc;
+        M[b * R + a] = acc;
+    }
+    __syncthreads();
+    for (int k = N - 1; k > 0; k--) {
+        if (k == j) continue;
+        for (int i = tid; i < R2; i += nt) M[k * R2 + i] *= M[i];
+    }
+    __syncthreads();
+    for (int k = N; k > 0; k--) {
+        if (k == j) continue;
+        if (k < N) {
+            load_upper(M + k * R2, R, sh.A);
+            jacobi_block<NB, NV>(sh.A, sh.V, R, sh.j, sh.sched);
+            if (tid < R) ord[sh.j.rank[tid]] = tid;
+            __syncthreads();
+            if (tid < R) lam[k * R + sh.j.rank[tid]] = sh.A[tri_index(R, tid, tid)];
+            // scaled eigenvectors, already transposed: row vs = sqrt(lam) * eigenvector
+            for (int o = tid; o < R2; o += nt) {
+                int vs = o / R, u = o % R;
+                int v = ord[vs];
+                double lv = sh.A[tri_index(R, v, v)];
+                M[k * R2 + vs * R + u] = sh.V[u * R + v] * sqrt(lv > 0.0 ? lv : 0.0);
+            }
+            __syncthreads();
+        } else {
+            // transpose slot N in place
+            for (int o = tid; o < R2; o += nt) {
+                int a = o / R, b = o % R;
+                if (a >= b) continue;
+                double x = M[N * R2 + a * R + b], y = M[N * R2 + b * R + a];
+                M[N * R2 + a * R + b] = y;
+                M[N * R2 + b * R + a] = x;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Eigen trees (PartitionTree(R, 1, R) per non-target mode k): leaves are the
+// outer products of the rows of M[offset(k)], internal nodes sum their
+// children; every node is then multiplied elementwise by G_k
+// (sts_cp.hpp:179-187, partition_tree.hpp:117-187).  One block per mode.
+__global__ __launch_bounds__(kThreads) void eig_tree_kernel(int N, int j, int R, const double* __restrict__ M,
+                                                            const double* __restrict__ grams, TreeGeom ge,
+                                                            double* __restrict__ eig_nodes) {
+    const int k = blockIdx.x;
+    if (k == j || k >= N) return;
+    const int R2 = R * R, P = tri_size(R), S = tri_stride(R);
+    const int off = (k + 1 == j) ? k + 2 : k + 1;
+    const double* W = M + (int64_t)off * R2;
+    double* T = eig_nodes + (int64_t)k * ge.node_count * S;
+    const int64_t first_leaf = (int64_t)ge.node_count - ge.leaf_count;
+    for (int64_t o = threadIdx.x; o < (int64_t)ge.leaf_count * P; o += blockDim.x) {
+        int64_t i = o / P;
+        int e = (int)(o % P);
+        int a = 0;
+        while (a + 1 < R && tri_index(R, a + 1, a + 1) <= e) a++;
+        int b = a + (e - tri_index(R, a, a));
+        int64_t c = first_leaf + i;
+        int64_t v = ge.leaf_idx(c);
+        T[c * S + e] = W[v * R + a] * W[v * R + b];
+    }
+    __syncthreads();
+    int64_t start = ge.nodes_before_lfill, end = first_leaf;
+    for (int lvl = (int)ge.lfill_level; lvl >= 0; lvl--) {
+        for (int64_t o = threadIdx.x; o < (end - start) * P; o += blockDim.x) {
+            int64_t c = start + o / P;
+            int e = (int)(o % P);
+            T[c * S + e] = 0.0 + (T[(2 * c + 1) * S + e] + T[(2 * c + 2) * S + e]);
+        }
+        __syncthreads();
+        end = start;
+        start = ((start + 1) / 2) - 1;
+    }
+    const double* G = grams + (int64_t)k * R2;
+    for (int64_t o = threadIdx.x; o < (int64_t)ge.node_count * P; o += blockDim.x) {
+        int64_t c = o / P;
+        int e = (int)(o % P);
+        int a = 0;
+        while (a + 1 < R && tri_index(R, a + 1, a + 1) <= e) a++;
+        int b = a + (e - tri_index(R, a, a));
+        T[c * S + e] *= G[a * R + b];
+    }
+}
+
+// ---------------------------------------------------------------- row-wise
+// res_i = a_i^T G a_i, G symmetric (full storage).  One thread per row.
+__global__ __launch_bounds__(kThreads) void dagat_kernel(const double* __restrict__ A, int64_t m, int R,
+                                                         const double* __restrict__ G, double* __restrict__ res) {
+    __shared__ double g[kMaxRank * kMaxRank];
+    for (int e = threadIdx.x; e < R * R; e += blockDim.x) g[e] = G[e];
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const double* a = A + i * R;
+        double r = 0.0;
+        for (int c = 0; c < R; c++) {
+            double t = 0.0;
+            for (int b = 0; b < R; b++) t += a[b] * g[b * R + c];
+            r += a[c] * t;
+        }
+        res[i] = r;
+    }
+}
+
+// out = Y G (G symmetric, full storage).  One thread per output element.
+__global__ __launch_bounds__(kThreads) void dsymm_kernel(const double* __restrict__ G, const double* __restrict__ Y,
+                                                         int64_t m, int R, double* __restrict__ out) {
+    __shared__ double g[kMaxRank * kMaxRank];
+    for (int e = threadIdx.x; e < R * R; e += blockDim.x) g[e] = G[e];
+    __syncthreads();
+    const int64_t total = m * R;
+    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
+        int64_t i = o / R;
+        int c = (int)(o % R);
+        const double* y = Y + i * R;
+        double acc = 0.0;
+        for (int b = 0; b < R; b++) acc += y[b] * g[b * R + c];
+        out[o] = acc;
+    }
+}
+
+// Column sums of squares, stage 1: part[block][c].
+__global__ __launch_bounds__(kThreads) void col_sumsq_partial(const double* __restrict__ U, int64_t m, int R,
+                                                              double* __restrict__ part) {
+    __shared__ double red[kThreads];
+    const int lanes = kThreads / R;           // rows processed in parallel
+    const int c = threadIdx.x % R, rl = threadIdx.x / R;
+    double acc = 0.0;
+    if (rl < lanes) {
+        const int64_t rows_per_block = (m + gridDim.x - 1) / gridDim.x;
+        const int64_t r0 = blockIdx.x * rows_per_block;
+        const int64_t r1 = r0 + rows_per_block < m ? r0 + rows_per_block : m;
+        for (int64_t i = r0 + rl; i < r1; i += lanes) {
+            double v = U[i * R + c];
+            acc += v * v;
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < R) {
+        double s = 0.0;
+        for (int l = 0; l < lanes; l++) s += red[l * R + threadIdx.x];
+        part[(int64_t)blockIdx.x * R + threadIdx.x] = s;
+    }
+}
+
+__global__ void col_sumsq_final(const double* __restrict__ part, int nparts, int R, double* __restrict__ sigma) {
+    int c = threadIdx.x;
+    if (c >= R) return;
+    double s = 0.0;
+    for (int p = 0; p < nparts; p++) s += part[(int64_t)p * R + c];
+    sigma[c] = sqrt(s);
+}
+
+__global__ __launch_bounds__(kThreads) void scale_cols_kernel(double* __restrict__ U, int64_t m, int R,
+                                                              const double* __restrict__ sigma) {
+    const int64_t total = m * R;
+    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x)
+        U[o] = U[o] / sigma[o % R];
+}
+
+__global__ __launch_bounds__(kThreads) void cos_init_kernel(double* __restrict__ U, int64_t m, int R) {
+    const int64_t total = m * R;
+    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x)
+        U[o] = cos((double)(uint64_t)o);
+}
+
+// sum_ab (prod_k G_k)[a][b] sigma_a sigma_b  (chain_had_prod over all modes).
+__global__ __launch_bounds__(kThreads) void chain_had_norm_kernel(const double* __restrict__ grams, int N, int R,
+                                                                  const double* __restrict__ sigma,
+                                                                  double* __restrict__ out) {
+    __shared__ double red[kThreads];
+    const int R2 = R * R;
+    double acc = 0.0;
+    for (int o = threadIdx.x; o < R2; o += blockDim.x) {
+        double p = 1.0;
+        for (int k = 0; k < N; k++) p *= grams[(int64_t)k * R2 + o];
+        acc += p * sigma[o / R] * sigma[o % R];
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int t = 0; t < kThreads; t++) s += red[t];
+        *out = s;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void preprocess_kernel(double* __restrict__ v, int64_t n, int kind) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (kind == 1) v[i] = log(v[i] + 1.0);
+        else if (kind == 2) v[i] = 1.0;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void sumsq_partial(const double* __restrict__ v, int64_t n,
+                                                          double* __restrict__ part) {
+    __shared__ double red[kThreads];
+    double acc = 0.0;
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) acc += v[i] * v[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kThreads) void plainsum_partial(const double* __restrict__ v, int64_t n,
+                                                             double* __restrict__ part) {
+    __shared__ double red[kThreads];
+    double acc = 0.0;
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) acc += v[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void sum_final(const double* __restrict__ part, int nparts, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int p = 0; p < nparts; p++) s += part[p];
+    *out = s;
+}
+
+__global__ void pack_upper_kernel(const double* __restrict__ M, int R, double* __restrict__ packed) {
+    const int P = tri_size(R);
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int a = 0;
+        while (a + 1 < R && tri_index(R, a + 1, a + 1) <= e) a++;
+        int b = a + (e - tri_index(R, a, a));
+        packed[e] = M[a * R + b];
+    }
+}
+
+}  // namespace
+
+namespace k {
+
+void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part) {
+    int nb = (int)grid_for(m, kGramRows, kGramMaxBlocks);
+    part.ensure((size_t)nb * R * R * sizeof(double));
+    hipLaunchKernelGGL(gram_partial, dim3(nb), dim3(kThreads), 0, s, A, m, R, part.as<double>());
+    hipLaunchKernelGGL(gram_final, dim3(1), dim3(kThreads), 0, s, part.as<double>(), nb, R, G);
+    EX_HIP(hipGetLastError());
+}
+
+// Jacobi work per thread: (pair blocks, V items) = (1, 1) for R <= 16,
+// (1, 2) for R <= 32, (3, 8) for R <= 64.
+int jacobi_nb(int R) { return R <= 16 ? 1 : (R <= 32 ? 2 : 3); }
+
+#define JACOBI_DISPATCH(R, KERNEL, GRID, STREAM, ...)                                                     \
+    do {                                                                                                  \
+        switch (jacobi_nb(R)) {                                                                           \
+            case 1: hipLaunchKernelGGL((KERNEL<1, 1>), GRID, dim3(kJacobiThreads), 0, STREAM, __VA_ARGS__); break; \
+            case 2: hipLaunchKernelGGL((KERNEL<1, 2>), GRID, dim3(kJacobiThreads), 0, STREAM, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL((KERNEL<3, 8>), GRID, dim3(kJacobiThreads), 0, STREAM, __VA_ARGS__); break; \
+        }                                                                                                 \
+    } while (0)
+
+void eigh(hipStream_t s, const double* A, int R, double* lam, double* V, int ncopies) {
+    JACOBI_DISPATCH(R, eigh_kernel, dim3(ncopies), s, A, R, lam, V);
+    EX_HIP(hipGetLastError());
+}
+
+void pinv(hipStream_t s, const double* M, int R, double tol, double* out, int ncopies) {
+    JACOBI_DISPATCH(R, pinv_kernel, dim3(ncopies), s, M, R, tol, out);
+    EX_HIP(hipGetLastError());
+}
+
+void dagat(hipStream_t s, const double* A, int64_t m, int R, const double* G, double* res) {
+    hipLaunchKernelGGL(dagat_kernel, dim3(grid_for(m, kThreads, 4096)), dim3(kThreads), 0, s, A, m, R, G, res);
+    EX_HIP(hipGetLastError());
+}
+
+void dsymm(hipStream_t s, const double* G, const double* Y, int64_t m, int R, double* out) {
+    hipLaunchKernelGGL(dsymm_kernel, dim3(grid_for(m * R, kThreads, 8192)), dim3(kThreads), 0, s, G, Y, m, R, out);
+    EX_HIP(hipGetLastError());
+}
+
+void col_sumsq(hipStream_t s, const double* U, int64_t m, int R, double* sigma, DevBuf& part) {
+    int nb = (int)grid_for(m, 1024, 1024);
+    part.ensure((size_t)nb * R * sizeof(double));
+    hipLaunchKernelGGL(col_sumsq_partial, dim3(nb), dim3(kThreads), 0, s, U, m, R, part.as<double>());
+    hipLaunchKernelGGL(col_sumsq_final, dim3(1), dim3(64), 0, s, part.as<double>(), nb, R, sigma);
+    EX_HIP(hipGetLastError());
+}
+
+void scale_cols(hipStream_t s, double* U, int64_t m, int R, const double* sigma) {
+    hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_for(m * R, kThreads, 8192)), dim3(kThreads), 0, s, U, m, R, sigma);
+    EX_HIP(hipGetLastError());
+}
+
+void cos_init(hipStream_t s, double* U, int64_t m, int R) {
+    hipLaunchKernelGGL(cos_init_kernel, dim3(grid_for(m * R, kThreads, 8192)), dim3(kThreads), 0, s, U, m, R);
+    EX_HIP(hipGetLastError());
+}
+
+void chain_had_norm(hipStream_t s, const double* grams, int N, int R, const double* sigma, double* out) {
+    hipLaunchKernelGGL(chain_had_norm_kernel, dim3(1), dim3(kThreads), 0, s, grams, N, R, sigma, out);
+    EX_HIP(hipGetLastError());
+}
+
+void preprocess(hipStream_t s, double* vals, int64_t nnz, int kind) {
+    if (kind == 0 || nnz == 0) return;
+    hipLaunchKernelGGL(preprocess_kernel, dim3(grid_for(nnz, kThreads, 8192)), dim3(kThreads), 0, s, vals, nnz, kind);
+    EX_HIP(hipGetLastError());
+}
+
+void sumsq(hipStream_t s, const double* v, int64_t n, double* out, DevBuf& part) {
+    int nb = (int)grid_for(n, 4096, 1024);
+    part.ensure((size_t)nb * sizeof(double));
+    hipLaunchKernelGGL(sumsq_partial, dim3(nb), dim3(kThreads), 0, s, v, n, part.as<double>());
+    hipLaunchKernelGGL(sum_final, dim3(1), dim3(64), 0, s, part.as<double>(), nb, out);
+    EX_HIP(hipGetLastError());
+}
+
+void plain_sum(hipStream_t s, const double* v, int64_t n, double* out, DevBuf& part) {
+    int nb = (int)grid_for(n, 4096, 1024);
+    part.ensure((size_t)nb * sizeof(double));
+    hipLaunchKernelGGL(plainsum_partial, dim3(nb), dim3(kThreads), 0, s, v, n, part.as<double>());
+    hipLaunchKernelGGL(sum_final, dim3(1), dim3(64), 0, s, part.as<double>(), nb, out);
+    EX_HIP(hipGetLastError());
+}
+
+void sts_compute_m(hipStream_t s, int N, int j, int R, const double* grams, double* M, double* lam) {
+    JACOBI_DISPATCH(R, sts_compute_m_kernel, dim3(1), s, N, j, R, grams, M, lam);
+    EX_HIP(hipGetLastError());
+}
+
+void eig_tree_build(hipStream_t s, int N, int j, int R, const double* M, const double* grams, double* eig_nodes) {
+    TreeGeom ge;
+    ge.init(R, 1);
+    hipLaunchKernelGGL(eig_tree_kernel, dim3(N), dim3(kThreads), 0, s, N, j, R, M, grams, ge, eig_nodes);
+    EX_HIP(hipGetLastError());
+}
+
+void pack_upper(hipStream_t s, const double* M, int R, double* packed) {
+    hipLaunchKernelGGL(pack_upper_kernel, dim3(1), dim3(kThreads), 0, s, M, R, packed);
+    EX_HIP(hipGetLastError());
+}
+
+}  // namespace k
+}  // namespace exafac
