@@ -205,6 +205,7 @@ def main():
         "nnz_per_s": round(cnt.get("H", 0) / g["elapsed"], 1),
         "fit": {"initial": g["fit0"], "final": g["fit1"]},
         "spans_ms_per_sweep": {k: round(v["ms"] / K, 4) for k, v in spans.items()},
+        "counters_per_mode_step": {k: round(v / max(cnt.get("mode_steps", 1), 1), 1) for k, v in cnt.items()},
         "roofline": roof,
         "cpu_baseline": cpu,
         "device": g["device"],

@@ -407,7 +407,8 @@ double als_fit_impl(ex_als* a) {
     double* d = a->scal.as<double>();
     k::chain_had_norm(st, grams.as<double>(), N, R, f->sigma.as<double>(), d);
     DevBuf fpart;
-    k::fit_bmb(st, a->t->lk[0], N, f->ptrs, f->sigma.as<double>(), R, d + 1, fpart);
+    k::fit_bmb(st, a->t->coords.as<uint32_t>(), a->t->vals.as<double>(), a->t->nnz, N, f->ptrs,
+               f->sigma.as<double>(), R, d + 1, fpart);
     double h[2];
     d2h(st, h, d, sizeof(h));
     return 1.0 - std::sqrt(h[0] + h[1]) / a->t->norm;

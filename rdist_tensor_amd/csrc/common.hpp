@@ -175,8 +175,8 @@ void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int
 int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const uint64_t* kd, int64_t Ju, const double* design,
                        int R, double* Y, int64_t I_j, DevBuf* tmp, int64_t* T_out = nullptr,
                        hipEvent_t* seg_events = nullptr);
-void fit_bmb(hipStream_t s, const ModeLookup& L0, int N, const double* const* U, const double* sigma, int R,
-             double* out, DevBuf& part);
+void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
+             const double* sigma, int R, double* out, DevBuf& part);
 // sort / scan primitives (rocPRIM)
 size_t sort_pairs_u64_u32(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* kin, uint64_t* kout,
                           const uint32_t* vin, uint32_t* vout, int64_t n, int bits);

@@ -64,4 +64,15 @@ __device__ __forceinline__ double bcast(double v, int src) {
     return __shfl(v, src, 64);
 }
 
+// Value of lane `l` (wavefront-uniform) broadcast through a scalar register.
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = readlane_u32((uint32_t)u, l), hi = readlane_u32((uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 }  // namespace exafac

@@ -14,15 +14,22 @@ constexpr int kGramRows = 64;      // rows staged per block iteration
 constexpr int kGramMaxBlocks = 512;
 
 // ---------------------------------------------------------------- Gram
-// Stage 1: block b accumulates rows of chunks b, b+B, ... into part[b].
+// Stage 1: block b accumulates the rows of chunks b, b+B, ... into part[b]
+// (upper triangle).  Each thread owns a 4 x 4 tile of outputs (tiles with
+// ta > tb are skipped) and one of G = 256 / ceil(R/4)^2 row groups; the row
+// groups are combined in LDS in group order.
 __global__ __launch_bounds__(kThreads) void gram_partial(const double* __restrict__ A, int64_t m, int R,
                                                          double* __restrict__ part) {
     __shared__ double tile[kGramRows * kMaxRank];
     const int R2 = R * R;
-    double acc[kMaxRank * kMaxRank / kThreads];
-    const int nown = (R2 + kThreads - 1) / kThreads;
+    const int nt = (R + 3) / 4, tpg = nt * nt, G = kThreads / tpg;
+    const int g = threadIdx.x / tpg, t = threadIdx.x % tpg, ta = t / nt, tb = t % nt;
+    const bool active = g < G && ta <= tb;
+    double acc[4][4];
 #pragma unroll
-    for (int i = 0; i < kMaxRank * kMaxRank / kThreads; i++) acc[i] = 0.0;
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[i][k] = 0.0;
     const int64_t nchunks = (m + kGramRows - 1) / kGramRows;
     for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
         const int64_t r0 = ch * kGramRows;
@@ -30,36 +37,59 @@ __global__ __launch_bounds__(kThreads) void gram_partial(const double* __restric
         __syncthreads();
         for (int e = threadIdx.x; e < rows * R; e += kThreads) tile[e] = A[r0 * R + e];
         __syncthreads();
+        if (active) {
+            for (int r = g; r < rows; r += G) {
+                double va[4], vb[4];
 #pragma unroll
-        for (int i = 0; i < kMaxRank * kMaxRank / kThreads; i++) {
-            if (i >= nown) break;
-            int o = threadIdx.x + i * kThreads;
-            if (o >= R2) break;
-            int a = o / R, b = o % R;
-            if (a > b) continue;
-            double s = acc[i];
-            for (int r = 0; r < rows; r++) s += tile[r * R + a] * tile[r * R + b];
-            acc[i] = s;
+                for (int i = 0; i < 4; i++) {
+                    const int ca = 4 * ta + i, cb = 4 * tb + i;
+                    va[i] = ca < R ? tile[r * R + ca] : 0.0;
+                    vb[i] = cb < R ? tile[r * R + cb] : 0.0;
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+#pragma unroll
+                    for (int k = 0; k < 4; k++) acc[i][k] += va[i] * vb[k];
+            }
         }
     }
-    for (int i = 0; i < nown; i++) {
-        int o = threadIdx.x + i * kThreads;
-        if (o < R2) part[(int64_t)blockIdx.x * R2 + o] = acc[i];
+    // combine the row groups: red[g][a][b]
+    __syncthreads();
+    double* red = tile;   // G * R2 <= kThreads * 16 = kGramRows * kMaxRank doubles
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int a = 4 * ta + i, b = 4 * tb + k;
+                if (a < R && b < R) red[g * R2 + a * R + b] = acc[i][k];
+            }
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < R2; o += kThreads) {
+        const int a = o / R, b = o % R;
+        if (a > b) continue;
+        double s = 0.0;
+        for (int q = 0; q < G; q++) s += red[q * R2 + o];
+        part[(int64_t)blockIdx.x * R2 + o] = s;
     }
 }
 
-// Stage 2: G[a][b] = sum of the partials in block order; lower mirrored.
+// Stage 2: G[a][b] = sum over the partials (upper triangle), one wavefront per
+// output entry: lane l sums partials l, l+64, ... in order, then a fixed
+// butterfly combines the lanes.  Lower triangle mirrored.
 __global__ __launch_bounds__(kThreads) void gram_final(const double* __restrict__ part, int nparts, int R,
                                                        double* __restrict__ G) {
+    const int lane = threadIdx.x % 64;
     const int R2 = R * R;
-    for (int o = threadIdx.x; o < R2; o += kThreads) {
-        int a = o / R, b = o % R;
-        int ua = a < b ? a : b, ub = a < b ? b : a;
-        int src = ua * R + ub;
-        double s = 0.0;
-        for (int p = 0; p < nparts; p++) s += part[(int64_t)p * R2 + src];
-        G[o] = s;
-    }
+    const int o = blockIdx.x * (kThreads / 64) + threadIdx.x / 64;
+    if (o >= R2) return;
+    const int a = o / R, b = o % R;
+    const int src = (a < b ? a : b) * R + (a < b ? b : a);
+    double s = 0.0;
+    for (int p = lane; p < nparts; p += 64) s += part[(int64_t)p * R2 + src];
+    s = wave_sum(s);
+    if (lane == 0) G[o] = s;
 }
 
 // ---------------------------------------------------------------- Jacobi
@@ -88,21 +118,19 @@ constexpr int kJacobiThreads = 256;
 // New value of element (r, c), r < c, r in pair X, c in pair Y (X != Y):
 // rotation X applied to the rows, then rotation Y to the columns.  v[i][k] =
 // A[{p_X, q_X}[i]][{p_Y, q_Y}[k]]; ir / ic = position of r / c in its pair.
-__device__ __forceinline__ double rot_elem(const JacobiShared& sh, int X, int Y, int ir, int ic, double v00,
-                                           double v01, double v10, double v11) {
+__device__ __forceinline__ double rot_elem(int rX, double cX, double sX, int rY, double cY, double sY, int ir, int ic,
+                                           double v00, double v01, double v10, double v11) {
 #pragma clang fp contract(off)
     double nr0, nr1;
-    if (sh.rot[X]) {
-        const double c = sh.c[X], s = sh.s[X];
-        if (ir == 0) { nr0 = c * v00 - s * v10; nr1 = c * v01 - s * v11; }
-        else { nr0 = s * v00 + c * v10; nr1 = s * v01 + c * v11; }
+    if (rX) {
+        if (ir == 0) { nr0 = cX * v00 - sX * v10; nr1 = cX * v01 - sX * v11; }
+        else { nr0 = sX * v00 + cX * v10; nr1 = sX * v01 + cX * v11; }
     } else {
         nr0 = ir == 0 ? v00 : v10;
         nr1 = ir == 0 ? v01 : v11;
     }
-    if (!sh.rot[Y]) return ic == 0 ? nr0 : nr1;
-    const double c = sh.c[Y], s = sh.s[Y];
-    return ic == 0 ? c * nr0 - s * nr1 : s * nr0 + c * nr1;
+    if (!rY) return ic == 0 ? nr0 : nr1;
+    return ic == 0 ? cY * nr0 - sY * nr1 : sY * nr0 + cY * nr1;
 }
 
 // Cyclic Jacobi on the packed upper triangle A (LDS) with eigenvectors V
@@ -184,20 +212,21 @@ __device__ void jacobi_block(double* A, double* V, int R, JacobiShared& sh, uint
                 for (int u = 0; u < 4; u++) ni[t][u] = -1;
                 const int P = bP[t], Q = bQ[t];
                 if (P < 0) continue;
+                // every parameter this block needs, loaded in one batch
                 const int rP = sh.rot[P], rQ = sh.rot[Q];
+                const int p1 = sh.p[P], q1 = sh.q[P], p2 = sh.p[Q], q2 = sh.q[Q];
+                const double cP = sh.c[P], sP = sh.s[P], cQ = sh.c[Q], sQ = sh.s[Q];
                 if (!rP && !rQ) continue;
-                const int p1 = sh.p[P], q1 = sh.q[P];
                 if (P == Q) {
-                    const double tt = sh.t[P], apq = sh.apq[P];
+                    const double tt = sh.t[P], apq = sh.apq[P], app = sh.app[P], aqq = sh.aqq[P];
                     ni[t][0] = tri_index(R, p1, p1);
-                    nv[t][0] = sh.app[P] - tt * apq;
+                    nv[t][0] = app - tt * apq;
                     ni[t][1] = tri_index(R, q1, q1);
-                    nv[t][1] = sh.aqq[P] + tt * apq;
+                    nv[t][1] = aqq + tt * apq;
                     ni[t][2] = tri_index(R, p1, q1);
                     nv[t][2] = 0.0;
                     continue;
                 }
-                const int p2 = sh.p[Q], q2 = sh.q[Q];
                 const bool vq1 = q1 < n, vq2 = q2 < n;
                 const double v00 = symA(A, R, p1, p2);
                 const double v01 = vq2 ? symA(A, R, p1, q2) : 0.0;
@@ -209,8 +238,8 @@ __device__ void jacobi_block(double* A, double* V, int R, JacobiShared& sh, uint
                     const int ra = ir ? q1 : p1, cb = ic ? q2 : p2;
                     if (ra >= n || cb >= n) continue;
                     double val;
-                    if (ra < cb) val = rot_elem(sh, P, Q, ir, ic, v00, v01, v10, v11);
-                    else val = rot_elem(sh, Q, P, ic, ir, v00, v10, v01, v11);
+                    if (ra < cb) val = rot_elem(rP, cP, sP, rQ, cQ, sQ, ir, ic, v00, v01, v10, v11);
+                    else val = rot_elem(rQ, cQ, sQ, rP, cP, sP, ic, ir, v00, v10, v01, v11);
                     ni[t][u] = ra < cb ? tri_index(R, ra, cb) : tri_index(R, cb, ra);
                     nv[t][u] = val;
                 }
@@ -218,9 +247,9 @@ __device__ void jacobi_block(double* A, double* V, int R, JacobiShared& sh, uint
 #pragma unroll
             for (int u = 0; u < NV; u++) {
                 const int kk = vk[u], i = vi[u];
-                if (kk < 0 || !sh.rot[i]) continue;
-                const int p = sh.p[i], q = sh.q[i];
+                const int ri = sh.rot[i], p = sh.p[i], q = sh.q[i];
                 const double c = sh.c[i], s = sh.s[i];
+                if (kk < 0 || !ri) continue;
                 const double vp = V[kk * n + p], vq = V[kk * n + q];
                 V[kk * n + p] = c * vp - s * vq;
                 V[kk * n + q] = s * vp + c * vq;
@@ -515,12 +544,17 @@ __global__ __launch_bounds__(kThreads) void col_sumsq_partial(const double* __re
     }
 }
 
-__global__ void col_sumsq_final(const double* __restrict__ part, int nparts, int R, double* __restrict__ sigma) {
-    int c = threadIdx.x;
+// One wavefront per column: lane l sums partials l, l+64, ... in order, then
+// a fixed butterfly; sigma = sqrt of the total.
+__global__ __launch_bounds__(kThreads) void col_sumsq_final(const double* __restrict__ part, int nparts, int R,
+                                                            double* __restrict__ sigma) {
+    const int lane = threadIdx.x % 64;
+    const int c = blockIdx.x * (kThreads / 64) + threadIdx.x / 64;
     if (c >= R) return;
     double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[(int64_t)p * R + c];
-    sigma[c] = sqrt(s);
+    for (int p = lane; p < nparts; p += 64) s += part[(int64_t)p * R + c];
+    s = wave_sum(s);
+    if (lane == 0) sigma[c] = sqrt(s);
 }
 
 __global__ __launch_bounds__(kThreads) void scale_cols_kernel(double* __restrict__ U, int64_t m, int R,
@@ -621,7 +655,7 @@ void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& p
     int nb = (int)grid_for(m, kGramRows, kGramMaxBlocks);
     part.ensure((size_t)nb * R * R * sizeof(double));
     hipLaunchKernelGGL(gram_partial, dim3(nb), dim3(kThreads), 0, s, A, m, R, part.as<double>());
-    hipLaunchKernelGGL(gram_final, dim3(1), dim3(kThreads), 0, s, part.as<double>(), nb, R, G);
+    hipLaunchKernelGGL(gram_final, dim3((R * R + 3) / 4), dim3(kThreads), 0, s, part.as<double>(), nb, R, G);
     EX_HIP(hipGetLastError());
 }
 
@@ -662,7 +696,7 @@ void col_sumsq(hipStream_t s, const double* U, int64_t m, int R, double* sigma, 
     int nb = (int)grid_for(m, 1024, 1024);
     part.ensure((size_t)nb * R * sizeof(double));
     hipLaunchKernelGGL(col_sumsq_partial, dim3(nb), dim3(kThreads), 0, s, U, m, R, part.as<double>());
-    hipLaunchKernelGGL(col_sumsq_final, dim3(1), dim3(64), 0, s, part.as<double>(), nb, R, sigma);
+    hipLaunchKernelGGL(col_sumsq_final, dim3((R + 3) / 4), dim3(kThreads), 0, s, part.as<double>(), nb, R, sigma);
     EX_HIP(hipGetLastError());
 }
 

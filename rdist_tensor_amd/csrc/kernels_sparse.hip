@@ -210,27 +210,49 @@ __global__ __launch_bounds__(kThreads) void seg_chunk_reduce(const uint32_t* __r
     for (int64_t w = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64; w < nch;
          w += (int64_t)gridDim.x * (kThreads / 64)) {
         const int64_t b = w * kSegChunk, e = b + kSegChunk < H ? b + kSegChunk : H;
-        uint32_t cur = hrow[b];
+        // the chunk's hits, 4 per lane: row, sample index, nonzero value
+        uint32_t hr[kSegChunk / 64], hc[kSegChunk / 64];
+        double hx[kSegChunk / 64];
+#pragma unroll
+        for (int m = 0; m < kSegChunk / 64; m++) {
+            const int64_t h = b + 64 * m + lane;
+            if (h < e) {
+                const uint64_t pay = hpay[h];
+                hr[m] = hrow[h];
+                hc[m] = (uint32_t)(pay >> 32);
+                hx[m] = val[(uint32_t)pay];
+            } else {
+                hr[m] = 0;
+                hc[m] = 0;
+                hx[m] = 0.0;
+            }
+        }
+        uint32_t cur = readlane_u32(hr[0], 0);
         bool cur_head = (b == 0) || hrow[b - 1] != cur;
         uint32_t f0 = 0, f1 = 0, heads = 0;
         double acc = 0.0;
-        for (int64_t h = b; h < e; h++) {
-            const uint32_t r = hrow[h];
-            if (r != cur) {
-                if (cur_head) {
-                    if (lane < R) Y[(int64_t)cur * R + lane] = acc;
-                    heads++;
-                } else {
-                    if (lane < R) carry[(2 * w) * R + lane] = acc;
-                    f0 = 1;
+        const int lr = lane < R ? lane : 0;
+#pragma unroll
+        for (int m = 0; m < kSegChunk / 64; m++) {
+            const int n = (int)(e - b - 64 * m) < 64 ? (int)(e - b - 64 * m) : 64;
+#pragma unroll 8
+            for (int l = 0; l < n; l++) {
+                const uint32_t r = readlane_u32(hr[m], l);
+                if (r != cur) {
+                    if (cur_head) {
+                        if (lane < R) Y[(int64_t)cur * R + lane] = acc;
+                        heads++;
+                    } else {
+                        if (lane < R) carry[(2 * w) * R + lane] = acc;
+                        f0 = 1;
+                    }
+                    cur = r;
+                    acc = 0.0;
+                    cur_head = true;
                 }
-                cur = r;
-                acc = 0.0;
-                cur_head = true;
+                const uint32_t c = readlane_u32(hc[m], l);
+                acc += design[(int64_t)c * R + lr] * readlane_f64(hx[m], l);
             }
-            const uint64_t pay = hpay[h];
-            const uint32_t c = (uint32_t)(pay >> 32), p = (uint32_t)pay;
-            if (lane < R) acc += design[(int64_t)c * R + lane] * val[p];
         }
         const bool ends = (e == H) || hrow[e] != cur;
         if (cur_head) heads++;
@@ -274,56 +296,49 @@ __global__ __launch_bounds__(kThreads) void seg_fixup(const uint32_t* __restrict
 // ---------------------------------------------------------------- fit
 struct FitArgs {
     const double* U[kMaxModes];
-    int shift[kMaxModes];
-    uint32_t mask[kMaxModes];
-    int N, j, R;
+    int N, R;
 };
 
-// One wavefront per fiber of the mode-j lookup: partial product sigma *
-// prod_{k != j} U_k[i_k] once per fiber, then x^2 - 2 x xhat per nonzero
-// (sort_lookup.hpp:377-438).  part[block] = block sum in fiber order.
-__global__ __launch_bounds__(kThreads) void fit_kernel(const uint64_t* __restrict__ fkey,
-                                                       const uint32_t* __restrict__ fptr, int64_t nfib,
-                                                       const uint32_t* __restrict__ rowj,
-                                                       const double* __restrict__ val, const double* __restrict__ sigma,
-                                                       FitArgs a, double* __restrict__ part) {
-    __shared__ double red[kThreads / 64];
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    const int64_t per = (nfib + gridDim.x - 1) / gridDim.x;
-    const int64_t f0 = blockIdx.x * per, f1 = f0 + per < nfib ? f0 + per : nfib;
+// Sum over nonzeros of x^2 - 2 x xhat, xhat = sum_r (sigma_r prod_{k != 0}
+// U_k[i_k][r]) U_0[i_0][r] (compute_2bmb, sort_lookup.hpp:377-438, with the
+// partial product recomputed per nonzero).  One thread per nonzero; block b
+// sums a contiguous nonzero range (threads stride it, then an LDS tree in a
+// fixed order) into part[b].
+__global__ __launch_bounds__(kThreads) void fit_kernel(const uint32_t* __restrict__ coords,
+                                                       const double* __restrict__ val, int64_t nnz,
+                                                       const double* __restrict__ sigma, FitArgs a,
+                                                       double* __restrict__ part) {
+    __shared__ double red[kThreads];
+    const int64_t per = (nnz + gridDim.x - 1) / gridDim.x;
+    const int64_t i0 = blockIdx.x * per, i1 = i0 + per < nnz ? i0 + per : nnz;
     double acc = 0.0;
-    for (int64_t f = f0 + wave; f < f1; f += kThreads / 64) {
-        uint64_t key = fkey[f];
-        double pp = 0.0;
-        if (lane < a.R) {
-            pp = sigma[lane];
-            for (int k = 0; k < a.N; k++) {
-                if (k == a.j) continue;
-                uint32_t ik = (uint32_t)(key >> a.shift[k]) & a.mask[k];
-                pp *= a.U[k][(int64_t)ik * a.R + lane];
-            }
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+        const uint32_t* c = coords + i * a.N;
+        const double* u0 = a.U[0] + (int64_t)c[0] * a.R;
+        double v = 0.0;
+        for (int r = 0; r < a.R; r++) {
+            double pp = sigma[r];
+            for (int k = 1; k < a.N; k++) pp *= a.U[k][(int64_t)c[k] * a.R + r];
+            v += pp * u0[r];
         }
-        for (uint32_t p = fptr[f]; p < fptr[f + 1]; p++) {
-            double v = lane < a.R ? pp * a.U[a.j][(int64_t)rowj[p] * a.R + lane] : 0.0;
-            v = wave_sum(v);
-            double x = val[p];
-            acc += x * x - 2 * v * x;
-        }
+        const double x = val[i];
+        acc += x * x - 2 * v * x;
     }
-    if (lane == 0) red[wave] = acc;
+    red[threadIdx.x] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < kThreads / 64; w++) s += red[w];
-        part[blockIdx.x] = s;
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 __global__ void fit_final(const double* __restrict__ part, int n, double* __restrict__ out) {
-    if (threadIdx.x != 0) return;
+    const int lane = threadIdx.x;
     double s = 0.0;
-    for (int i = 0; i < n; i++) s += part[i];
-    *out = s;
+    for (int i = lane; i < n; i += 64) s += part[i];
+    s = wave_sum(s);
+    if (lane == 0) *out = s;
 }
 
 KeyArgs key_args(const KeyLayout& kl) {
@@ -493,28 +508,15 @@ int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const uint64_t* kd, i
     return H;
 }
 
-void fit_bmb(hipStream_t s, const ModeLookup& L0, int N, const double* const* U, const double* sigma, int R,
-             double* out, DevBuf& part) {
+void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
+             const double* sigma, int R, double* out, DevBuf& part) {
     FitArgs a;
-    for (int k = 0; k < N; k++) {
-        a.U[k] = U[k];
-        a.shift[k] = L0.kl.shift[k];
-        int bits = 0;
-        if (k != L0.mode) {
-            // bits of mode k = distance to the next used shift above it
-            int next = 64;
-            for (int q = 0; q < N; q++)
-                if (q != L0.mode && L0.kl.shift[q] > L0.kl.shift[k] && L0.kl.shift[q] < next) next = L0.kl.shift[q];
-            if (next == 64) next = L0.kl.bits;
-            bits = next - L0.kl.shift[k];
-        }
-        a.mask[k] = bits >= 32 ? 0xFFFFFFFFu : ((1u << bits) - 1u);
-    }
-    a.N = N; a.j = L0.mode; a.R = R;
-    int nb = (int)grid_for(L0.nfib, 64, 4096);
+    for (int k = 0; k < N; k++) a.U[k] = U[k];
+    a.N = N;
+    a.R = R;
+    int nb = (int)grid_for(nnz, 4 * kThreads, 4096);
     part.ensure((size_t)nb * sizeof(double));
-    hipLaunchKernelGGL(fit_kernel, dim3(nb), dim3(kThreads), 0, s, L0.fkey.as<uint64_t>(), L0.fptr.as<uint32_t>(),
-                       L0.nfib, L0.rowj.as<uint32_t>(), L0.val.as<double>(), sigma, a, part.as<double>());
+    hipLaunchKernelGGL(fit_kernel, dim3(nb), dim3(kThreads), 0, s, coords, vals, nnz, sigma, a, part.as<double>());
     hipLaunchKernelGGL(fit_final, dim3(1), dim3(64), 0, s, part.as<double>(), nb, out);
     EX_HIP(hipGetLastError());
 }
