@@ -1,15 +1,21 @@
-"""Benchmark: randomized CP-ALS sweeps/s on one MI355X.
+"""Benchmark: randomized CP-ALS sweeps/s on one MI355X, mixed vs FP64.
 
 Workload (default): STS-CP on a 3-mode 1,000,000 x 1,000,000 x 100,000 tensor
-with 5e7 distinct Zipf(1.1) nonzeros, lognormal values stored FP32 (computed
-in FP64 on the plain path), R = 32, J = 65,536, seed 1.  A step is one ALS
-sweep (one randomized least-squares solve per mode).  Inputs are resident in
-HBM before the timed region.
+with 5e7 distinct Zipf(1.1) nonzeros, lognormal values stored FP32, R = 32,
+J = 65,536, seed 1.  A step is one ALS sweep (one randomized least-squares
+solve per mode).  The same tensor (resident in HBM before either timed region)
+is decomposed twice from the same initial factors: once on the plain FP64
+path and once on the mixed-precision path (FP32 data, BF16 MFMA Grams, FP64
+R x R solves), each for --steps timed sweeps after --warmup untimed sweeps
+(the factors and sampler are reset to the initial state after the warm-up,
+so both final fits are "after --steps sweeps").  `value` is the mixed path's
+sweeps/s; the FP64 path's figures, the ratio and both final fits sit beside
+it.
 
-Prints one JSON line (the driver's contract) with `roofline` (dominant
-kernel, timed live with HIP events on the library's stream) and
-`cpu_baseline` (the oracle, a bounded sample of the same workload on the host
-cores).
+Prints one JSON line (the driver's contract) with `roofline` (the dominant
+kernel of the reported path, timed live with HIP events on the library's
+stream) and `cpu_baseline` (the oracle, a bounded sample of the same
+workload on the host cores).
 """
 import argparse
 import json
@@ -45,8 +51,9 @@ def tree_levels(n, F):
 
 def sts_draw_bytes(dims, R, J, j, s=8):
     """Algorithmic bytes of one STS-CP draw (all non-target modes): node reads
-    per sample per level (packed upper R(R+1)/2 values, before cache reuse),
-    the leaf's F = R rows, the chosen rows, samples and weights."""
+    per sample per level (packed upper R(R+1)/2 values, before cache reuse;
+    eigen-tree nodes FP64, row-tree nodes s bytes), the leaf's F = R rows and
+    the chosen row (s bytes), the eigenvector row (FP64), samples and weights."""
     P = tri(R)
     total = 0
     for k, n in enumerate(dims):
@@ -54,8 +61,8 @@ def sts_draw_bytes(dims, R, J, j, s=8):
             continue
         le = tree_levels(R, 1) + 2          # root + levels + tail (eigen tree)
         lr = tree_levels(n, R) + 2          # root + levels + tail (row tree)
-        total += J * ((le + lr) * P * s + R * R * s + 2 * R * s)
-    total += J * P * s                      # weight: h^T pinv h
+        total += J * (le * P * 8 + lr * P * s + R * R * s + R * s + R * 8)
+    total += J * P * 8                      # weight: h^T pinv h
     total += J * (len(dims) * 4 + 8)        # samples + weights written
     return total
 
@@ -63,8 +70,9 @@ def sts_draw_bytes(dims, R, J, j, s=8):
 def mttkrp_bytes(N, R, Ju, H, T, s=8):
     """Algorithmic bytes of sampled MTTKRP + KRP gather per mode step:
     sample tuples and weights, gathered factor rows, touched nonzeros
-    (coordinates + value), output rows."""
-    return Ju * (4 * (N - 1) + s) + Ju * (N - 1) * R * s + H * (4 * N + s) + T * R * s
+    (coordinates + value), output rows.  s = bytes per factor / value / output
+    entry (8 FP64, 4 mixed); weights are FP64 on both paths."""
+    return Ju * (4 * (N - 1) + 8) + Ju * (N - 1) * R * s + H * (4 * N + s) + T * R * s
 
 
 def seg_reduce_bytes(R, Ju, H, T, s=8):
@@ -74,30 +82,33 @@ def seg_reduce_bytes(R, Ju, H, T, s=8):
     return H * (4 + 8 + s) + Ju * R * s + T * R * s + (T + 1) * 4
 
 
-def run_gpu(args, cfg, idx, vals, U0):
+def run_gpu(args, cfg, ctx, T, U0, precision):
     from rdist_tensor_amd import abi
-    ctx = abi.Context(0)
-    dev = ctx.device_name()
     dims = cfg["dims"]
     R, J = cfg["R"], cfg["J"]
     kind = abi.EX_SAMPLER_STS_CP if cfg["sampler"] == "sts_cp" else abi.EX_SAMPLER_CP_ARLS_LEV
+    prec = abi.EX_MIXED if precision == "mixed" else abi.EX_FP64
     t0 = time.time()
-    T = abi.Tensor(ctx, idx, vals, dims)
     F = abi.Factors(ctx, dims, R)
     for k in range(len(dims)):
         F.set(k, U0[k])
-    S = abi.Sampler(ctx, F, kind, seed=cfg["seed"])
-    A = abi.ALS(ctx, T, F, S)
+    S = abi.Sampler(ctx, F, kind, precision=prec, seed=cfg["seed"])
+    A = abi.ALS(ctx, T, F, S, precision=prec)
     A.init()
     ctx.sync()
-    log(f"[bench] device {dev}; setup (upload, lookups, sampler) {time.time() - t0:.1f} s")
+    log(f"[bench] {precision}: setup (factors, sampler) {time.time() - t0:.1f} s")
     fit0 = A.fit()
     N = len(dims)
     for w in range(args.warmup):
         for j in range(N):
             A.step(j, J, 1000 + w)
+    # back to the initial state: the timed sweeps start where an untimed run would
+    for k in range(N):
+        F.set(k, U0[k])
+        S.update(k)
+    F.set_sigma(np.ones(R))
     ctx.sync()
-    log(f"[bench] warmup {args.warmup} sweeps done")
+    log(f"[bench] {precision}: warmup {args.warmup} sweeps done")
     ctx.timing(True)
     ctx.sync()
     t0 = time.perf_counter()
@@ -109,8 +120,42 @@ def run_gpu(args, cfg, idx, vals, U0):
     timing = json.loads(ctx.timing_json())
     ctx.timing(False)
     fit1 = A.fit()
-    log(f"[bench] {args.steps} sweeps in {elapsed:.3f} s; fit {fit0:.6f} -> {fit1:.6f}")
-    return dict(device=dev, elapsed=elapsed, timing=timing, fit0=fit0, fit1=fit1)
+    log(f"[bench] {precision}: {args.steps} sweeps in {elapsed:.3f} s; fit {fit0:.6f} -> {fit1:.6f}")
+    del A, S, F
+    return dict(elapsed=elapsed, timing=timing, fit0=fit0, fit1=fit1)
+
+
+def summarize(args, cfg, g, vbytes):
+    """Throughput, dominant launch group and its roofline for one GPU run."""
+    N, R, J = len(cfg["dims"]), cfg["R"], cfg["J"]
+    K = args.steps
+    spans = g["timing"]["spans"]
+    cnt = g["timing"]["counters"]
+    steps = max(cnt.get("mode_steps", 1), 1)
+    dom = max(spans.items(), key=lambda kv: kv[1]["ms"])
+    name, st = dom[0], dom[1]
+    avg_s = st["ms"] / st["count"] * 1e-3
+    if name == "sts_draw":
+        per = np.mean([sts_draw_bytes(cfg["dims"], R, J, j, s=vbytes) for j in range(N)])
+    elif name == "seg_reduce":
+        per = seg_reduce_bytes(R, cnt["J_u"] / steps, cnt["H"] / steps, cnt["T"] / steps, s=vbytes)
+    else:
+        per = float("nan")
+    achieved = per / avg_s / 1e9
+    roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 4),
+            "algorithmic_bytes_per_launch": int(per) if per == per else None}
+    mt = spans.get("sampled_mttkrp")
+    mt_gbs = None
+    if mt and cnt.get("mode_steps"):
+        per_mt = mttkrp_bytes(N, R, cnt["J_u"] / steps, cnt["H"] / steps, cnt["T"] / steps, s=vbytes)
+        mt_gbs = per_mt / (mt["ms"] / mt["count"] * 1e-3) / 1e9
+    return dict(value=K / g["elapsed"], ms_per_step=g["elapsed"] / K * 1e3, roofline=roof,
+                sampled_mttkrp_gbs=None if mt_gbs is None else round(mt_gbs, 2),
+                nnz_per_s=round(cnt.get("H", 0) / g["elapsed"], 1),
+                spans_ms_per_sweep={k: round(v["ms"] / K, 4) for k, v in spans.items()},
+                counters_per_mode_step={k: round(v / steps, 1) for k, v in cnt.items()},
+                fit={"initial": g["fit0"], "final": g["fit1"]})
 
 
 def run_cpu_baseline(cfg, idx, vals, U0, budget_s=30.0):
@@ -139,12 +184,13 @@ def run_cpu_baseline(cfg, idx, vals, U0, budget_s=30.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="sts_1e6x1e6x1e5")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the configured nonzeros")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -155,61 +201,54 @@ def main():
     cfg, idx, vals = synthetic.make(args.config, args.scale)
     U0 = synthetic.factors(cfg["dims"], cfg["R"], cfg["seed"])
     log(f"[bench] generated {len(idx)} nonzeros in {time.time() - t0:.1f} s")
-    g = run_gpu(args, cfg, idx, vals, U0)
+    from rdist_tensor_amd import abi
+    ctx = abi.Context(0)
+    dev = ctx.device_name()
+    t0 = time.time()
+    T = abi.Tensor(ctx, idx, vals, cfg["dims"])
+    T.build_lookups()
+    ctx.sync()
+    log(f"[bench] device {dev}; tensor upload + lookups {time.time() - t0:.1f} s")
+    res = {}
+    for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+        g = run_gpu(args, cfg, ctx, T, U0, prec)
+        res[prec] = summarize(args, cfg, g, 4 if prec == "mixed" else 8)
     N, R, J = len(cfg["dims"]), cfg["R"], cfg["J"]
     K = args.steps
-    ms_per_step = g["elapsed"] / K * 1e3
-    spans = g["timing"]["spans"]
-    cnt = g["timing"]["counters"]
-    # dominant kernel (launch group) by total time
-    dom = max(spans.items(), key=lambda kv: kv[1]["ms"])
-    name, st = dom[0], dom[1]
-    avg_s = st["ms"] / st["count"] * 1e-3
-    if name == "sts_draw":
-        per = np.mean([sts_draw_bytes(cfg["dims"], R, J, j) for j in range(N)])
-    elif name == "seg_reduce":
-        per = seg_reduce_bytes(R, cnt["J_u"] / cnt["mode_steps"], cnt["H"] / cnt["mode_steps"],
-                               cnt["T"] / cnt["mode_steps"])
-    else:
-        per = float("nan")
-    achieved = per / avg_s / 1e9
-    roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 4)}
-    # sampled-MTTKRP GB/s (the metric's second half)
-    mt = spans.get("sampled_mttkrp")
-    mt_gbs = None
-    if mt and cnt.get("mode_steps"):
-        per_mt = mttkrp_bytes(N, R, cnt["J_u"] / cnt["mode_steps"], cnt["H"] / cnt["mode_steps"],
-                              cnt["T"] / cnt["mode_steps"])
-        mt_gbs = per_mt / (mt["ms"] / mt["count"] * 1e-3) / 1e9
+    head = "mixed" if "mixed" in res else "fp64"
+    h = res[head]
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(cfg, idx, vals, U0, args.cpu_budget)
     out = {
         "metric": "ALS sweeps/s + sampled-MTTKRP HBM GB/s, mixed vs FP64 GPU at equal fit",
-        "value": round(K / g["elapsed"], 4),
+        "value": round(h["value"], 4),
         "unit": "sweeps/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(h["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+bf16(mfma)/f64 solves" if head == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: STS-CP plain FP64, dims {cfg['dims']}, nnz {len(idx)}, "
-                               f"Zipf {cfg['zipf']}, R={R}, J={J}, seed {cfg['seed']}",
-                   "precision": "fp64", "sampler": cfg["sampler"]},
-        "sampled_mttkrp_gbs": None if mt_gbs is None else round(mt_gbs, 2),
-        "nnz_per_s": round(cnt.get("H", 0) / g["elapsed"], 1),
-        "fit": {"initial": g["fit0"], "final": g["fit1"]},
-        "spans_ms_per_sweep": {k: round(v["ms"] / K, 4) for k, v in spans.items()},
-        "counters_per_mode_step": {k: round(v / max(cnt.get("mode_steps", 1), 1), 1) for k, v in cnt.items()},
-        "roofline": roof,
+        "config": {"workload": f"{args.config}: STS-CP, dims {cfg['dims']}, nnz {len(idx)}, Zipf {cfg['zipf']}, "
+                               f"lognormal values, R={R}, J={J}, seed {cfg['seed']}, {K} sweeps per precision",
+                   "precision": head, "sampler": cfg["sampler"]},
+        "sampled_mttkrp_gbs": h["sampled_mttkrp_gbs"],
+        "nnz_per_s": h["nnz_per_s"],
+        "fit": h["fit"],
+        "roofline": h["roofline"],
         "cpu_baseline": cpu,
-        "device": g["device"],
+        "device": dev,
     }
+    if "mixed" in res and "fp64" in res:
+        out["mixed_vs_fp64"] = {"speedup": round(res["mixed"]["value"] / res["fp64"]["value"], 3),
+                                "final_fit_mixed": res["mixed"]["fit"]["final"],
+                                "final_fit_fp64": res["fp64"]["fit"]["final"],
+                                "final_fit_abs_diff": abs(res["mixed"]["fit"]["final"] - res["fp64"]["fit"]["final"])}
+    out["paths"] = res
     print(json.dumps(out), flush=True)
 
 

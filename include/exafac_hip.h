@@ -38,6 +38,15 @@ enum ex_status {
     EX_ERR_NOMEM = 5         /* device allocation failed */
 };
 
+/* EX_FP64: every array and product in FP64 (the reference's arithmetic).
+ * EX_MIXED: factors, nonzero values, design rows, row-tree nodes and the
+ * sampled MTTKRP in FP32; Gram products as BF16 on the matrix cores with
+ * FP32 accumulation and an FP64 final sum; R x R inverses, Cholesky factors,
+ * eigen trees, leverage normalisers and weights in FP64.  The STS-CP
+ * computeM factorisations use Cholesky instead of eigendecompositions (the
+ * row distribution is the same; the eigen-component draw differs), so mixed
+ * draws are not index-identical to FP64 draws.  The FP64 factor copies are
+ * refreshed from FP32 on read (ex_factors_get, fits). */
 enum ex_precision { EX_FP64 = 0, EX_MIXED = 1 };
 enum ex_sampler_kind { EX_SAMPLER_CP_ARLS_LEV = 0, EX_SAMPLER_STS_CP = 1 };
 enum ex_distribution { EX_DIST_ACCUMULATOR_STATIONARY = 0, EX_DIST_TENSOR_STATIONARY = 1 };

@@ -267,9 +267,6 @@ class Sampler:
         N = self.f.N
         samples = np.empty((J, N), np.uint32)
         w = np.empty(J)
-        dr = _p(f64(draws_row)) if draws_row is not None else None
-        de = _p(f64(draws_eig)) if draws_eig is not None else None
-        keep = (draws_row, draws_eig)  # noqa: F841 keep arrays alive during the call
         dr_arr = f64(draws_row) if draws_row is not None else None
         de_arr = f64(draws_eig) if draws_eig is not None else None
         dr = _p(dr_arr) if dr_arr is not None else None

@@ -163,6 +163,50 @@ struct ex_factors {
     DevBuf sigma;
     DevBuf part, gram;
     const double* ptrs[kMaxModes] = {};
+    // FP32 mirrors for the mixed-precision path, allocated when a mixed
+    // sampler is created.  Each mode's current values live in the FP64 copy,
+    // the FP32 copy or both; a stale copy is refreshed on its next read.
+    bool has32 = false;
+    DevBuf U32[kMaxModes];
+    const float* ptrs32[kMaxModes] = {};
+    bool stale32[kMaxModes] = {};
+    bool stale64[kMaxModes] = {};
+
+    void enable32() {
+        if (has32) return;
+        for (int k2 = 0; k2 < N; k2++) {
+            U32[k2].ensure((size_t)dims[k2] * R * 4);
+            ptrs32[k2] = U32[k2].as<float>();
+            stale32[k2] = true;
+        }
+        has32 = true;
+    }
+    // the FP64 copy of `mode` is about to be read
+    void sync64(int mode) {
+        if (!stale64[mode]) return;
+        k::to_f64(ctx->c.stream, U32[mode].as<float>(), dims[mode] * R, U[mode].as<double>());
+        stale64[mode] = false;
+    }
+    void sync64_all() {
+        for (int k2 = 0; k2 < N; k2++) sync64(k2);
+    }
+    void sync32(int mode) {
+        if (!stale32[mode]) return;
+        k::to_f32(ctx->c.stream, U[mode].as<double>(), dims[mode] * R, U32[mode].as<float>());
+        stale32[mode] = false;
+    }
+    void sync32_all() {
+        for (int k2 = 0; k2 < N; k2++) sync32(k2);
+    }
+    // the FP64 (wrote64) or FP32 (wrote32) copy of `mode` was written
+    void wrote64(int mode) {
+        stale64[mode] = false;
+        if (has32) stale32[mode] = true;
+    }
+    void wrote32(int mode) {
+        stale32[mode] = false;
+        stale64[mode] = true;
+    }
 };
 
 struct ex_sampler {
@@ -172,7 +216,7 @@ struct ex_sampler {
     uint64_t seed = 0;
     // CP-ARLS-LEV
     DevBuf scores[kMaxModes], cdf[kMaxModes], sums, gtmp, gpinv;
-    // STS-CP
+    // STS-CP (row-tree nodes FP64, or FP32 on the mixed path)
     TreeGeom g[kMaxModes];
     TreeGeom ge;
     DevBuf nodes[kMaxModes];
@@ -189,7 +233,7 @@ struct ex_als {
     ex_sampler* s = nullptr;
     int dist = 0, precision = 0;
     bool initialized = false;
-    DevBuf sd, wd, kd, dg, dm, G, Gi, Y, scal;
+    DevBuf sd, wd, kd, dg, dm, G, Gi, Gi32, Y, scal;
     DevBuf tmp[4];
     DevBuf part;
     double t_region[7] = {};
@@ -199,11 +243,38 @@ struct ex_als {
 
 namespace {
 
-void sampler_update_impl(ex_sampler* s, int j) {
+// Mixed precision: FP32 factor, BF16 MFMA Grams, FP64 inverse and scores.
+void sampler_update_mixed(ex_sampler* s, int j) {
     ex_factors* f = s->f;
     hipStream_t st = s->ctx->c.stream;
     const int R = f->R;
     const int64_t I = f->dims[j];
+    f->sync32(j);
+    const float* U = f->U32[j].as<float>();
+    Span sp(s->ctx, s->kind == EX_SAMPLER_CP_ARLS_LEV ? "arls_update" : "tree_build");
+    if (s->kind == EX_SAMPLER_CP_ARLS_LEV) {
+        s->gtmp.ensure((size_t)R * R * 8);
+        s->gpinv.ensure((size_t)R * R * 8);
+        k::gram_bf16(st, U, I, R, s->gtmp.as<double>(), s->part);
+        k::spd_inverse(st, s->gtmp.as<double>(), R, 1e-11, s->gpinv.as<double>(), nullptr);
+        s->scores[j].ensure((size_t)I * 8);
+        s->cdf[j].ensure((size_t)I * 8);
+        k::dagat32(st, U, I, R, s->gpinv.as<double>(), s->scores[j].as<double>());
+        k::arls_cdf(st, s->scores[j].as<double>(), I, s->cdf[j].as<double>(), s->sums.as<double>() + j, s->part);
+    } else {
+        k::tree_build_f32(st, U, s->g[j], R, s->nodes[j].as<float>());
+        k::gram_bf16(st, U, I, R, s->grams.as<double>() + (int64_t)j * R * R, s->part);
+    }
+    sp.end();
+}
+
+void sampler_update_impl(ex_sampler* s, int j) {
+    if (s->precision == EX_MIXED) return sampler_update_mixed(s, j);
+    ex_factors* f = s->f;
+    hipStream_t st = s->ctx->c.stream;
+    const int R = f->R;
+    const int64_t I = f->dims[j];
+    f->sync64(j);
     const double* U = f->U[j].as<double>();
     Span sp(s->ctx, s->kind == EX_SAMPLER_CP_ARLS_LEV ? "arls_update" : "tree_build");
     if (s->kind == EX_SAMPLER_CP_ARLS_LEV) {
@@ -242,16 +313,20 @@ void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const do
                      weights);
         sp.end();
     } else {
+        const bool mixed = s->precision == EX_MIXED;
         Span sm(s->ctx, "sts_compute_m");
-        k::sts_compute_m(st, N, j, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
+        if (mixed) k::sts_compute_m_mixed_launch(st, N, j, R, s->grams.as<double>(), s->M.as<double>());
+        else k::sts_compute_m(st, N, j, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
         k::eig_tree_build(st, N, j, R, s->M.as<double>(), s->grams.as<double>(), s->eig.as<double>());
         k::pack_upper(st, s->M.as<double>(), R, s->pinvp.as<double>());
         sm.end();
         k::StsModeArgs m[kMaxModes];
         const int S = tri_stride(R);
+        if (mixed) f->sync32_all();
+        else f->sync64_all();
         for (int k2 = 0; k2 < N; k2++) {
-            m[k2].rows = f->U[k2].as<double>();
-            m[k2].nodes = s->nodes[k2].as<double>();
+            m[k2].rows = mixed ? (const void*)f->U32[k2].p : (const void*)f->U[k2].p;
+            m[k2].nodes = s->nodes[k2].p;
             m[k2].eig = s->eig.as<double>() + (int64_t)k2 * s->ge.node_count * S;
             int off = (k2 + 1 == j) ? k2 + 2 : k2 + 1;
             m[k2].w = s->M.as<double>() + (int64_t)off * R * R;
@@ -260,7 +335,7 @@ void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const do
         }
         Span sd(s->ctx, "sts_draw");
         k::sts_draw(st, N, j, R, J, m, s->pinvp.as<double>(), s->seed, sweep, draws_row_dev, draws_eig_dev, samples,
-                    weights);
+                    weights, mixed ? 4 : 8);
         sd.end();
     }
 }
@@ -335,22 +410,42 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     Span s_dd(a->ctx, "dedup");
     const int64_t Ju = k::dedup(st, s->samples.as<uint32_t>(), s->weights.as<double>(), J, N, kl,
                                 a->sd.as<uint32_t>(), a->wd.as<double>(), a->kd.as<uint64_t>(), a->tmp);
-    a->dg.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
-    a->dm.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
+    const bool mixed = a->precision == EX_MIXED;
+    const size_t vb = mixed ? 4 : 8;   // bytes per design / MTTKRP value
+    a->dg.ensure((size_t)(Ju > 0 ? Ju : 1) * R * vb);
+    a->dm.ensure((size_t)(Ju > 0 ? Ju : 1) * R * vb);
     a->G.ensure((size_t)R * R * 8);
     a->Gi.ensure((size_t)R * R * 8);
+    a->Gi32.ensure((size_t)R * R * 4);
     s_dd.end();
     Span s_de(a->ctx, "design_gather");
-    k::design(st, a->sd.as<uint32_t>(), a->wd.as<double>(), Ju, N, j, R, f->ptrs, a->dg.as<double>(),
-              a->dm.as<double>());
+    if (mixed) {
+        f->sync32_all();
+        k::design32(st, a->sd.as<uint32_t>(), a->wd.as<double>(), Ju, N, j, R, f->ptrs32, a->dg.as<float>(),
+                    a->dm.as<float>());
+    } else {
+        f->sync64_all();
+        k::design(st, a->sd.as<uint32_t>(), a->wd.as<double>(), Ju, N, j, R, f->ptrs, a->dg.as<double>(),
+                  a->dm.as<double>());
+    }
     s_de.end();
     Span s_dp(a->ctx, "design_gram_pinv");
-    k::gram(st, a->dg.as<double>(), Ju, R, a->G.as<double>(), a->part);
-    k::pinv(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>());
+    if (mixed) {
+        k::gram_bf16(st, a->dg.as<float>(), Ju, R, a->G.as<double>(), a->part);
+        k::spd_inverse(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>(), a->Gi32.as<float>());
+    } else {
+        k::gram(st, a->dg.as<double>(), Ju, R, a->G.as<double>(), a->part);
+        k::pinv(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>());
+    }
     s_dp.end();
     ev.mark(2, st);
     // Region 4: sampled MTTKRP (:252-274)
-    a->Y.ensure((size_t)Ij * R * 8);
+    a->Y.ensure((size_t)Ij * R * vb);
+    ModeLookup& L = a->t->lk[j];
+    if (mixed && !L.val32.p) {
+        L.val32.ensure((size_t)(a->t->nnz > 0 ? a->t->nnz : 1) * 4);
+        k::to_f32(st, L.val.as<double>(), a->t->nnz, L.val32.as<float>());
+    }
     Span s_mt(a->ctx, "sampled_mttkrp");
     int64_t Tn = 0;
     hipEvent_t seg_ev[2] = {nullptr, nullptr};
@@ -358,9 +453,13 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
         seg_ev[0] = a->ctx->get_event();
         seg_ev[1] = a->ctx->get_event();
     }
-    const int64_t H = k::sampled_mttkrp(st, a->t->lk[j], a->kd.as<uint64_t>(), Ju, a->dm.as<double>(), R,
-                                        a->Y.as<double>(), Ij, a->tmp, a->ctx->timing ? &Tn : nullptr,
-                                        a->ctx->timing ? seg_ev : nullptr);
+    const int64_t H =
+        mixed ? k::sampled_mttkrp<float>(st, L, L.val32.as<float>(), a->kd.as<uint64_t>(), Ju, a->dm.as<float>(), R,
+                                         a->Y.as<float>(), Ij, a->tmp, a->ctx->timing ? &Tn : nullptr,
+                                         a->ctx->timing ? seg_ev : nullptr)
+              : k::sampled_mttkrp<double>(st, L, L.val.as<double>(), a->kd.as<uint64_t>(), Ju, a->dm.as<double>(),
+                                          R, a->Y.as<double>(), Ij, a->tmp, a->ctx->timing ? &Tn : nullptr,
+                                          a->ctx->timing ? seg_ev : nullptr);
     s_mt.end();
     if (a->ctx->timing) {
         if (H > 0) a->ctx->pending.push_back({"seg_reduce", {seg_ev[0], seg_ev[1]}});
@@ -375,11 +474,19 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     }
     ev.mark(3, st);
     // Region 6: U_j = Y G+, renormalise (:280-289)
-    double* Uj = f->U[j].as<double>();
     Span s_fu(a->ctx, "factor_update");
-    k::dsymm(st, a->Gi.as<double>(), a->Y.as<double>(), Ij, R, Uj);
-    k::col_sumsq(st, Uj, Ij, R, f->sigma.as<double>(), a->part);
-    k::scale_cols(st, Uj, Ij, R, f->sigma.as<double>());
+    if (mixed) {
+        float* Uj = f->U32[j].as<float>();
+        k::dsymm32(st, a->Gi32.as<float>(), a->Y.as<float>(), Ij, R, Uj);
+        k::renormalize32(st, Uj, Ij, R, f->sigma.as<double>(), a->part);
+        f->wrote32(j);
+    } else {
+        double* Uj = f->U[j].as<double>();
+        k::dsymm(st, a->Gi.as<double>(), a->Y.as<double>(), Ij, R, Uj);
+        k::col_sumsq(st, Uj, Ij, R, f->sigma.as<double>(), a->part);
+        k::scale_cols(st, Uj, Ij, R, f->sigma.as<double>());
+        f->wrote64(j);
+    }
     s_fu.end();
     ev.mark(4, st);
     // Region 7: sampler update (:291-294)
@@ -402,6 +509,7 @@ double als_fit_impl(ex_als* a) {
     a->G.ensure((size_t)N * R * R * 8 > (size_t)R * R * 8 ? (size_t)N * R * R * 8 : (size_t)R * R * 8);
     DevBuf grams;
     grams.ensure((size_t)N * R * R * 8);
+    f->sync64_all();
     for (int k2 = 0; k2 < N; k2++)
         k::gram(st, f->U[k2].as<double>(), f->dims[k2], R, grams.as<double>() + (int64_t)k2 * R * R, a->part);
     double* d = a->scal.as<double>();
@@ -574,8 +682,8 @@ int ex_sampled_mttkrp(ex_tensor* t, int mode, int64_t Ju, const uint32_t* sample
         h2d(st, ds.p, samples, (size_t)Ju * N * 4);
         h2d(st, dd.p, design, (size_t)Ju * R * 8);
         k::keys_from_coords(st, ds.as<uint32_t>(), Ju, t->lk[mode].kl, dk.as<uint64_t>(), nullptr);
-        int64_t H = k::sampled_mttkrp(st, t->lk[mode], dk.as<uint64_t>(), Ju, dd.as<double>(), (int)R, dy.as<double>(),
-                                      Ij, tmp);
+        int64_t H = k::sampled_mttkrp<double>(st, t->lk[mode], t->lk[mode].val.as<double>(), dk.as<uint64_t>(), Ju,
+                                              dd.as<double>(), (int)R, dy.as<double>(), Ij, tmp);
         d2h(st, Y, dy.p, (size_t)Ij * R * 8);
         if (hits) *hits = H;
     });
@@ -616,6 +724,7 @@ int ex_factors_set(ex_factors* f, int mode, const double* host) {
         EX_CHECK(f && host, EX_ERR_ARG, "null argument");
         EX_CHECK(mode >= 0 && mode < f->N, EX_ERR_ARG, "mode out of range");
         h2d(f->ctx->c.stream, f->U[mode].p, host, (size_t)f->dims[mode] * f->R * 8);
+        f->wrote64(mode);
         EX_HIP(hipStreamSynchronize(f->ctx->c.stream));
     });
 }
@@ -624,6 +733,7 @@ int ex_factors_get(ex_factors* f, int mode, double* host) {
     return guarded([&] {
         EX_CHECK(f && host, EX_ERR_ARG, "null argument");
         EX_CHECK(mode >= 0 && mode < f->N, EX_ERR_ARG, "mode out of range");
+        f->sync64(mode);
         d2h(f->ctx->c.stream, host, f->U[mode].p, (size_t)f->dims[mode] * f->R * 8);
     });
 }
@@ -650,8 +760,10 @@ int ex_factors_renormalize(ex_factors* f, int mode, double* sigma) {
         hipStream_t st = f->ctx->c.stream;
         DevBuf sg;
         sg.ensure((size_t)f->R * 8);
+        f->sync64(mode);
         k::col_sumsq(st, f->U[mode].as<double>(), f->dims[mode], f->R, sg.as<double>(), f->part);
         k::scale_cols(st, f->U[mode].as<double>(), f->dims[mode], f->R, sg.as<double>());
+        f->wrote64(mode);
         if (sigma) d2h(st, sigma, sg.p, (size_t)f->R * 8);
         EX_HIP(hipStreamSynchronize(st));
     });
@@ -667,6 +779,7 @@ int ex_factors_init_deterministic(ex_factors* f) {
             k::cos_init(st, f->U[k2].as<double>(), f->dims[k2], f->R);
             k::col_sumsq(st, f->U[k2].as<double>(), f->dims[k2], f->R, sg.as<double>(), f->part);
             k::scale_cols(st, f->U[k2].as<double>(), f->dims[k2], f->R, sg.as<double>());
+            f->wrote64(k2);
         }
         std::vector<double> ones(f->R, 1.0);
         h2d(st, f->sigma.p, ones.data(), (size_t)f->R * 8);
@@ -680,6 +793,7 @@ int ex_factors_gram(ex_factors* f, int mode, double* G) {
         EX_CHECK(mode >= 0 && mode < f->N, EX_ERR_ARG, "mode out of range");
         hipStream_t st = f->ctx->c.stream;
         f->gram.ensure((size_t)f->R * f->R * 8);
+        f->sync64(mode);
         k::gram(st, f->U[mode].as<double>(), f->dims[mode], f->R, f->gram.as<double>(), f->part);
         d2h(st, G, f->gram.p, (size_t)f->R * f->R * 8);
     });
@@ -690,7 +804,7 @@ int ex_sampler_create(ex_ctx* ctx, ex_factors* f, int kind, int precision, uint6
     return guarded([&] {
         EX_CHECK(ctx && f && out, EX_ERR_ARG, "null argument");
         EX_CHECK(kind == EX_SAMPLER_CP_ARLS_LEV || kind == EX_SAMPLER_STS_CP, EX_ERR_ARG, "unknown sampler kind");
-        EX_CHECK(precision == EX_FP64, EX_ERR_UNSUPPORTED, "only the FP64 sampler is available");
+        EX_CHECK(precision == EX_FP64 || precision == EX_MIXED, EX_ERR_ARG, "unknown precision");
         auto s = std::make_unique<ex_sampler>();
         s->ctx = ctx;
         s->f = f;
@@ -700,12 +814,14 @@ int ex_sampler_create(ex_ctx* ctx, ex_factors* f, int kind, int precision, uint6
         const int N = f->N, R = f->R;
         s->sums.ensure((size_t)N * 8);
         s->part.ensure(4096);
+        if (precision == EX_MIXED) f->enable32();
         if (kind == EX_SAMPLER_STS_CP) {
             const int S = tri_stride(R);
+            const size_t node_bytes = precision == EX_MIXED ? (size_t)tri_stride32(R) * 4 : (size_t)S * 8;
             s->ge.init(R, 1);
             for (int k2 = 0; k2 < N; k2++) {
                 s->g[k2].init(f->dims[k2], R);
-                s->nodes[k2].ensure((size_t)s->g[k2].node_count * S * 8);
+                s->nodes[k2].ensure((size_t)s->g[k2].node_count * node_bytes);
             }
             s->grams.ensure((size_t)N * R * R * 8);
             s->M.ensure((size_t)(N + 2) * R * R * 8);
@@ -769,7 +885,8 @@ int ex_sts_compute_m(ex_sampler* s, int mode, double* M) {
         EX_CHECK(mode >= 0 && mode < s->f->N, EX_ERR_ARG, "mode out of range");
         const int N = s->f->N, R = s->f->R;
         hipStream_t st = s->ctx->c.stream;
-        k::sts_compute_m(st, N, mode, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
+        if (s->precision == EX_MIXED) k::sts_compute_m_mixed_launch(st, N, mode, R, s->grams.as<double>(), s->M.as<double>());
+        else k::sts_compute_m(st, N, mode, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
         d2h(st, M, s->M.p, (size_t)(N + 2) * R * R * 8);
     });
 }

@@ -73,6 +73,8 @@ __host__ __device__ inline int tri_size(int R) { return R * (R + 1) / 2; }
 __host__ __device__ inline int tri_index(int R, int a, int b) { return a * R - (a * (a - 1)) / 2 + (b - a); }
 // Node stride in elements: packed size rounded up to 16 elements (128 B in FP64).
 __host__ __device__ inline int tri_stride(int R) { return (tri_size(R) + 15) & ~15; }
+// FP32 node stride: packed size rounded up to 32 elements (128 B).
+__host__ __device__ inline int tri_stride32(int R) { return (tri_size(R) + 31) & ~31; }
 
 // Heap-ordered segment tree over ceil(n/F) leaves (PartitionTree geometry,
 // partition_tree.hpp:82-115).
@@ -114,6 +116,7 @@ struct ModeLookup {
     DevBuf fptr;   // u32[nfib+1] offsets into rowj/val
     DevBuf rowj;   // u32[nnz] target-mode coordinate
     DevBuf val;    // f64[nnz]
+    DevBuf val32;  // f32[nnz], built on first use by the mixed-precision path
 };
 
 struct Ctx {
@@ -150,16 +153,18 @@ void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double
 void sts_compute_m(hipStream_t s, int N, int j, int R, const double* grams, double* M, double* lam);
 void eig_tree_build(hipStream_t s, int N, int j, int R, const double* M, const double* grams, double* eig_nodes);
 struct StsModeArgs {
-    const double* rows;    // factor U_k (I_k x R)
-    const double* nodes;   // packed row-tree nodes
+    const void* rows;      // factor U_k (I_k x R), double or float as the nodes
+    const void* nodes;     // packed row-tree nodes (stride tri_stride / tri_stride32)
     const double* eig;     // packed eigen-tree nodes (2R-1)
     const double* w;       // R x R scaled eigenvectors, rows = components
     TreeGeom g;
     TreeGeom ge;
 };
+// node_bytes = 8: FP64 rows and row-tree nodes; 4: FP32 (the mixed path).
 void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* modes,
               const double* pinv_packed, uint64_t seed, uint64_t sweep,
-              const double* draws_row, const double* draws_eig, uint32_t* samples, double* weights);
+              const double* draws_row, const double* draws_eig, uint32_t* samples, double* weights,
+              int node_bytes = 8);
 void pack_upper(hipStream_t s, const double* M, int R, double* packed);
 // sparse
 void keys_from_coords(hipStream_t s, const uint32_t* coords, int64_t n, const KeyLayout& kl, uint64_t* keys,
@@ -172,11 +177,28 @@ int64_t dedup(hipStream_t s, const uint32_t* samples, const double* weights, int
               uint32_t* sd, double* wd, uint64_t* kd, DevBuf* tmp);
 void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
             const double* const* U, double* d_gram, double* d_mttkrp);
-int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const uint64_t* kd, int64_t Ju, const double* design,
-                       int R, double* Y, int64_t I_j, DevBuf* tmp, int64_t* T_out = nullptr,
+// V = double (FP64 path) or float (mixed path): nonzero values, design rows and Y.
+template <typename V>
+int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju,
+                       const V* design, int R, V* Y, int64_t I_j, DevBuf* tmp, int64_t* T_out = nullptr,
                        hipEvent_t* seg_events = nullptr);
 void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
              const double* sigma, int R, double* out, DevBuf& part);
+// final stages shared with the mixed-precision kernels
+void gram_final_launch(hipStream_t s, const double* part, int nb, int R, double* G);
+void col_sumsq_final_launch(hipStream_t s, const double* part, int nb, int R, double* sigma);
+// mixed precision (FP32 data, BF16 MFMA Grams, FP64 R x R solves)
+void gram_bf16(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part);
+void tree_build_f32(hipStream_t s, const float* U, const TreeGeom& g, int R, float* nodes);
+void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, float* out32);
+void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M);
+void to_f32(hipStream_t s, const double* a, int64_t n, float* b);
+void to_f64(hipStream_t s, const float* a, int64_t n, double* b);
+void design32(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
+              const float* const* U, float* dg, float* dm);
+void dsymm32(hipStream_t s, const float* G, const float* Y, int64_t m, int R, float* out);
+void renormalize32(hipStream_t s, float* U, int64_t m, int R, double* sigma, DevBuf& part);
+void dagat32(hipStream_t s, const float* A, int64_t m, int R, const double* G, double* res);
 // sort / scan primitives (rocPRIM)
 size_t sort_pairs_u64_u32(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* kin, uint64_t* kout,
                           const uint32_t* vin, uint32_t* vout, int64_t n, int bits);

@@ -75,4 +75,11 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+__device__ __forceinline__ float readlane_f32(float v, int l) {
+    return __uint_as_float(readlane_u32(__float_as_uint(v), l));
+}
+
+__device__ __forceinline__ double readlane_val(double v, int l) { return readlane_f64(v, l); }
+__device__ __forceinline__ float readlane_val(float v, int l) { return readlane_f32(v, l); }
+
 }  // namespace exafac

@@ -655,7 +655,17 @@ void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& p
     int nb = (int)grid_for(m, kGramRows, kGramMaxBlocks);
     part.ensure((size_t)nb * R * R * sizeof(double));
     hipLaunchKernelGGL(gram_partial, dim3(nb), dim3(kThreads), 0, s, A, m, R, part.as<double>());
-    hipLaunchKernelGGL(gram_final, dim3((R * R + 3) / 4), dim3(kThreads), 0, s, part.as<double>(), nb, R, G);
+    EX_HIP(hipGetLastError());
+    gram_final_launch(s, part.as<double>(), nb, R, G);
+}
+
+void gram_final_launch(hipStream_t s, const double* part, int nb, int R, double* G) {
+    hipLaunchKernelGGL(gram_final, dim3((R * R + 3) / 4), dim3(kThreads), 0, s, part, nb, R, G);
+    EX_HIP(hipGetLastError());
+}
+
+void col_sumsq_final_launch(hipStream_t s, const double* part, int nb, int R, double* sigma) {
+    hipLaunchKernelGGL(col_sumsq_final, dim3((R + 3) / 4), dim3(kThreads), 0, s, part, nb, R, sigma);
     EX_HIP(hipGetLastError());
 }
 
@@ -696,8 +706,8 @@ void col_sumsq(hipStream_t s, const double* U, int64_t m, int R, double* sigma, 
     int nb = (int)grid_for(m, 1024, 1024);
     part.ensure((size_t)nb * R * sizeof(double));
     hipLaunchKernelGGL(col_sumsq_partial, dim3(nb), dim3(kThreads), 0, s, U, m, R, part.as<double>());
-    hipLaunchKernelGGL(col_sumsq_final, dim3((R + 3) / 4), dim3(kThreads), 0, s, part.as<double>(), nb, R, sigma);
     EX_HIP(hipGetLastError());
+    col_sumsq_final_launch(s, part.as<double>(), nb, R, sigma);
 }
 
 void scale_cols(hipStream_t s, double* U, int64_t m, int R, const double* sigma) {

@@ -1,4 +1,5 @@
-// Sampler kernels (FP64): STS-CP segment trees and descent, CP-ARLS-LEV CDF
+// Sampler kernels: STS-CP segment trees and descent (FP64, or FP32 row trees on
+// the mixed-precision path), CP-ARLS-LEV CDF
 // and inverse-CDF draw.
 //
 // Reference semantics: partition_tree.hpp:117-380 (build_tree, PTSample),
@@ -98,40 +99,44 @@ struct PairView {
         }
     }
     // xx[t] = w * x_a * x_b for the query x (held by lanes 0..R-1)
-    __device__ void outer(double x, double (&xx)[T]) const {
+    template <typename TX>
+    __device__ void outer(TX x, TX (&xx)[T]) const {
 #pragma unroll
-        for (int t = 0; t < T; t++) xx[t] = w[t] * (__shfl(x, a[t], 64) * __shfl(x, b[t], 64));
+        for (int t = 0; t < T; t++) xx[t] = (TX)w[t] * (__shfl(x, a[t], 64) * __shfl(x, b[t], 64));
     }
 };
 
-template <int T>
-__device__ __forceinline__ double quad(const double* __restrict__ node, const double (&xx)[T], int lane, int P) {
-    double acc = 0.0;
+// x^T G x for a packed node: per-lane partial in the node's precision, the
+// wavefront total in FP64.
+template <typename TN, int T>
+__device__ __forceinline__ double quad(const TN* __restrict__ node, const TN (&xx)[T], int lane, int P) {
+    TN acc = 0;
 #pragma unroll
     for (int t = 0; t < T; t++) {
         int e = lane + 64 * t;
         if (e < P) acc += node[e] * xx[t];
     }
-    return wave_sum(acc);
+    return wave_sum((double)acc);
 }
 
 // Root-to-leaf descent of one heap tree for the query held in xx; returns the
 // leaf node and the (low, high) draw interval (partition_tree.hpp:226-307).
-template <int T>
-__device__ __forceinline__ int64_t descend(const double* __restrict__ nodes, const TreeGeom& g, int S, int P,
-                                           const double (&xx)[T], int lane, double u, double& low, double& high) {
-    const double m = quad<T>(nodes, xx, lane, P);
+// Masses and cutoffs are FP64 whatever the node precision.
+template <typename TN, int T>
+__device__ __forceinline__ int64_t descend(const TN* __restrict__ nodes, const TreeGeom& g, int S, int P,
+                                           const TN (&xx)[T], int lane, double u, double& low, double& high) {
+    const double m = quad<TN, T>(nodes, xx, lane, P);
     int64_t c = 0;
     low = 0.0;
     high = 1.0;
     for (uint32_t lvl = 0; lvl < g.lfill_level; lvl++) {
-        double mL = quad<T>(nodes + (2 * c + 1) * S, xx, lane, P);
+        double mL = quad<TN, T>(nodes + (2 * c + 1) * S, xx, lane, P);
         double cutoff = low + mL / m;
         if (u <= cutoff) { c = 2 * c + 1; high = cutoff; }
         else { c = 2 * c + 2; low = cutoff; }
     }
     if (!g.is_leaf(c)) {
-        double mL = quad<T>(nodes + (2 * c + 1) * S, xx, lane, P);
+        double mL = quad<TN, T>(nodes + (2 * c + 1) * S, xx, lane, P);
         double cutoff = low + mL / m;
         if (u <= cutoff) { c = 2 * c + 1; high = cutoff; }
         else { c = 2 * c + 2; low = cutoff; }
@@ -139,33 +144,41 @@ __device__ __forceinline__ int64_t descend(const double* __restrict__ nodes, con
     return c;
 }
 
-template <int T>
+// STS-CP KRPDrawSamples for one sample per wavefront, every non-target mode in
+// turn (sts_cp.hpp:228-286).  TN: row-tree node and factor precision (double
+// on the plain path, float on the mixed path); the eigen trees, the scaled
+// eigenvectors and the final weight stay FP64.
+template <typename TN, int T>
 __global__ __launch_bounds__(kThreads) void sts_draw_kernel(StsArgs a) {
     const int lane = threadIdx.x % 64;
     const int64_t s = (int64_t)blockIdx.x * kWaves + threadIdx.x / 64;
     if (s >= a.J) return;   // whole wavefront leaves together
     const int R = a.R, N = a.N, j = a.j;
-    const int P = tri_size(R), S = tri_stride(R);
+    const int P = tri_size(R), Se = tri_stride(R);
+    const int S = sizeof(TN) == 8 ? tri_stride(R) : tri_stride32(R);
     PairView<T> pv;
     pv.init(R, lane);
-    double xx[T];
-    double h = lane < R ? 1.0 : 0.0;
+    double xd[T];
+    TN xn[T];
+    TN h = lane < R ? (TN)1 : (TN)0;
     uint32_t eig_idx = 0;
     for (int k = 0; k < N; k++) {
         if (k == j) continue;
         const k::StsModeArgs& md = a.m[k];
+        const TN* rows_k = static_cast<const TN*>(md.rows);
+        const TN* nodes_k = static_cast<const TN*>(md.nodes);
         // eigen tree: query h, F = 1
-        pv.outer(h, xx);
+        pv.outer((double)h, xd);
         double ue = a.draws_eig ? a.draws_eig[(int64_t)k * a.J + s] : uniform01(a.seed, stream_id(a.sweep, j, k, 1), s);
         double low, high;
-        int64_t c = descend<T>(md.eig, md.ge, S, P, xx, lane, ue, low, high);
+        int64_t c = descend<double, T>(md.eig, md.ge, Se, P, xd, lane, ue, low, high);
         const int64_t v = md.ge.leaf_idx(c);
         eig_idx = (uint32_t)v;
-        double sh = lane < R ? h * md.w[v * R + lane] : 0.0;
+        TN sh = lane < R ? (TN)((double)h * md.w[v * R + lane]) : (TN)0;
         // row tree: query sh, F = R
-        pv.outer(sh, xx);
+        pv.outer(sh, xn);
         double ur = a.draws_row ? a.draws_row[(int64_t)k * a.J + s] : uniform01(a.seed, stream_id(a.sweep, j, k, 2), s);
-        c = descend<T>(md.nodes, md.g, S, P, xx, lane, ur, low, high);
+        c = descend<TN, T>(nodes_k, md.g, S, P, xn, lane, ur, low, high);
         const int64_t leaf = md.g.leaf_idx(c);
         const int64_t F = md.g.F;
         const int rows = (int)((md.g.n - leaf * F) < F ? (md.g.n - leaf * F) : F);
@@ -174,23 +187,24 @@ __global__ __launch_bounds__(kThreads) void sts_draw_kernel(StsArgs a) {
         // on every lane (a shuffle from an inactive lane is undefined), and
         // lanes past the last real row read row 0 of the leaf and drop it.
         const bool real_row = lane < rows;
-        const double* ur_row = md.rows + (leaf * F + (real_row ? lane : 0)) * R;
-        double q = 0.0;
+        const TN* ur_row = rows_k + (leaf * F + (real_row ? lane : 0)) * R;
+        TN q = 0;
         for (int b = 0; b < R; b++) q += ur_row[b] * __shfl(sh, b, 64);
-        if (!real_row) q = 0.0;
-        double cum = wave_inclusive_scan(q * q, lane);
+        if (!real_row) q = 0;
+        const double qd = (double)q;
+        double cum = wave_inclusive_scan(qd * qd, lane);
         double total = __shfl(cum, (int)F - 1, 64);
         bool cond = lane < (int)F - 1 && frac < cum / total;
         uint64_t mask = __ballot(cond);
         int64_t res = mask ? (int64_t)__ffsll((unsigned long long)mask) - 1 : F - 1;
         if (res >= rows) res = rows - 1;
         const int64_t row = leaf * F + res;
-        if (lane < R) h *= md.rows[row * R + lane];
+        if (lane < R) h *= rows_k[row * R + lane];
         if (lane == 0) a.samples[s * N + k] = (uint32_t)row;
     }
-    // weight = R / (J * h^T pinv h)
-    pv.outer(h, xx);
-    double q = quad<T>(a.pinvp, xx, lane, P);
+    // weight = R / (J * h^T pinv h), FP64
+    pv.outer((double)h, xd);
+    double q = quad<double, T>(a.pinvp, xd, lane, P);
     if (lane == 0) {
         a.samples[s * N + j] = eig_idx;
         a.weights[s] = (double)R / (q * (double)a.J);
@@ -330,7 +344,8 @@ void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double
 
 void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* modes, const double* pinv_packed,
               uint64_t seed, uint64_t sweep, const double* draws_row, const double* draws_eig, uint32_t* samples,
-              double* weights) {
+              double* weights, int node_bytes) {
+    EX_CHECK(node_bytes == 8 || node_bytes == 4, 1, "node precision must be FP64 or FP32");
     StsArgs a;
     for (int k = 0; k < N; k++) a.m[k] = modes[k];
     a.N = N; a.j = j; a.R = R; a.J = J;
@@ -340,12 +355,21 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
     a.samples = samples; a.weights = weights;
     unsigned nb = grid_for(J, kWaves, 1LL << 30);
     const int P = tri_size(R);
-    if (P <= 64 * 3)
-        hipLaunchKernelGGL(sts_draw_kernel<3>, dim3(nb), dim3(kThreads), 0, s, a);
-    else if (P <= 64 * 9)
-        hipLaunchKernelGGL(sts_draw_kernel<9>, dim3(nb), dim3(kThreads), 0, s, a);
-    else
-        hipLaunchKernelGGL(sts_draw_kernel<33>, dim3(nb), dim3(kThreads), 0, s, a);
+    if (node_bytes == 8) {
+        if (P <= 64 * 3)
+            hipLaunchKernelGGL((sts_draw_kernel<double, 3>), dim3(nb), dim3(kThreads), 0, s, a);
+        else if (P <= 64 * 9)
+            hipLaunchKernelGGL((sts_draw_kernel<double, 9>), dim3(nb), dim3(kThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL((sts_draw_kernel<double, 33>), dim3(nb), dim3(kThreads), 0, s, a);
+    } else {
+        if (P <= 64 * 3)
+            hipLaunchKernelGGL((sts_draw_kernel<float, 3>), dim3(nb), dim3(kThreads), 0, s, a);
+        else if (P <= 64 * 9)
+            hipLaunchKernelGGL((sts_draw_kernel<float, 9>), dim3(nb), dim3(kThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL((sts_draw_kernel<float, 33>), dim3(nb), dim3(kThreads), 0, s, a);
+    }
     EX_HIP(hipGetLastError());
 }
 

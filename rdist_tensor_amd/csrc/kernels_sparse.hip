@@ -199,12 +199,13 @@ __global__ __launch_bounds__(kThreads) void expand_hits(const uint32_t* __restri
 // chunk (the touched rows).
 constexpr int kSegChunk = 256;
 
+template <typename V>
 __global__ __launch_bounds__(kThreads) void seg_chunk_reduce(const uint32_t* __restrict__ hrow,
                                                              const uint64_t* __restrict__ hpay,
-                                                             const double* __restrict__ val,
-                                                             const double* __restrict__ design, int R, int64_t H,
-                                                             int64_t nch, double* __restrict__ Y,
-                                                             double* __restrict__ carry, uint32_t* __restrict__ cflag,
+                                                             const V* __restrict__ val,
+                                                             const V* __restrict__ design, int R, int64_t H,
+                                                             int64_t nch, V* __restrict__ Y,
+                                                             V* __restrict__ carry, uint32_t* __restrict__ cflag,
                                                              unsigned long long* __restrict__ T) {
     const int lane = threadIdx.x % 64;
     for (int64_t w = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64; w < nch;
@@ -212,7 +213,7 @@ __global__ __launch_bounds__(kThreads) void seg_chunk_reduce(const uint32_t* __r
         const int64_t b = w * kSegChunk, e = b + kSegChunk < H ? b + kSegChunk : H;
         // the chunk's hits, 4 per lane: row, sample index, nonzero value
         uint32_t hr[kSegChunk / 64], hc[kSegChunk / 64];
-        double hx[kSegChunk / 64];
+        V hx[kSegChunk / 64];
 #pragma unroll
         for (int m = 0; m < kSegChunk / 64; m++) {
             const int64_t h = b + 64 * m + lane;
@@ -224,13 +225,13 @@ __global__ __launch_bounds__(kThreads) void seg_chunk_reduce(const uint32_t* __r
             } else {
                 hr[m] = 0;
                 hc[m] = 0;
-                hx[m] = 0.0;
+                hx[m] = (V)0;
             }
         }
         uint32_t cur = readlane_u32(hr[0], 0);
         bool cur_head = (b == 0) || hrow[b - 1] != cur;
         uint32_t f0 = 0, f1 = 0, heads = 0;
-        double acc = 0.0;
+        V acc = 0;
         const int lr = lane < R ? lane : 0;
 #pragma unroll
         for (int m = 0; m < kSegChunk / 64; m++) {
@@ -247,11 +248,11 @@ __global__ __launch_bounds__(kThreads) void seg_chunk_reduce(const uint32_t* __r
                         f0 = 1;
                     }
                     cur = r;
-                    acc = 0.0;
+                    acc = 0;
                     cur_head = true;
                 }
                 const uint32_t c = readlane_u32(hc[m], l);
-                acc += design[(int64_t)c * R + lr] * readlane_f64(hx[m], l);
+                acc += design[(int64_t)c * R + lr] * readlane_val(hx[m], l);
             }
         }
         const bool ends = (e == H) || hrow[e] != cur;
@@ -273,17 +274,18 @@ __global__ __launch_bounds__(kThreads) void seg_chunk_reduce(const uint32_t* __r
     }
 }
 
+template <typename V>
 __global__ __launch_bounds__(kThreads) void seg_fixup(const uint32_t* __restrict__ hrow, int64_t H, int64_t nch,
-                                                      const double* __restrict__ carry,
+                                                      const V* __restrict__ carry,
                                                       const uint32_t* __restrict__ cflag, int R,
-                                                      double* __restrict__ Y) {
+                                                      V* __restrict__ Y) {
     const int lane = threadIdx.x % 64;
     for (int64_t w = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64; w < nch;
          w += (int64_t)gridDim.x * (kThreads / 64)) {
         if (!cflag[2 * w + 1]) continue;
         const int64_t ew = (w + 1) * kSegChunk;   // < H: the row runs past this chunk
         const uint32_t row = hrow[ew - 1];
-        double total = lane < R ? carry[(2 * w + 1) * R + lane] : 0.0;
+        V total = lane < R ? carry[(2 * w + 1) * R + lane] : (V)0;
         for (int64_t k = w + 1; k < nch; k++) {
             if (lane < R) total += carry[(2 * k) * R + lane];
             const int64_t ek = (k + 1) * kSegChunk;
@@ -452,9 +454,11 @@ void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int
 }
 
 // Y (I_j x R) is zero-filled here; returns the number of hits H.
-int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const uint64_t* kd, int64_t Ju, const double* design, int R,
-                       double* Y, int64_t I_j, DevBuf* tmp, int64_t* T_out, hipEvent_t* seg_events) {
-    EX_HIP(hipMemsetAsync(Y, 0, (size_t)I_j * R * sizeof(double), s));
+template <typename V>
+int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju,
+                       const V* design, int R, V* Y, int64_t I_j, DevBuf* tmp, int64_t* T_out,
+                       hipEvent_t* seg_events) {
+    EX_HIP(hipMemsetAsync(Y, 0, (size_t)I_j * R * sizeof(V), s));
     if (T_out) *T_out = 0;
     if (Ju == 0 || L.nfib == 0) return 0;
     // stage 1: fiber ranges and offsets
@@ -493,20 +497,25 @@ int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const uint64_t* kd, i
                        L.rowj.as<uint32_t>(), r0, p0);
     sort_pairs_u32_u64(s, c2 + o_t2, sortH, r0, r1, p0, p1, H, rbits);
     const int64_t nch = (H + kSegChunk - 1) / kSegChunk;
-    tmp[3].ensure((size_t)nch * 2 * R * sizeof(double) + (size_t)nch * 2 * 4 + 64);
-    double* carry = tmp[3].as<double>();
+    tmp[3].ensure((size_t)nch * 2 * R * sizeof(V) + (size_t)nch * 2 * 4 + 64);
+    V* carry = tmp[3].as<V>();
     uint32_t* cflag = (uint32_t*)(carry + nch * 2 * R);
     if (T_out) EX_HIP(hipMemsetAsync(dT, 0, 8, s));
     if (seg_events) EX_HIP(hipEventRecord(seg_events[0], s));
     const unsigned nb = grid_for(nch, kThreads / 64, 65536);
-    hipLaunchKernelGGL(seg_chunk_reduce, dim3(nb), dim3(kThreads), 0, s, r1, p1, L.val.as<double>(), design, R, H, nch,
-                       Y, carry, cflag, T_out ? (unsigned long long*)dT : nullptr);
-    hipLaunchKernelGGL(seg_fixup, dim3(nb), dim3(kThreads), 0, s, r1, H, nch, carry, cflag, R, Y);
+    hipLaunchKernelGGL(seg_chunk_reduce<V>, dim3(nb), dim3(kThreads), 0, s, r1, p1, vals, design, R, H, nch, Y, carry,
+                       cflag, T_out ? (unsigned long long*)dT : nullptr);
+    hipLaunchKernelGGL(seg_fixup<V>, dim3(nb), dim3(kThreads), 0, s, r1, H, nch, carry, cflag, R, Y);
     if (seg_events) EX_HIP(hipEventRecord(seg_events[1], s));
     EX_HIP(hipGetLastError());
     if (T_out) *T_out = (int64_t)read_u64(s, dT);
     return H;
 }
+
+template int64_t sampled_mttkrp<double>(hipStream_t, const ModeLookup&, const double*, const uint64_t*, int64_t,
+                                        const double*, int, double*, int64_t, DevBuf*, int64_t*, hipEvent_t*);
+template int64_t sampled_mttkrp<float>(hipStream_t, const ModeLookup&, const float*, const uint64_t*, int64_t,
+                                       const float*, int, float*, int64_t, DevBuf*, int64_t*, hipEvent_t*);
 
 void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
              const double* sigma, int R, double* out, DevBuf& part) {

@@ -1,0 +1,159 @@
+"""Mixed-precision path (FP32 data, BF16 MFMA Grams with FP32 accumulation,
+FP64 R x R solves) against FP64 closed forms and against the FP64 path.
+
+Tolerances: BF16 keeps 8 significand bits, so a Gram entry built from BF16
+operands carries a relative error of order 2^-9 / sqrt(terms) after FP32
+accumulation; quantities derived from Grams (inverses, leverage scores) are
+checked at 2e-2 relative (Frobenius / max-relative), FP32 data paths at 1e-5.
+The end-to-end bar is the final fit: within 1e-3 of the FP64 path's.
+"""
+import numpy as np
+import pytest
+
+from conftest import normalized_factors, random_tensor
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def set_factors(ctx, dims, R, U):
+    from rdist_tensor_amd import abi
+    F = abi.Factors(ctx, dims, R)
+    for k in range(len(dims)):
+        F.set(k, U[k])
+    return F
+
+
+@pytest.mark.parametrize("R", [8, 16, 32, 64])
+def test_mixed_arls_scores(ctx, R):
+    from rdist_tensor_amd import abi
+    rng = np.random.default_rng(300 + R)
+    dims = [5000, 3000, 1001]
+    U = normalized_factors(rng, dims, R)
+    F = set_factors(ctx, dims, R, U)
+    S = abi.Sampler(ctx, F, abi.EX_SAMPLER_CP_ARLS_LEV, precision=abi.EX_MIXED)
+    for k in range(3):
+        lev = np.einsum("ir,rs,is->i", U[k], np.linalg.pinv(U[k].T @ U[k]), U[k])
+        got = S.arls_scores(k)
+        err = np.max(np.abs(got / lev - 1.0))
+        print(f"R={R} mode {k}: leverage max rel err {err:.2e}, sum {got.sum():.6f} (rank {R})")
+        assert err < 2e-2
+        assert abs(got.sum() - R) < 1e-2 * R
+
+
+@pytest.mark.parametrize("R", [8, 16, 32, 64])
+def test_mixed_compute_m_factorisations(ctx, R):
+    """Slot 0 = (Hadamard of the non-target Grams)^+, slot N rows w_v with
+    sum_v w_v w_v^T = slot 0, slot k rows with sum_v w_v w_v^T equal to
+    (Hadamard of the Grams of modes >= k, != j) o slot 0, slot N+1 the
+    Hadamard product itself (sts_cp.hpp:80-188)."""
+    from rdist_tensor_amd import abi
+    rng = np.random.default_rng(400 + R)
+    dims = [2000, 1500, 1200, 900]
+    N = len(dims)
+    U = normalized_factors(rng, dims, R)
+    F = set_factors(ctx, dims, R, U)
+    S = abi.Sampler(ctx, F, abi.EX_SAMPLER_STS_CP, precision=abi.EX_MIXED)
+    G = [u.T @ u for u in U]
+    for j in range(N):
+        M = S.sts_m(j)
+        H = np.ones((R, R))
+        for k in range(N):
+            if k != j:
+                H = H * G[k]
+        Hi = np.linalg.pinv(H)
+        assert rel_err(M[N + 1], H) < 2e-2, j
+        assert rel_err(M[0], Hi) < 2e-2, j
+        assert rel_err(M[N].T @ M[N], M[0]) < 1e-10, j
+        for k in range(1, N):
+            if k == j:
+                continue
+            T = np.ones((R, R))
+            for m in range(k, N):
+                if m != j:
+                    T = T * G[m]
+            target = T * Hi
+            W = M[k]
+            assert rel_err(W.T @ W, target) < 2e-2, (j, k)
+
+
+def planted_tensor(rng, dims, R, noise):
+    """A dense tensor (every coordinate a nonzero) of CP rank R plus noise;
+    returns the coordinates, the values and the planted factors."""
+    A = [rng.standard_normal((d, R)) for d in dims]
+    grid = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1).reshape(-1, len(dims))
+    vals = np.ones((len(grid), R))
+    for k in range(len(dims)):
+        vals *= A[k][grid[:, k]]
+    vals = vals.sum(1)
+    vals += noise * np.linalg.norm(vals) / np.sqrt(len(vals)) * rng.standard_normal(len(vals))
+    return grid.astype(np.uint32), vals, A
+
+
+@pytest.mark.parametrize("kind", ["sts", "arls"])
+def test_mixed_fit_within_1e3_of_fp64(ctx, kind):
+    """Both precisions start from the planted factors plus a 30% perturbation
+    (inside the basin of the planted solution: from a random start, randomized
+    ALS on this tensor can sit on plateaus whose level depends on the draws,
+    and the two precisions draw differently) and must reach the same final
+    fit (the noise floor, 0.95) within 1e-3."""
+    from rdist_tensor_amd import abi
+    rng = np.random.default_rng(501)
+    dims = [120, 100, 80]
+    R = 8
+    idx, vals, A0 = planted_tensor(rng, dims, R, noise=0.05)
+    U0 = [a + 0.3 * np.linalg.norm(a, axis=0) / np.sqrt(len(a)) * rng.standard_normal(a.shape) for a in A0]
+    U0 = [u / np.linalg.norm(u, axis=0) for u in U0]
+    sk = abi.EX_SAMPLER_STS_CP if kind == "sts" else abi.EX_SAMPLER_CP_ARLS_LEV
+    fits = {}
+    for prec in (abi.EX_FP64, abi.EX_MIXED):
+        T = abi.Tensor(ctx, idx, vals, dims)
+        F = set_factors(ctx, dims, R, U0)
+        S = abi.Sampler(ctx, F, sk, precision=prec, seed=5)
+        A = abi.ALS(ctx, T, F, S, precision=prec)
+        A.init()
+        hist = []
+        for sweep in range(1, 13):
+            for j in range(3):
+                A.step(j, 8192, sweep)
+            hist.append(A.fit())
+        fits[prec] = hist
+        print(f"{kind} precision {prec}: fits " + " ".join(f"{f:.5f}" for f in hist))
+    f64, fmx = fits[abi.EX_FP64][-1], fits[abi.EX_MIXED][-1]
+    assert f64 > 0.9
+    assert abs(fmx - f64) < 1e-3
+
+
+def test_mixed_factor_copies_stay_consistent(ctx):
+    """Factors written by a mixed step are visible through the FP64 accessors,
+    and factors set from the host are seen by the next mixed step."""
+    from rdist_tensor_amd import abi
+    rng = np.random.default_rng(601)
+    dims = [300, 200, 100]
+    R = 16
+    idx, vals = random_tensor(rng, dims, 20000)
+    U = normalized_factors(rng, dims, R)
+    T = abi.Tensor(ctx, idx, vals, dims)
+    F = set_factors(ctx, dims, R, U)
+    S = abi.Sampler(ctx, F, abi.EX_SAMPLER_STS_CP, precision=abi.EX_MIXED, seed=3)
+    A = abi.ALS(ctx, T, F, S, precision=abi.EX_MIXED)
+    A.init()
+    A.step(0, 4096, 1)
+    U0 = F.get(0)
+    # renormalised columns, FP32-representable values
+    assert np.allclose(np.linalg.norm(U0, axis=0), 1.0, atol=1e-5)
+    assert np.array_equal(U0.astype(np.float32).astype(np.float64), U0)
+    # a host write of mode 1 reaches the mixed path: the sampler's FP32 tree
+    # for mode 1 is rebuilt from the new factor
+    V = normalized_factors(rng, [dims[1]], R)[0]
+    F.set(1, V)
+    S.update(1)
+    A.step(0, 4096, 2)
+    assert np.allclose(F.get(1), V)
+    assert np.isfinite(A.fit())
