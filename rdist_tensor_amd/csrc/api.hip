@@ -220,7 +220,7 @@ struct ex_sampler {
     TreeGeom g[kMaxModes];
     TreeGeom ge;
     DevBuf nodes[kMaxModes];
-    DevBuf grams, M, lam, eig, pinvp;
+    DevBuf grams, M, lam, eig, eig32, pinvp;
     DevBuf part;
     // draw outputs and injected draws
     DevBuf samples, weights, draws_row, draws_eig;
@@ -318,16 +318,17 @@ void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const do
         if (mixed) k::sts_compute_m_mixed_launch(st, N, j, R, s->grams.as<double>(), s->M.as<double>());
         else k::sts_compute_m(st, N, j, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
         k::eig_tree_build(st, N, j, R, s->M.as<double>(), s->grams.as<double>(), s->eig.as<double>());
+        if (mixed) k::eig_tree_to_f32(st, N, j, R, s->eig.as<double>(), s->eig32.as<float>());
         k::pack_upper(st, s->M.as<double>(), R, s->pinvp.as<double>());
         sm.end();
         k::StsModeArgs m[kMaxModes];
-        const int S = tri_stride(R);
         if (mixed) f->sync32_all();
         else f->sync64_all();
         for (int k2 = 0; k2 < N; k2++) {
             m[k2].rows = mixed ? (const void*)f->U32[k2].p : (const void*)f->U[k2].p;
             m[k2].nodes = s->nodes[k2].p;
-            m[k2].eig = s->eig.as<double>() + (int64_t)k2 * s->ge.node_count * S;
+            if (mixed) m[k2].eig = s->eig32.as<float>() + (int64_t)k2 * s->ge.node_count * tri_stride32(R);
+            else m[k2].eig = s->eig.as<double>() + (int64_t)k2 * s->ge.node_count * tri_stride(R);
             int off = (k2 + 1 == j) ? k2 + 2 : k2 + 1;
             m[k2].w = s->M.as<double>() + (int64_t)off * R * R;
             m[k2].g = s->g[k2];
@@ -827,6 +828,7 @@ int ex_sampler_create(ex_ctx* ctx, ex_factors* f, int kind, int precision, uint6
             s->M.ensure((size_t)(N + 2) * R * R * 8);
             s->lam.ensure((size_t)(N + 1) * R * 8);
             s->eig.ensure((size_t)N * s->ge.node_count * S * 8);
+            if (precision == EX_MIXED) s->eig32.ensure((size_t)N * s->ge.node_count * tri_stride32(R) * 4);
             s->pinvp.ensure((size_t)S * 8);
         }
         for (int k2 = 0; k2 < N; k2++) sampler_update_impl(s.get(), k2);

@@ -152,10 +152,11 @@ void arls_draw(hipStream_t s, int N, int j, int64_t J, const double* const* cdfs
 void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double* nodes);
 void sts_compute_m(hipStream_t s, int N, int j, int R, const double* grams, double* M, double* lam);
 void eig_tree_build(hipStream_t s, int N, int j, int R, const double* M, const double* grams, double* eig_nodes);
+void eig_tree_to_f32(hipStream_t s, int N, int j, int R, const double* eig, float* eig32);
 struct StsModeArgs {
     const void* rows;      // factor U_k (I_k x R), double or float as the nodes
     const void* nodes;     // packed row-tree nodes (stride tri_stride / tri_stride32)
-    const double* eig;     // packed eigen-tree nodes (2R-1)
+    const void* eig;       // packed eigen-tree nodes (2R-1), same precision and stride as the row tree
     const double* w;       // R x R scaled eigenvectors, rows = components
     TreeGeom g;
     TreeGeom ge;

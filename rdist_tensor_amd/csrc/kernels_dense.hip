@@ -443,44 +443,62 @@ __global__ __launch_bounds__(kJacobiThreads) void sts_compute_m_kernel(int N, in
 __global__ __launch_bounds__(kThreads) void eig_tree_kernel(int N, int j, int R, const double* __restrict__ M,
                                                             const double* __restrict__ grams, TreeGeom ge,
                                                             double* __restrict__ eig_nodes) {
+    __shared__ uint16_t pr[kMaxRank * (kMaxRank + 1) / 2];
     const int k = blockIdx.x;
     if (k == j || k >= N) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
     const int R2 = R * R, P = tri_size(R), S = tri_stride(R);
+    for (int e = tid; e < P; e += nt) {
+        int a = 0;
+        while (a + 1 < R && tri_index(R, a + 1, a + 1) <= e) a++;
+        pr[e] = (uint16_t)(a | ((a + (e - tri_index(R, a, a))) << 8));
+    }
+    __syncthreads();
     const int off = (k + 1 == j) ? k + 2 : k + 1;
     const double* W = M + (int64_t)off * R2;
     double* T = eig_nodes + (int64_t)k * ge.node_count * S;
-    const int64_t first_leaf = (int64_t)ge.node_count - ge.leaf_count;
-    for (int64_t o = threadIdx.x; o < (int64_t)ge.leaf_count * P; o += blockDim.x) {
-        int64_t i = o / P;
-        int e = (int)(o % P);
-        int a = 0;
-        while (a + 1 < R && tri_index(R, a + 1, a + 1) <= e) a++;
-        int b = a + (e - tri_index(R, a, a));
-        int64_t c = first_leaf + i;
-        int64_t v = ge.leaf_idx(c);
-        T[c * S + e] = W[v * R + a] * W[v * R + b];
+    const int first_leaf = (int)ge.node_count - (int)ge.leaf_count;
+    // leaves: outer products of the rows of W (padding of the node stride zeroed)
+    for (int i = 0; i < (int)ge.leaf_count; i++) {
+        const int c = first_leaf + i;
+        const int v = (int)ge.leaf_idx(c);
+        for (int e = tid; e < S; e += nt) {
+            double x = 0.0;
+            if (e < P) {
+                const int a = pr[e] & 0xFF, b = pr[e] >> 8;
+                x = W[v * R + a] * W[v * R + b];
+            }
+            T[(int64_t)c * S + e] = x;
+        }
     }
     __syncthreads();
-    int64_t start = ge.nodes_before_lfill, end = first_leaf;
+    int start = (int)ge.nodes_before_lfill, end = first_leaf;
     for (int lvl = (int)ge.lfill_level; lvl >= 0; lvl--) {
-        for (int64_t o = threadIdx.x; o < (end - start) * P; o += blockDim.x) {
-            int64_t c = start + o / P;
-            int e = (int)(o % P);
-            T[c * S + e] = 0.0 + (T[(2 * c + 1) * S + e] + T[(2 * c + 2) * S + e]);
-        }
+        for (int c = start; c < end; c++)
+            for (int e = tid; e < S; e += nt)
+                T[(int64_t)c * S + e] = 0.0 + (T[(int64_t)(2 * c + 1) * S + e] + T[(int64_t)(2 * c + 2) * S + e]);
         __syncthreads();
         end = start;
         start = ((start + 1) / 2) - 1;
     }
     const double* G = grams + (int64_t)k * R2;
-    for (int64_t o = threadIdx.x; o < (int64_t)ge.node_count * P; o += blockDim.x) {
-        int64_t c = o / P;
-        int e = (int)(o % P);
-        int a = 0;
-        while (a + 1 < R && tri_index(R, a + 1, a + 1) <= e) a++;
-        int b = a + (e - tri_index(R, a, a));
-        T[c * S + e] *= G[a * R + b];
-    }
+    for (int c = 0; c < (int)ge.node_count; c++)
+        for (int e = tid; e < P; e += nt) {
+            const int a = pr[e] & 0xFF, b = pr[e] >> 8;
+            T[(int64_t)c * S + e] *= G[a * R + b];
+        }
+}
+
+// FP32 copy of the eigen trees (node stride tri_stride32), one block per mode.
+__global__ __launch_bounds__(kThreads) void eig_tree_f32_kernel(int N, int j, int R, int nodes,
+                                                                const double* __restrict__ eig,
+                                                                float* __restrict__ eig32) {
+    const int k = blockIdx.x;
+    if (k == j || k >= N) return;
+    const int P = tri_size(R), S = tri_stride(R), S32 = tri_stride32(R);
+    for (int c = 0; c < nodes; c++)
+        for (int e = threadIdx.x; e < S32; e += blockDim.x)
+            eig32[((int64_t)k * nodes + c) * S32 + e] = e < P ? (float)eig[((int64_t)k * nodes + c) * S + e] : 0.0f;
 }
 
 // ---------------------------------------------------------------- row-wise
@@ -756,6 +774,13 @@ void eig_tree_build(hipStream_t s, int N, int j, int R, const double* M, const d
     TreeGeom ge;
     ge.init(R, 1);
     hipLaunchKernelGGL(eig_tree_kernel, dim3(N), dim3(kThreads), 0, s, N, j, R, M, grams, ge, eig_nodes);
+    EX_HIP(hipGetLastError());
+}
+
+void eig_tree_to_f32(hipStream_t s, int N, int j, int R, const double* eig, float* eig32) {
+    TreeGeom ge;
+    ge.init(R, 1);
+    hipLaunchKernelGGL(eig_tree_f32_kernel, dim3(N), dim3(kThreads), 0, s, N, j, R, (int)ge.node_count, eig, eig32);
     EX_HIP(hipGetLastError());
 }
 

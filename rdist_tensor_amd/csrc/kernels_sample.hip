@@ -72,7 +72,7 @@ struct StsArgs {
     k::StsModeArgs m[kMaxModes];
     int N, j, R;
     int64_t J;
-    const double* pinvp;    // packed upper of M[0] = pinv of the Hadamard product
+    const double* pinvp;    // packed upper of M[0] = pinv of the Hadamard product (FP64)
     uint64_t seed, sweep;
     const double* draws_row;
     const double* draws_eig;
@@ -80,132 +80,224 @@ struct StsArgs {
     double* weights;
 };
 
-// Per-lane view of the packed upper triangle: lane owns e = lane + 64 t.
-template <int T>
-struct PairView {
-    int a[T], b[T];
-    double w[T];   // 1 on the diagonal, 2 off it
-    __device__ void init(int R, int lane) {
-        const int P = tri_size(R);
-#pragma unroll
-        for (int t = 0; t < T; t++) {
-            int e = lane + 64 * t;
-            if (e < P) {
-                tri_ab(R, e, a[t], b[t]);
-                w[t] = a[t] == b[t] ? 1.0 : 2.0;
-            } else {
-                a[t] = 0; b[t] = 0; w[t] = 0.0;
-            }
-        }
-    }
-    // xx[t] = w * x_a * x_b for the query x (held by lanes 0..R-1)
-    template <typename TX>
-    __device__ void outer(TX x, TX (&xx)[T]) const {
-#pragma unroll
-        for (int t = 0; t < T; t++) xx[t] = (TX)w[t] * (__shfl(x, a[t], 64) * __shfl(x, b[t], 64));
-    }
-};
-
-// x^T G x for a packed node: per-lane partial in the node's precision, the
-// wavefront total in FP64.
-template <typename TN, int T>
-__device__ __forceinline__ double quad(const TN* __restrict__ node, const TN (&xx)[T], int lane, int P) {
-    TN acc = 0;
-#pragma unroll
-    for (int t = 0; t < T; t++) {
-        int e = lane + 64 * t;
-        if (e < P) acc += node[e] * xx[t];
-    }
-    return wave_sum((double)acc);
+// Lane exchange inside rows of 16 lanes through DPP (CTRL: quad_perm,
+// row_half_mirror or row_mirror encodings).
+template <int CTRL>
+__device__ __forceinline__ float dpp_x(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_x(double v) {
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// Root-to-leaf descent of one heap tree for the query held in xx; returns the
-// leaf node and the (low, high) draw interval (partition_tree.hpp:226-307).
-// Masses and cutoffs are FP64 whatever the node precision.
-template <typename TN, int T>
-__device__ __forceinline__ int64_t descend(const TN* __restrict__ nodes, const TreeGeom& g, int S, int P,
-                                           const TN (&xx)[T], int lane, double u, double& low, double& high) {
-    const double m = quad<TN, T>(nodes, xx, lane, P);
+// Sum over aligned groups of G lanes (16, 32 or 64).  Each step adds the
+// same two operands on both partner lanes, so every lane of a group ends
+// with the same bits.
+template <int G, typename V>
+__device__ __forceinline__ V group_total(V v) {
+    v += dpp_x<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_x<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_x<0x141>(v);   // row_half_mirror: quad 0 <-> quad 1 of each half row
+    v += dpp_x<0x140>(v);   // row_mirror: half rows swapped
+    if (G >= 32) v += __shfl_xor(v, 16, 64);
+    if (G >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+template <typename TN> struct Vec;
+template <> struct Vec<float> {
+    static constexpr int W = 4;
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <> struct Vec<double> {
+    static constexpr int W = 2;
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+// LDS of one draw block: the packed-upper (a, b) table, and per sample the
+// running Hadamard product h, the current query and the leaf's q^2.
+constexpr int kDrawThreads = 256;
+constexpr int kMaxSlots = kDrawThreads / 16;
+template <typename TN>
+struct DrawShared {
+    uint16_t pairs[kMaxRank * (kMaxRank + 1) / 2];
+    TN h[kMaxSlots][kMaxRank];
+    TN x[kMaxSlots][kMaxRank];
+    double q2[kMaxSlots][kMaxRank];
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// xx[e] = w_e x_a x_b for the lane's packed elements e = (t G + l) W + i,
+// w = 1 on the diagonal and 2 off it, 0 past the triangle.
+template <typename TN, int G, int NV>
+__device__ __forceinline__ void outer_g(const uint16_t* pairs, const TN* x, int P, int l,
+                                        TN (&xx)[NV * Vec<TN>::W]) {
+    constexpr int W = Vec<TN>::W;
+#pragma unroll
+    for (int t = 0; t < NV; t++)
+#pragma unroll
+        for (int i = 0; i < W; i++) {
+            const int e = (t * G + l) * W + i;
+            TN v = 0;
+            if (e < P) {
+                const uint32_t pr = pairs[e];
+                const int a = pr & 0xFF, b = pr >> 8;
+                v = (a == b ? (TN)1 : (TN)2) * (x[a] * x[b]);
+            }
+            xx[t * W + i] = v;
+        }
+}
+
+// x^T G x of one packed node for the group's query: vector loads of W
+// consecutive elements per lane, lane partial in TN, group total, FP64 out.
+template <typename TN, int G, int NV>
+__device__ __forceinline__ double quad_g(const TN* __restrict__ node, const TN (&xx)[NV * Vec<TN>::W], int P,
+                                         int l) {
+    constexpr int W = Vec<TN>::W;
+    typedef typename Vec<TN>::type VT;
+    const VT* nv = reinterpret_cast<const VT*>(node);
+    TN acc = 0;
+#pragma unroll
+    for (int t = 0; t < NV; t++) {
+        const int v = t * G + l;
+        if (v * W < P) {
+            const VT d = nv[v];
+#pragma unroll
+            for (int i = 0; i < W; i++) acc += d[i] * xx[t * W + i];
+        }
+    }
+    return (double)group_total<G>(acc);
+}
+
+// Root-to-leaf descent of one heap tree (partition_tree.hpp:226-307): masses
+// and cutoffs in FP64, left if u <= low + m_left / m_root.
+template <typename TN, int G, int NV>
+__device__ __forceinline__ int64_t descend_g(const TN* __restrict__ nodes, const TreeGeom& g, int S, int P,
+                                             const TN (&xx)[NV * Vec<TN>::W], int l, double u, double& low,
+                                             double& high) {
+    const double m = quad_g<TN, G, NV>(nodes, xx, P, l);
     int64_t c = 0;
     low = 0.0;
     high = 1.0;
     for (uint32_t lvl = 0; lvl < g.lfill_level; lvl++) {
-        double mL = quad<TN, T>(nodes + (2 * c + 1) * S, xx, lane, P);
-        double cutoff = low + mL / m;
+        const double mL = quad_g<TN, G, NV>(nodes + (2 * c + 1) * S, xx, P, l);
+        const double cutoff = low + mL / m;
         if (u <= cutoff) { c = 2 * c + 1; high = cutoff; }
         else { c = 2 * c + 2; low = cutoff; }
     }
     if (!g.is_leaf(c)) {
-        double mL = quad<TN, T>(nodes + (2 * c + 1) * S, xx, lane, P);
-        double cutoff = low + mL / m;
+        const double mL = quad_g<TN, G, NV>(nodes + (2 * c + 1) * S, xx, P, l);
+        const double cutoff = low + mL / m;
         if (u <= cutoff) { c = 2 * c + 1; high = cutoff; }
         else { c = 2 * c + 2; low = cutoff; }
     }
     return c;
 }
 
-// STS-CP KRPDrawSamples for one sample per wavefront, every non-target mode in
-// turn (sts_cp.hpp:228-286).  TN: row-tree node and factor precision (double
-// on the plain path, float on the mixed path); the eigen trees, the scaled
-// eigenvectors and the final weight stay FP64.
-template <typename TN, int T>
-__global__ __launch_bounds__(kThreads) void sts_draw_kernel(StsArgs a) {
-    const int lane = threadIdx.x % 64;
-    const int64_t s = (int64_t)blockIdx.x * kWaves + threadIdx.x / 64;
-    if (s >= a.J) return;   // whole wavefront leaves together
+// STS-CP KRPDrawSamples (sts_cp.hpp:228-286): one sample per group of G
+// lanes, 64 / G samples per wavefront, every non-target mode in turn.  TN is
+// the precision of the factors and of the row- and eigen-tree nodes (double
+// on the plain path, float on the mixed path); masses, cutoffs, the leaf's
+// cumulative sums and the weight are FP64.
+template <typename TN, int G, int NV>
+__global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
+    constexpr int W = Vec<TN>::W;
+    constexpr int SPW = 64 / G;
+    __shared__ DrawShared<TN> sh;
     const int R = a.R, N = a.N, j = a.j;
-    const int P = tri_size(R), Se = tri_stride(R);
+    const int P = tri_size(R);
     const int S = sizeof(TN) == 8 ? tri_stride(R) : tri_stride32(R);
-    PairView<T> pv;
-    pv.init(R, lane);
-    double xd[T];
-    TN xn[T];
-    TN h = lane < R ? (TN)1 : (TN)0;
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        int ea, eb;
+        tri_ab(R, e, ea, eb);
+        sh.pairs[e] = (uint16_t)(ea | (eb << 8));
+    }
+    __syncthreads();
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int grp = lane / G, l = lane % G;
+    const int slot = wave * SPW + grp;
+    const int64_t s = ((int64_t)blockIdx.x * (kDrawThreads / 64) + wave) * SPW + grp;
+    if (s >= a.J) return;   // whole groups leave together
+    TN* hs = sh.h[slot];
+    TN* xs = sh.x[slot];
+    double* q2 = sh.q2[slot];
+    for (int b = l; b < R; b += G) hs[b] = (TN)1;
+    wave_lds_sync();
+    TN xx[NV * W];
     uint32_t eig_idx = 0;
     for (int k = 0; k < N; k++) {
         if (k == j) continue;
         const k::StsModeArgs& md = a.m[k];
         const TN* rows_k = static_cast<const TN*>(md.rows);
-        const TN* nodes_k = static_cast<const TN*>(md.nodes);
         // eigen tree: query h, F = 1
-        pv.outer((double)h, xd);
-        double ue = a.draws_eig ? a.draws_eig[(int64_t)k * a.J + s] : uniform01(a.seed, stream_id(a.sweep, j, k, 1), s);
+        outer_g<TN, G, NV>(sh.pairs, hs, P, l, xx);
+        const double ue = a.draws_eig ? a.draws_eig[(int64_t)k * a.J + s]
+                                      : uniform01(a.seed, stream_id(a.sweep, j, k, 1), s);
         double low, high;
-        int64_t c = descend<double, T>(md.eig, md.ge, Se, P, xd, lane, ue, low, high);
+        int64_t c = descend_g<TN, G, NV>(static_cast<const TN*>(md.eig), md.ge, S, P, xx, l, ue, low, high);
         const int64_t v = md.ge.leaf_idx(c);
         eig_idx = (uint32_t)v;
-        TN sh = lane < R ? (TN)((double)h * md.w[v * R + lane]) : (TN)0;
-        // row tree: query sh, F = R
-        pv.outer(sh, xn);
-        double ur = a.draws_row ? a.draws_row[(int64_t)k * a.J + s] : uniform01(a.seed, stream_id(a.sweep, j, k, 2), s);
-        c = descend<TN, T>(nodes_k, md.g, S, P, xn, lane, ur, low, high);
+        for (int b = l; b < R; b += G) xs[b] = (TN)((double)hs[b] * md.w[v * R + b]);
+        wave_lds_sync();
+        // row tree: query sh = h * w_v, F = R
+        outer_g<TN, G, NV>(sh.pairs, xs, P, l, xx);
+        const double ur = a.draws_row ? a.draws_row[(int64_t)k * a.J + s]
+                                      : uniform01(a.seed, stream_id(a.sweep, j, k, 2), s);
+        c = descend_g<TN, G, NV>(static_cast<const TN*>(md.nodes), md.g, S, P, xx, l, ur, low, high);
         const int64_t leaf = md.g.leaf_idx(c);
-        const int64_t F = md.g.F;
+        const int F = (int)md.g.F;
         const int rows = (int)((md.g.n - leaf * F) < F ? (md.g.n - leaf * F) : F);
         const double frac = (ur - low) / (high - low);
-        // q_r = U[leaf*F + r] . sh for the leaf's real rows.  The shuffles run
-        // on every lane (a shuffle from an inactive lane is undefined), and
-        // lanes past the last real row read row 0 of the leaf and drop it.
-        const bool real_row = lane < rows;
-        const TN* ur_row = rows_k + (leaf * F + (real_row ? lane : 0)) * R;
-        TN q = 0;
-        for (int b = 0; b < R; b++) q += ur_row[b] * __shfl(sh, b, 64);
-        if (!real_row) q = 0;
-        const double qd = (double)q;
-        double cum = wave_inclusive_scan(qd * qd, lane);
-        double total = __shfl(cum, (int)F - 1, 64);
-        bool cond = lane < (int)F - 1 && frac < cum / total;
-        uint64_t mask = __ballot(cond);
-        int64_t res = mask ? (int64_t)__ffsll((unsigned long long)mask) - 1 : F - 1;
+        // q_r = U[leaf F + r] . sh for the leaf's real rows (0 for the rest),
+        // q_r^2 into LDS; the cumulative sums run in row order
+        for (int r = l; r < F; r += G) {
+            TN q = 0;
+            if (r < rows) {
+                const TN* ur_row = rows_k + (leaf * F + r) * R;
+                for (int b = 0; b < R; b++) q += ur_row[b] * xs[b];
+            }
+            const double qd = (double)q;
+            q2[r] = qd * qd;
+        }
+        wave_lds_sync();
+        double total = 0.0;
+        for (int r = 0; r < F; r++) total += q2[r];
+        int res = F - 1;
+        for (int r0 = 0; r0 < F; r0 += G) {
+            const int r = r0 + l;
+            bool cond = false;
+            if (r < F - 1) {
+                double cum = 0.0;
+                for (int r2 = 0; r2 <= r; r2++) cum += q2[r2];
+                cond = frac < cum / total;
+            }
+            const uint64_t bits = (__ballot(cond) >> (grp * G)) & (G == 64 ? ~0ull : ((1ull << G) - 1));
+            if (bits) { res = r0 + __ffsll((unsigned long long)bits) - 1; break; }
+        }
         if (res >= rows) res = rows - 1;
         const int64_t row = leaf * F + res;
-        if (lane < R) h *= rows_k[row * R + lane];
-        if (lane == 0) a.samples[s * N + k] = (uint32_t)row;
+        for (int b = l; b < R; b += G) hs[b] *= rows_k[row * R + b];
+        wave_lds_sync();
+        if (l == 0) a.samples[s * N + k] = (uint32_t)row;
     }
     // weight = R / (J * h^T pinv h), FP64
-    pv.outer((double)h, xd);
-    double q = quad<double, T>(a.pinvp, xd, lane, P);
-    if (lane == 0) {
+    double q = 0.0;
+    for (int e = l; e < P; e += G) {
+        const uint32_t pr = sh.pairs[e];
+        const int ea = pr & 0xFF, eb = pr >> 8;
+        q += a.pinvp[e] * ((ea == eb ? 1.0 : 2.0) * ((double)hs[ea] * (double)hs[eb]));
+    }
+    q = group_total<G>(q);
+    if (l == 0) {
         a.samples[s * N + j] = eig_idx;
         a.weights[s] = (double)R / (q * (double)a.J);
     }
@@ -353,22 +445,20 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
     a.seed = seed; a.sweep = sweep;
     a.draws_row = draws_row; a.draws_eig = draws_eig;
     a.samples = samples; a.weights = weights;
-    unsigned nb = grid_for(J, kWaves, 1LL << 30);
-    const int P = tri_size(R);
+#define DRAW_LAUNCH(TN, G, NV)                                                                         \
+    hipLaunchKernelGGL((sts_draw_kernel<TN, G, NV>), dim3(grid_for(J, (kDrawThreads / 64) * (64 / G), 1LL << 30)), \
+                       dim3(kDrawThreads), 0, s, a)
+    // lanes per sample G and W-wide vectors per lane NV: G * NV * W >= R (R + 1) / 2
     if (node_bytes == 8) {
-        if (P <= 64 * 3)
-            hipLaunchKernelGGL((sts_draw_kernel<double, 3>), dim3(nb), dim3(kThreads), 0, s, a);
-        else if (P <= 64 * 9)
-            hipLaunchKernelGGL((sts_draw_kernel<double, 9>), dim3(nb), dim3(kThreads), 0, s, a);
-        else
-            hipLaunchKernelGGL((sts_draw_kernel<double, 33>), dim3(nb), dim3(kThreads), 0, s, a);
+        if (R <= 16) DRAW_LAUNCH(double, 16, 5);
+        else if (R <= 32) DRAW_LAUNCH(double, 32, 9);
+        else if (R <= 45) DRAW_LAUNCH(double, 64, 9);
+        else DRAW_LAUNCH(double, 64, 17);
     } else {
-        if (P <= 64 * 3)
-            hipLaunchKernelGGL((sts_draw_kernel<float, 3>), dim3(nb), dim3(kThreads), 0, s, a);
-        else if (P <= 64 * 9)
-            hipLaunchKernelGGL((sts_draw_kernel<float, 9>), dim3(nb), dim3(kThreads), 0, s, a);
-        else
-            hipLaunchKernelGGL((sts_draw_kernel<float, 33>), dim3(nb), dim3(kThreads), 0, s, a);
+        if (R <= 16) DRAW_LAUNCH(float, 16, 3);
+        else if (R <= 32) DRAW_LAUNCH(float, 16, 9);
+        else if (R <= 45) DRAW_LAUNCH(float, 32, 9);
+        else DRAW_LAUNCH(float, 64, 9);
     }
     EX_HIP(hipGetLastError());
 }
