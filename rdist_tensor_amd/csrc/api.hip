@@ -262,8 +262,9 @@ void sampler_update_mixed(ex_sampler* s, int j) {
         k::dagat32(st, U, I, R, s->gpinv.as<double>(), s->scores[j].as<double>());
         k::arls_cdf(st, s->scores[j].as<double>(), I, s->cdf[j].as<double>(), s->sums.as<double>() + j, s->part);
     } else {
+        // the root of the row tree is U_j^T U_j
         k::tree_build_f32(st, U, s->g[j], R, s->nodes[j].as<float>());
-        k::gram_bf16(st, U, I, R, s->grams.as<double>() + (int64_t)j * R * R, s->part);
+        k::unpack_root<float>(st, s->nodes[j].as<float>(), R, s->grams.as<double>() + (int64_t)j * R * R);
     }
     sp.end();
 }
@@ -289,9 +290,9 @@ void sampler_update_impl(ex_sampler* s, int j) {
         k::arls_cdf(st, s->scores[j].as<double>(), I, s->cdf[j].as<double>(), s->sums.as<double>() + j, s->part);
     } else {
         // sts_cp.hpp:63-65 -> ExactLeverageTree::build_tree: the local tree and
-        // ancestor_grams[0] = compute_gram(U_j)
+        // ancestor_grams[0] = compute_gram(U_j), which is the tree's root
         k::tree_build(st, U, s->g[j], R, s->nodes[j].as<double>());
-        k::gram(st, U, I, R, s->grams.as<double>() + (int64_t)j * R * R, s->part);
+        k::unpack_root<double>(st, s->nodes[j].as<double>(), R, s->grams.as<double>() + (int64_t)j * R * R);
     }
     sp.end();
 }

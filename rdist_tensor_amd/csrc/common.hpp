@@ -150,6 +150,12 @@ void arls_draw(hipStream_t s, int N, int j, int64_t J, const double* const* cdfs
                const double* draws, uint32_t* samples, double* weights);
 // STS
 void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double* nodes);
+// internal nodes of a tree whose leaves are built (node = left + right)
+template <typename T>
+void level_sums(hipStream_t s, const TreeGeom& g, int S, T* nodes);
+// the root node (packed upper, the Gram of the whole factor) as a full R x R FP64 matrix
+template <typename T>
+void unpack_root(hipStream_t s, const T* nodes, int R, double* G);
 void sts_compute_m(hipStream_t s, int N, int j, int R, const double* grams, double* M, double* lam);
 void eig_tree_build(hipStream_t s, int N, int j, int R, const double* M, const double* grams, double* eig_nodes);
 void eig_tree_to_f32(hipStream_t s, int N, int j, int R, const double* eig, float* eig32);

@@ -21,21 +21,6 @@ __device__ __forceinline__ f32x4 mfma_bf16(const bf16x8& a, const bf16x8& b, con
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Tiles (ta <= tb) of an R x R Gram in 16 x 16 blocks.
-__host__ __device__ inline int n_tiles(int R) {
-    const int nt = (R + 15) / 16;
-    return nt * (nt + 1) / 2;
-}
-
-__device__ __forceinline__ void tile_of(int R, int t, int& ta, int& tb) {
-    const int nt = (R + 15) / 16;
-    ta = 0;
-    while (t >= nt - ta) { t -= nt - ta; ta++; }
-    tb = ta + t;
-}
-
-constexpr int kMaxTiles = 10;   // R <= 64: 4 x 5 / 2
-
 // Load the 8 K-values this lane contributes for column `col` from rows
 // r0 + 8 (lane/16) + i (zero beyond `rows` or past R).
 __device__ __forceinline__ bf16x8 load_k8(const float* __restrict__ A, int64_t r0, int rows, int R, int col,
@@ -51,54 +36,56 @@ __device__ __forceinline__ bf16x8 load_k8(const float* __restrict__ A, int64_t r
     return v;
 }
 
+// Accumulates the upper tiles (ta <= tb) of the 32-row chunk's A^T A on
+// MFMA; NT = ceil(R / 16) column tiles, all indices compile-time.
+template <int NT>
+__device__ __forceinline__ void gram_chunk(const float* __restrict__ A, int64_t r0, int rows, int R, int lane,
+                                           f32x4 (&acc)[NT][NT]) {
+    bf16x8 col[NT];
+#pragma unroll
+    for (int c = 0; c < NT; c++) col[c] = load_k8(A, r0, rows, R, 16 * c + lane % 16, lane);
+#pragma unroll
+    for (int ta = 0; ta < NT; ta++)
+#pragma unroll
+        for (int tb = ta; tb < NT; tb++) acc[ta][tb] = mfma_bf16(col[ta], col[tb], acc[ta][tb]);
+}
+
 // ---------------------------------------------------------------- Gram (BF16 MFMA)
 // Block partials of A^T A (A: m x R, FP32) on MFMA.  Each wavefront takes
 // 32-row chunks w, w+4, ... of the block's rows; tiles (ta <= tb) accumulate in
 // FP32; the four wavefronts' tiles are added into one FP64 LDS array in wave
 // order and the block writes its upper-triangle partial (FP64) for the FP64
 // final reduction.
+template <int NT>
 __global__ __launch_bounds__(kThreads) void gram_bf16_partial(const float* __restrict__ A, int64_t m, int R,
                                                               double* __restrict__ part) {
     __shared__ double red[kMaxRank * kMaxRank];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    const int ntl = n_tiles(R);
-    f32x4 acc[kMaxTiles];
+    f32x4 acc[NT][NT];
 #pragma unroll
-    for (int t = 0; t < kMaxTiles; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ta = 0; ta < NT; ta++)
+#pragma unroll
+        for (int tb = 0; tb < NT; tb++) acc[ta][tb] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int64_t nchunks = (m + 31) / 32;
     for (int64_t ch = (int64_t)blockIdx.x * 4 + wave; ch < nchunks; ch += (int64_t)gridDim.x * 4) {
         const int64_t r0 = ch * 32;
-        const int rows = (int)((m - r0) < 32 ? (m - r0) : 32);
-        bf16x8 col[4];
-        const int nt = (R + 15) / 16;
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-            if (c < nt) col[c] = load_k8(A, r0, rows, R, 16 * c + lane % 16, lane);
-#pragma unroll
-        for (int t = 0; t < kMaxTiles; t++) {
-            if (t >= ntl) break;
-            int ta, tb;
-            tile_of(R, t, ta, tb);
-            acc[t] = mfma_bf16(col[ta], col[tb], acc[t]);
-        }
+        gram_chunk<NT>(A, r0, (int)((m - r0) < 32 ? (m - r0) : 32), R, lane, acc);
     }
     // wave 0 stores its tiles, waves 1..3 add theirs in turn
     for (int w = 0; w < 4; w++) {
         if (wave == w) {
 #pragma unroll
-            for (int t = 0; t < kMaxTiles; t++) {
-                if (t >= ntl) break;
-                int ta, tb;
-                tile_of(R, t, ta, tb);
+            for (int ta = 0; ta < NT; ta++)
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int a = 16 * ta + 4 * (lane / 16) + i, b = 16 * tb + lane % 16;
-                    if (a < R && b < R) {
-                        const double v = (double)acc[t][i];
-                        red[a * R + b] = w == 0 ? v : red[a * R + b] + v;
+                for (int tb = ta; tb < NT; tb++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int a = 16 * ta + 4 * (lane / 16) + i, b = 16 * tb + lane % 16;
+                        if (a < R && b < R) {
+                            const double v = (double)acc[ta][tb][i];
+                            red[a * R + b] = w == 0 ? v : red[a * R + b] + v;
+                        }
                     }
-                }
-            }
         }
         __syncthreads();
     }
@@ -111,233 +98,261 @@ __global__ __launch_bounds__(kThreads) void gram_bf16_partial(const float* __res
 
 // ---------------------------------------------------------------- tree leaves (BF16 MFMA)
 // One wavefront per leaf: the leaf's F = R rows (zero-padded to a multiple of
-// 32) times themselves on MFMA; the packed upper triangle is written in FP32.
+// 32) times themselves on MFMA; the packed upper triangle is written in FP32
+// with the node stride's padding zeroed.
+template <int NT>
 __global__ __launch_bounds__(kThreads) void leaf_gram_bf16(const float* __restrict__ U, int64_t n, int R, TreeGeom g,
                                                            float* __restrict__ nodes, int S) {
     const int lane = threadIdx.x % 64;
-    const int ntl = n_tiles(R), nt = (R + 15) / 16;
     const int64_t first_leaf = (int64_t)g.node_count - g.leaf_count;
     for (int64_t i = (int64_t)blockIdx.x * 4 + threadIdx.x / 64; i < (int64_t)g.leaf_count;
          i += (int64_t)gridDim.x * 4) {
         const int64_t c = first_leaf + i;
         const int64_t r0 = g.leaf_idx(c) * g.F;
         const int rows = (int)((n - r0) < g.F ? (n - r0) : g.F);
-        f32x4 acc[kMaxTiles];
+        f32x4 acc[NT][NT];
 #pragma unroll
-        for (int t = 0; t < kMaxTiles; t++) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < rows; k0 += 32) {
-            const int kr = rows - k0 < 32 ? rows - k0 : 32;
-            bf16x8 col[4];
+        for (int ta = 0; ta < NT; ta++)
 #pragma unroll
-            for (int q = 0; q < 4; q++)
-                if (q < nt) col[q] = load_k8(U, r0 + k0, kr, R, 16 * q + lane % 16, lane);
-#pragma unroll
-            for (int t = 0; t < kMaxTiles; t++) {
-                if (t >= ntl) break;
-                int ta, tb;
-                tile_of(R, t, ta, tb);
-                acc[t] = mfma_bf16(col[ta], col[tb], acc[t]);
-            }
-        }
+            for (int tb = 0; tb < NT; tb++) acc[ta][tb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < rows; k0 += 32)
+            gram_chunk<NT>(U, r0 + k0, rows - k0 < 32 ? rows - k0 : 32, R, lane, acc);
         float* out = nodes + c * S;
 #pragma unroll
-        for (int t = 0; t < kMaxTiles; t++) {
-            if (t >= ntl) break;
-            int ta, tb;
-            tile_of(R, t, ta, tb);
+        for (int ta = 0; ta < NT; ta++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int a = 16 * ta + 4 * (lane / 16) + q, b = 16 * tb + lane % 16;
-                if (a < R && b < R && a <= b) out[tri_index(R, a, b)] = acc[t][q];
-            }
-        }
-        // padding of the node stride
+            for (int tb = ta; tb < NT; tb++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int a = 16 * ta + 4 * (lane / 16) + q, b = 16 * tb + lane % 16;
+                    if (a < R && b < R && a <= b) out[tri_index(R, a, b)] = acc[ta][tb][q];
+                }
         for (int e = tri_size(R) + lane; e < S; e += 64) out[e] = 0.0f;
     }
 }
 
-__global__ __launch_bounds__(kThreads) void level_sum_f32(float* __restrict__ nodes, int64_t start, int64_t end, int S) {
-    const int64_t total = (end - start) * S;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = start + o / S;
-        const int e = (int)(o % S);
-        nodes[c * S + e] = nodes[(2 * c + 1) * S + e] + nodes[(2 * c + 2) * S + e];
-    }
-}
-
 // ---------------------------------------------------------------- FP64 SPD solves
-// One workgroup, the matrix in LDS (full R x R, FP64).
+// One workgroup of 256 threads holds an R x R matrix in registers: thread t
+// owns column j = t % R and rows g, g + GR, g + 2 GR, ... (g = t / R,
+// GR = 256 / R row groups, at most RPT rows).  Each elimination step
+// publishes one column (and row) through LDS; the buffers alternate between
+// steps, so one barrier per step suffices.
+constexpr int kSpdThreads = 256;
+
 struct SpdShared {
-    double A[kMaxRank * kMaxRank];
-    double col[kMaxRank], row[kMaxRank];
-    double piv;
-    int skip;
+    double col[2][kMaxRank];
+    double row[2][kMaxRank];
+    double full[kMaxRank * kMaxRank];
 };
 
-// Gauss-Jordan inverse of a symmetric positive semidefinite matrix in place.
-// A pivot <= tol (absolute) marks a null direction: its row and column are
-// zeroed, which gives the pseudo-inverse on the complementary subspace when
-// the null space is spanned by coordinate directions and a regularised inverse
-// otherwise.
-__device__ void spd_inverse_lds(SpdShared& sh, int R, double tol) {
-    const int tid = threadIdx.x, nt = blockDim.x;
+struct SpdLayout {
+    int j, g, GR;
+    bool on;
+    __device__ SpdLayout(int R) {
+        GR = kSpdThreads / R;
+        j = threadIdx.x % R;
+        g = threadIdx.x / R;
+        on = g < GR;
+    }
+    __device__ int row(int m) const { return g + GR * m; }
+};
+
+template <int RPT>
+__device__ double spd_max_diag(SpdShared& sh, const SpdLayout& L, const double (&a)[RPT], int R) {
+    if (L.on)
+#pragma unroll
+        for (int m = 0; m < RPT; m++)
+            if (L.row(m) == L.j) sh.col[0][L.j] = a[m];
+    __syncthreads();
+    double mx = 0.0;
+    for (int i = 0; i < R; i++) mx = fmax(mx, sh.col[0][i]);
+    __syncthreads();
+    return mx;
+}
+
+// Gauss-Jordan inverse of a symmetric positive semidefinite matrix.  A pivot
+// <= tol (absolute) marks a null direction: its row and column are zeroed,
+// which gives the pseudo-inverse on the complementary subspace when the null
+// space is spanned by coordinate directions and a regularised inverse
+// otherwise.  The upper triangle of the result is mirrored to the lower.
+template <int RPT>
+__device__ void spd_inverse_regs(SpdShared& sh, const SpdLayout& L, double (&a)[RPT], int R, double tol) {
     for (int k = 0; k < R; k++) {
-        if (tid == 0) {
-            sh.piv = sh.A[k * R + k];
-            sh.skip = !(sh.piv > tol);
-        }
-        __syncthreads();
-        const double piv = sh.piv;
-        const int skip = sh.skip;
-        for (int i = tid; i < R; i += nt) {
-            sh.col[i] = sh.A[i * R + k];
-            sh.row[i] = sh.A[k * R + i];
-        }
-        __syncthreads();
-        for (int o = tid; o < R * R; o += nt) {
-            const int i = o / R, j = o % R;
-            double v = sh.A[o];
-            if (skip) {
-                if (i == k || j == k) v = 0.0;
-            } else if (i == k && j == k) {
-                v = 1.0 / piv;
-            } else if (i == k) {
-                v = sh.row[j] / piv;
-            } else if (j == k) {
-                v = -sh.col[i] / piv;
-            } else {
-                v = v - sh.col[i] * sh.row[j] / piv;
+        const int b = k & 1;
+        if (L.on) {
+#pragma unroll
+            for (int m = 0; m < RPT; m++) {
+                const int i = L.row(m);
+                if (i < R && L.j == k) sh.col[b][i] = a[m];
+                if (i == k) sh.row[b][L.j] = a[m];
             }
-            sh.A[o] = v;
         }
         __syncthreads();
+        const double piv = sh.col[b][k];
+        const bool skip = !(piv > tol);
+        const double inv = skip ? 0.0 : 1.0 / piv;
+        if (L.on) {
+            const double rk = sh.row[b][L.j];
+            const int j = L.j;
+#pragma unroll
+            for (int m = 0; m < RPT; m++) {
+                const int i = L.row(m);
+                if (i >= R) continue;
+                const double ci = sh.col[b][i];
+                double v = a[m];
+                if (skip) {
+                    if (i == k || j == k) v = 0.0;
+                } else if (i == k && j == k) {
+                    v = inv;
+                } else if (i == k) {
+                    v = rk * inv;
+                } else if (j == k) {
+                    v = -ci * inv;
+                } else {
+                    v = v - ci * (rk * inv);
+                }
+                a[m] = v;
+            }
+        }
     }
-    // Gauss-Jordan on a symmetric matrix yields the inverse; flip the sign
-    // convention of the eliminated columns back: A^-1 = the accumulated matrix
-    // with off-pivot signs restored by symmetry (take the upper triangle).
-    for (int o = tid; o < R * R; o += nt) {
-        const int i = o / R, j = o % R;
-        if (i > j) sh.A[o] = sh.A[j * R + i];
-    }
+    // mirror the upper triangle
+    __syncthreads();
+    if (L.on)
+#pragma unroll
+        for (int m = 0; m < RPT; m++)
+            if (L.row(m) < R) sh.full[L.row(m) * R + L.j] = a[m];
+    __syncthreads();
+    if (L.on)
+#pragma unroll
+        for (int m = 0; m < RPT; m++) {
+            const int i = L.row(m);
+            if (i < R && i > L.j) a[m] = sh.full[L.j * R + i];
+        }
     __syncthreads();
 }
 
-// Semidefinite Cholesky A = L L^T (lower, in place, strictly-upper zeroed);
-// a pivot <= tol * max diagonal gives a zero column.
-__device__ void spd_cholesky_lds(SpdShared& sh, int R, double tol_rel) {
-    const int tid = threadIdx.x, nt = blockDim.x;
-    __shared__ double dmax;
-    if (tid == 0) {
-        double m = 0.0;
-        for (int i = 0; i < R; i++) m = fmax(m, sh.A[i * R + i]);
-        dmax = m;
-    }
-    __syncthreads();
-    const double thr = tol_rel * dmax;
+// Semidefinite Cholesky A = L L^T in registers (lower; entries above the
+// diagonal zeroed); a pivot <= thr gives a zero column.
+template <int RPT>
+__device__ void spd_cholesky_regs(SpdShared& sh, const SpdLayout& L, double (&a)[RPT], int R, double thr) {
     for (int k = 0; k < R; k++) {
-        if (tid == 0) {
-            const double d = sh.A[k * R + k];
-            sh.skip = !(d > thr);
-            sh.piv = sh.skip ? 0.0 : sqrt(d);
-        }
+        const int b = k & 1;
+        if (L.on && L.j == k)
+#pragma unroll
+            for (int m = 0; m < RPT; m++)
+                if (L.row(m) < R) sh.col[b][L.row(m)] = a[m];
         __syncthreads();
-        const double piv = sh.piv;
-        const int skip = sh.skip;
-        for (int i = tid; i < R; i += nt) {
-            double v = (i > k && !skip) ? sh.A[i * R + k] / piv : 0.0;
-            if (i == k) v = piv;
-            sh.col[i] = v;
+        const double d = sh.col[b][k];
+        const bool skip = !(d > thr);
+        const double piv = skip ? 0.0 : sqrt(d);
+        const double inv = skip ? 0.0 : 1.0 / piv;
+        if (L.on && L.j >= k) {
+            const int j = L.j;
+            const double lj = j == k ? piv : sh.col[b][j] * inv;
+#pragma unroll
+            for (int m = 0; m < RPT; m++) {
+                const int i = L.row(m);
+                if (i >= R || i < j) continue;
+                const double li = i == k ? piv : sh.col[b][i] * inv;
+                if (j == k) a[m] = li;
+                else a[m] -= li * lj;
+            }
         }
-        __syncthreads();
-        for (int o = tid; o < R * R; o += nt) {
-            const int i = o / R, j = o % R;
-            if (j == k) sh.A[o] = i >= k ? sh.col[i] : 0.0;
-            else if (i == k && j > k) sh.A[o] = 0.0;
-            else if (i > k && j > k && j <= i) sh.A[o] -= sh.col[i] * sh.col[j];
-        }
-        __syncthreads();
     }
-    for (int o = tid; o < R * R; o += nt) {
-        const int i = o / R, j = o % R;
-        if (j > i) sh.A[o] = 0.0;
-    }
+    if (L.on)
+#pragma unroll
+        for (int m = 0; m < RPT; m++)
+            if (L.row(m) < L.j) a[m] = 0.0;
     __syncthreads();
 }
 
-__device__ void load_sym_upper(SpdShared& sh, const double* M, int R) {
-    for (int o = threadIdx.x; o < R * R; o += blockDim.x) {
-        const int i = o / R, j = o % R;
-        sh.A[o] = i <= j ? M[o] : M[j * R + i];
-    }
-    __syncthreads();
-}
-
-// Inverse of a symmetric positive semidefinite R x R matrix; pivots at or
-// below tol_rel * (largest diagonal entry) are dropped.
-__global__ __launch_bounds__(kThreads) void spd_inverse_kernel(const double* __restrict__ M, int R, double tol_rel,
-                                                               double* __restrict__ out, float* __restrict__ out32) {
+// Inverse of a symmetric positive semidefinite R x R matrix (upper triangle
+// of M read); pivots at or below tol_rel * (largest diagonal entry) dropped.
+template <int RPT>
+__global__ __launch_bounds__(kSpdThreads) void spd_inverse_kernel(const double* __restrict__ M, int R, double tol_rel,
+                                                                  double* __restrict__ out, float* __restrict__ out32) {
     __shared__ SpdShared sh;
-    __shared__ double dmax;
-    load_sym_upper(sh, M, R);
-    if (threadIdx.x == 0) {
-        double m = 0.0;
-        for (int i = 0; i < R; i++) m = fmax(m, sh.A[i * R + i]);
-        dmax = m;
+    const SpdLayout L(R);
+    double a[RPT];
+#pragma unroll
+    for (int m = 0; m < RPT; m++) {
+        const int i = L.row(m), j = L.j;
+        a[m] = (L.on && i < R) ? (i <= j ? M[i * R + j] : M[j * R + i]) : 0.0;
     }
-    __syncthreads();
-    spd_inverse_lds(sh, R, tol_rel * dmax);
-    for (int o = threadIdx.x; o < R * R; o += blockDim.x) {
-        out[o] = sh.A[o];
-        if (out32) out32[o] = (float)sh.A[o];
-    }
+    const double dmax = spd_max_diag<RPT>(sh, L, a, R);
+    spd_inverse_regs<RPT>(sh, L, a, R, tol_rel * dmax);
+    if (L.on)
+#pragma unroll
+        for (int m = 0; m < RPT; m++) {
+            const int i = L.row(m);
+            if (i >= R) continue;
+            out[i * R + L.j] = a[m];
+            if (out32) out32[i * R + L.j] = (float)a[m];
+        }
 }
 
 // computeM (sts_cp.hpp:80-188) with the eigendecompositions replaced by
 // factorisations M = sum_v w_v w_v^T (rows w_v): slot 0 = (Hadamard product)^-1
 // (Gauss-Jordan, pivots <= 1e-8 dropped), slot N rows from the Cholesky factor
 // of slot 0, slots k rows from the Cholesky factor of (later Grams (.) slot 0).
-// Any such factorisation gives the same row-sampling distribution.
-__global__ __launch_bounds__(kThreads) void sts_compute_m_mixed(int N, int j, int R, const double* __restrict__ grams,
-                                                                double* __restrict__ M) {
+// Any such factorisation gives the same row-sampling distribution.  Every
+// thread reads and writes only its own elements of each slot.
+template <int RPT>
+__global__ __launch_bounds__(kSpdThreads) void sts_compute_m_mixed(int N, int j, int R,
+                                                                   const double* __restrict__ grams,
+                                                                   double* __restrict__ M) {
     __shared__ SpdShared sh;
-    const int R2 = R * R, tid = threadIdx.x, nt = blockDim.x;
-    for (int i = tid; i < R2; i += nt) M[N * R2 + i] = 1.0;
-    __syncthreads();
-    int last = N;
+    const SpdLayout L(R);
+    const int R2 = R * R;
+    double a[RPT], s0[RPT];
+#pragma unroll
+    for (int m = 0; m < RPT; m++) a[m] = 1.0;
+    if (L.on)
+#pragma unroll
+        for (int m = 0; m < RPT; m++)
+            if (L.row(m) < R) M[N * R2 + L.row(m) * R + L.j] = 1.0;
     for (int k = N - 1; k >= 0; k--) {
         if (k == j) continue;
-        for (int i = tid; i < R2; i += nt) M[k * R2 + i] = grams[k * R2 + i] * M[last * R2 + i];
-        __syncthreads();
-        last = k;
+        if (L.on)
+#pragma unroll
+            for (int m = 0; m < RPT; m++) {
+                const int o = L.row(m) * R + L.j;
+                if (L.row(m) < R) {
+                    a[m] *= grams[k * R2 + o];
+                    M[k * R2 + o] = a[m];
+                }
+            }
     }
-    if (j == 0) {
-        for (int i = tid; i < R2; i += nt) M[i] = M[R2 + i];
-        __syncthreads();
+    // a: the full Hadamard product (slot 0; slot 1's copy when j == 0)
+    if (L.on)
+#pragma unroll
+        for (int m = 0; m < RPT; m++)
+            if (L.row(m) < R) M[(N + 1) * R2 + L.row(m) * R + L.j] = a[m];
+#pragma unroll
+    for (int m = 0; m < RPT; m++)
+        if (!(L.on && L.row(m) < R)) a[m] = 0.0;
+    spd_inverse_regs<RPT>(sh, L, a, R, 1e-8);
+#pragma unroll
+    for (int m = 0; m < RPT; m++) {
+        s0[m] = a[m];
+        if (L.on && L.row(m) < R) M[L.row(m) * R + L.j] = a[m];
     }
-    for (int i = tid; i < R2; i += nt) M[(N + 1) * R2 + i] = M[i];
-    __syncthreads();
-    load_sym_upper(sh, M, R);
-    spd_inverse_lds(sh, R, 1e-8);
-    for (int i = tid; i < R2; i += nt) M[i] = sh.A[i];
-    __syncthreads();
     // slot N: rows = columns of chol(slot 0)
-    spd_cholesky_lds(sh, R, 1e-12);
-    for (int o = tid; o < R2; o += nt) {
-        const int v = o / R, u = o % R;
-        M[N * R2 + v * R + u] = sh.A[u * R + v];
-    }
-    __syncthreads();
+    spd_cholesky_regs<RPT>(sh, L, a, R, 1e-12 * spd_max_diag<RPT>(sh, L, a, R));
+    if (L.on)
+#pragma unroll
+        for (int m = 0; m < RPT; m++)
+            if (L.row(m) < R) M[N * R2 + L.j * R + L.row(m)] = a[m];
     for (int k = N - 1; k > 0; k--) {
         if (k == j) continue;
-        for (int i = tid; i < R2; i += nt) M[k * R2 + i] *= M[i];
-        __syncthreads();
-        load_sym_upper(sh, M + k * R2, R);
-        spd_cholesky_lds(sh, R, 1e-12);
-        for (int o = tid; o < R2; o += nt) {
-            const int v = o / R, u = o % R;
-            M[k * R2 + v * R + u] = sh.A[u * R + v];
-        }
-        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < RPT; m++)
+            a[m] = (L.on && L.row(m) < R) ? M[k * R2 + L.row(m) * R + L.j] * s0[m] : 0.0;
+        spd_cholesky_regs<RPT>(sh, L, a, R, 1e-12 * spd_max_diag<RPT>(sh, L, a, R));
+        if (L.on)
+#pragma unroll
+            for (int m = 0; m < RPT; m++)
+                if (L.row(m) < R) M[k * R2 + L.j * R + L.row(m)] = a[m];
     }
 }
 
@@ -445,38 +460,47 @@ __global__ __launch_bounds__(kThreads) void dagat_f32(const float* __restrict__ 
 
 namespace k {
 
+#define NT_DISPATCH(R, KERNEL, GRID, BLOCK, STREAM, ...)                                                 \
+    do {                                                                                               \
+        switch (((R) + 15) / 16) {                                                                     \
+            case 1: hipLaunchKernelGGL((KERNEL<1>), GRID, BLOCK, 0, STREAM, __VA_ARGS__); break;       \
+            case 2: hipLaunchKernelGGL((KERNEL<2>), GRID, BLOCK, 0, STREAM, __VA_ARGS__); break;       \
+            case 3: hipLaunchKernelGGL((KERNEL<3>), GRID, BLOCK, 0, STREAM, __VA_ARGS__); break;       \
+            default: hipLaunchKernelGGL((KERNEL<4>), GRID, BLOCK, 0, STREAM, __VA_ARGS__); break;      \
+        }                                                                                              \
+    } while (0)
+
 void gram_bf16(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part) {
     const int nb = (int)grid_for((m + 31) / 32, 4, 512);
     part.ensure((size_t)nb * R * R * sizeof(double));
-    hipLaunchKernelGGL(gram_bf16_partial, dim3(nb), dim3(kThreads), 0, s, A, m, R, part.as<double>());
+    NT_DISPATCH(R, gram_bf16_partial, dim3(nb), dim3(kThreads), s, A, m, R, part.as<double>());
     EX_HIP(hipGetLastError());
     gram_final_launch(s, part.as<double>(), nb, R, G);
 }
 
 void tree_build_f32(hipStream_t s, const float* U, const TreeGeom& g, int R, float* nodes) {
     const int S = tri_stride32(R);
-    hipLaunchKernelGGL(leaf_gram_bf16, dim3(grid_for(g.leaf_count, 4, 1 << 16)), dim3(kThreads), 0, s, U, g.n, R, g,
-                       nodes, S);
+    NT_DISPATCH(R, leaf_gram_bf16, dim3(grid_for(g.leaf_count, 4, 1 << 16)), dim3(kThreads), s, U, g.n, R, g, nodes,
+                S);
     EX_HIP(hipGetLastError());
-    const int64_t first_leaf = (int64_t)g.node_count - g.leaf_count;
-    int64_t start = g.nodes_before_lfill, end = first_leaf;
-    for (int64_t lvl = g.lfill_level; lvl >= 0; lvl--) {
-        if (end > start)
-            hipLaunchKernelGGL(level_sum_f32, dim3(grid_for((end - start) * S, kThreads, 8192)), dim3(kThreads), 0, s,
-                               nodes, start, end, S);
-        end = start;
-        start = ((start + 1) / 2) - 1;
-    }
-    EX_HIP(hipGetLastError());
+    level_sums(s, g, S, nodes);
 }
 
+// rows per thread: 256 / R row groups cover R rows
+#define RM_DISPATCH(R, KERNEL, STREAM, ...)                                                                  \
+    do {                                                                                                    \
+        if ((R) <= 16) hipLaunchKernelGGL((KERNEL<1>), dim3(1), dim3(kSpdThreads), 0, STREAM, __VA_ARGS__);  \
+        else if ((R) <= 35) hipLaunchKernelGGL((KERNEL<5>), dim3(1), dim3(kSpdThreads), 0, STREAM, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERNEL<16>), dim3(1), dim3(kSpdThreads), 0, STREAM, __VA_ARGS__);          \
+    } while (0)
+
 void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, float* out32) {
-    hipLaunchKernelGGL(spd_inverse_kernel, dim3(1), dim3(kThreads), 0, s, M, R, tol_rel, out, out32);
+    RM_DISPATCH(R, spd_inverse_kernel, s, M, R, tol_rel, out, out32);
     EX_HIP(hipGetLastError());
 }
 
 void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M) {
-    hipLaunchKernelGGL(sts_compute_m_mixed, dim3(1), dim3(kThreads), 0, s, N, j, R, grams, M);
+    RM_DISPATCH(R, sts_compute_m_mixed, s, N, j, R, grams, M);
     EX_HIP(hipGetLastError());
 }
 

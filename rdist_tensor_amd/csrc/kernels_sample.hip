@@ -56,14 +56,73 @@ __global__ __launch_bounds__(64 * WPB) void leaf_gram_kernel(const double* __res
     }
 }
 
-// Internal nodes of one level: node = left + right.
-__global__ __launch_bounds__(kThreads) void level_sum_kernel(double* __restrict__ nodes, int64_t start, int64_t end,
+// Internal nodes, two levels per launch: block p (a node of the upper level)
+// forms its children that are internal (in [s1, e1)) from their own children,
+// writes them, and then itself from its two children.  node = 0 + (left +
+// right) at every level, as a level-by-level pass computes it.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void level_pair_kernel(T* __restrict__ nodes, int64_t s0, int64_t s1,
+                                                              int64_t e1, int S) {
+    const int64_t p = s0 + blockIdx.x;
+    for (int e = threadIdx.x; e < S; e += blockDim.x) {
+        T v[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int64_t c = 2 * p + 1 + q;
+            if (c >= s1 && c < e1) {
+                v[q] = (T)0 + (nodes[(2 * c + 1) * S + e] + nodes[(2 * c + 2) * S + e]);
+                nodes[c * S + e] = v[q];
+            } else {
+                v[q] = nodes[c * S + e];
+            }
+        }
+        nodes[p * S + e] = (T)0 + (v[0] + v[1]);
+    }
+}
+
+// The remaining top levels in one workgroup: [start, end) first, then every
+// level above it up to the root.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void level_top_kernel(T* __restrict__ nodes, int64_t start, int64_t end,
                                                              int S) {
-    const int64_t total = (end - start) * S;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c = start + o / S;
-        int e = (int)(o % S);
-        nodes[c * S + e] = 0.0 + (nodes[(2 * c + 1) * S + e] + nodes[(2 * c + 2) * S + e]);
+    while (true) {
+        for (int64_t c = start; c < end; c++)
+            for (int e = threadIdx.x; e < S; e += blockDim.x)
+                nodes[c * S + e] = (T)0 + (nodes[(2 * c + 1) * S + e] + nodes[(2 * c + 2) * S + e]);
+        __syncthreads();
+        if (start <= 0) break;
+        end = start;
+        start = (start + 1) / 2 - 1;
+    }
+}
+
+// Depths D..0 in one launch once depth D + 1 is complete and built: thread e
+// reads element e of the 2^(D+1) nodes of depth D + 1 and forms every node
+// above them pairwise, as the level-by-level pass does.
+template <typename T, int D>
+__global__ __launch_bounds__(kThreads) void level_fused_top_kernel(T* __restrict__ nodes, int S) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= S) return;
+    constexpr int NB = 1 << (D + 1);
+    T v[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) v[q] = nodes[(int64_t)(NB - 1 + q) * S + e];
+#pragma unroll
+    for (int d = D; d >= 0; d--)
+#pragma unroll
+        for (int q = 0; q < (1 << d); q++) {
+            v[q] = (T)0 + (v[2 * q] + v[2 * q + 1]);
+            nodes[(int64_t)((1 << d) - 1 + q) * S + e] = v[q];
+        }
+}
+
+// Root node (packed upper) -> full symmetric R x R in FP64.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void unpack_root_kernel(const T* __restrict__ root, int R,
+                                                               double* __restrict__ G) {
+    for (int o = threadIdx.x; o < R * R; o += blockDim.x) {
+        const int a = o / R, b = o % R;
+        G[o] = (double)root[a <= b ? tri_index(R, a, b) : tri_index(R, b, a)];
     }
 }
 
@@ -411,6 +470,50 @@ __global__ __launch_bounds__(kThreads) void arls_draw_kernel(ArlsArgs a) {
 
 namespace k {
 
+template <typename T>
+void level_sums(hipStream_t s, const TreeGeom& g, int S, T* nodes) {
+    const int64_t first_leaf = (int64_t)g.node_count - g.leaf_count;
+    int depth = (int)g.lfill_level;
+    int64_t start = g.nodes_before_lfill, end = first_leaf;   // internal nodes of depth lfill_level
+    bool paired = false;
+    // pairs of levels (one launch per pair) down to depth 4
+    while (depth >= 5) {
+        const int64_t s0 = ((int64_t)1 << (depth - 1)) - 1, e0 = ((int64_t)1 << depth) - 1;
+        hipLaunchKernelGGL(level_pair_kernel<T>, dim3((unsigned)(e0 - s0)), dim3(kThreads), 0, s, nodes, s0, start,
+                           end, S);
+        depth -= 2;
+        paired = true;
+        start = ((int64_t)1 << depth) - 1;
+        end = ((int64_t)1 << (depth + 1)) - 1;
+    }
+    if (depth < 0) return;
+    if (paired) {
+        // depth + 1 is complete and built: the rest in one launch
+        const dim3 grid((S + kThreads - 1) / kThreads);
+        switch (depth) {
+            case 0: hipLaunchKernelGGL((level_fused_top_kernel<T, 0>), grid, dim3(kThreads), 0, s, nodes, S); break;
+            case 1: hipLaunchKernelGGL((level_fused_top_kernel<T, 1>), grid, dim3(kThreads), 0, s, nodes, S); break;
+            case 2: hipLaunchKernelGGL((level_fused_top_kernel<T, 2>), grid, dim3(kThreads), 0, s, nodes, S); break;
+            case 3: hipLaunchKernelGGL((level_fused_top_kernel<T, 3>), grid, dim3(kThreads), 0, s, nodes, S); break;
+            default: hipLaunchKernelGGL((level_fused_top_kernel<T, 4>), grid, dim3(kThreads), 0, s, nodes, S); break;
+        }
+    } else {
+        // small trees (fewer than 32 leaves at the fill level)
+        hipLaunchKernelGGL(level_top_kernel<T>, dim3(1), dim3(kThreads), 0, s, nodes, start, end, S);
+    }
+    EX_HIP(hipGetLastError());
+}
+template void level_sums<double>(hipStream_t, const TreeGeom&, int, double*);
+template void level_sums<float>(hipStream_t, const TreeGeom&, int, float*);
+
+template <typename T>
+void unpack_root(hipStream_t s, const T* nodes, int R, double* G) {
+    hipLaunchKernelGGL(unpack_root_kernel<T>, dim3(1), dim3(kThreads), 0, s, nodes, R, G);
+    EX_HIP(hipGetLastError());
+}
+template void unpack_root<double>(hipStream_t, const double*, int, double*);
+template void unpack_root<float>(hipStream_t, const float*, int, double*);
+
 void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double* nodes) {
     const int S = tri_stride(R);
     if (R <= 32) {
@@ -421,17 +524,7 @@ void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double
         hipLaunchKernelGGL(leaf_gram_kernel<1>, dim3(nb), dim3(64), 0, s, U, g.n, R, g, nodes);
     }
     EX_HIP(hipGetLastError());
-    int64_t first_leaf = (int64_t)g.node_count - g.leaf_count;
-    int64_t start = g.nodes_before_lfill, end = first_leaf;
-    for (int64_t lvl = g.lfill_level; lvl >= 0; lvl--) {
-        if (end > start) {
-            hipLaunchKernelGGL(level_sum_kernel, dim3(grid_for((end - start) * S, kThreads, 8192)), dim3(kThreads), 0,
-                               s, nodes, start, end, S);
-        }
-        end = start;
-        start = ((start + 1) / 2) - 1;
-    }
-    EX_HIP(hipGetLastError());
+    level_sums<double>(s, g, S, nodes);
 }
 
 void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* modes, const double* pinv_packed,
