@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["api.hip", "kernels_dense.hip", "kernels_sample.hip", "kernels_sparse.hip", "kernels_mixed.hip", "sort.hip"]
-HEADERS = ["common.hpp", "device.hpp"]
+HEADERS = ["common.hpp", "device.hpp", "dsymm.hpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

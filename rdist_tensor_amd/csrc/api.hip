@@ -233,7 +233,7 @@ struct ex_als {
     ex_sampler* s = nullptr;
     int dist = 0, precision = 0;
     bool initialized = false;
-    DevBuf sd, wd, kd, dg, dm, G, Gi, Gi32, Y, scal;
+    DevBuf sd, wd, kd, dg, dm, G, Gi, Gi32, GY, Ms, Y, scal;
     DevBuf tmp[4];
     DevBuf part;
     double t_region[7] = {};
@@ -318,8 +318,8 @@ void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const do
         Span sm(s->ctx, "sts_compute_m");
         if (mixed) k::sts_compute_m_mixed_launch(st, N, j, R, s->grams.as<double>(), s->M.as<double>());
         else k::sts_compute_m(st, N, j, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
-        k::eig_tree_build(st, N, j, R, s->M.as<double>(), s->grams.as<double>(), s->eig.as<double>());
-        if (mixed) k::eig_tree_to_f32(st, N, j, R, s->eig.as<double>(), s->eig32.as<float>());
+        if (mixed) k::eig_tree_build32(st, N, j, R, s->M.as<double>(), s->grams.as<double>(), s->eig32.as<float>());
+        else k::eig_tree_build(st, N, j, R, s->M.as<double>(), s->grams.as<double>(), s->eig.as<double>());
         k::pack_upper(st, s->M.as<double>(), R, s->pinvp.as<double>());
         sm.end();
         k::StsModeArgs m[kMaxModes];
@@ -478,9 +478,13 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     // Region 6: U_j = Y G+, renormalise (:280-289)
     Span s_fu(a->ctx, "factor_update");
     if (mixed) {
+        // U_j = Y Gi diag(1 / sigma) in one pass over Y: sigma from the Gram of Y
         float* Uj = f->U32[j].as<float>();
-        k::dsymm32(st, a->Gi32.as<float>(), a->Y.as<float>(), Ij, R, Uj);
-        k::renormalize32(st, Uj, Ij, R, f->sigma.as<double>(), a->part);
+        a->GY.ensure((size_t)R * R * 8);
+        a->Ms.ensure((size_t)R * R * 4);
+        k::gram_f32in(st, a->Y.as<float>(), Ij, R, a->GY.as<double>(), a->part);
+        k::sigma_from_gram(st, a->GY.as<double>(), a->Gi.as<double>(), R, f->sigma.as<double>(), a->Ms.as<float>());
+        k::dsymm32(st, a->Ms.as<float>(), a->Y.as<float>(), Ij, R, Uj);
         f->wrote32(j);
     } else {
         double* Uj = f->U[j].as<double>();

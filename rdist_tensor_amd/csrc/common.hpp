@@ -131,6 +131,8 @@ struct Ctx {
 namespace k {
 // dense
 void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part);
+// G = A^T A for FP32 A with FP64 products and sums
+void gram_f32in(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part);
 void eigh(hipStream_t s, const double* A, int R, double* lam, double* V, int ncopies = 1);
 void pinv(hipStream_t s, const double* M, int R, double tol, double* out, int ncopies = 1);
 void dagat(hipStream_t s, const double* A, int64_t m, int R, const double* G, double* res);
@@ -158,7 +160,7 @@ template <typename T>
 void unpack_root(hipStream_t s, const T* nodes, int R, double* G);
 void sts_compute_m(hipStream_t s, int N, int j, int R, const double* grams, double* M, double* lam);
 void eig_tree_build(hipStream_t s, int N, int j, int R, const double* M, const double* grams, double* eig_nodes);
-void eig_tree_to_f32(hipStream_t s, int N, int j, int R, const double* eig, float* eig32);
+void eig_tree_build32(hipStream_t s, int N, int j, int R, const double* M, const double* grams, float* eig_nodes);
 struct StsModeArgs {
     const void* rows;      // factor U_k (I_k x R), double or float as the nodes
     const void* nodes;     // packed row-tree nodes (stride tri_stride / tri_stride32)
@@ -205,6 +207,8 @@ void design32(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, i
               const float* const* U, float* dg, float* dm);
 void dsymm32(hipStream_t s, const float* G, const float* Y, int64_t m, int R, float* out);
 void renormalize32(hipStream_t s, float* U, int64_t m, int R, double* sigma, DevBuf& part);
+// sigma_c = ||(Y Gi)[:, c]|| from GY = Y^T Y; Ms = Gi diag(1 / sigma) (FP32)
+void sigma_from_gram(hipStream_t s, const double* GY, const double* Gi, int R, double* sigma, float* Ms);
 void dagat32(hipStream_t s, const float* A, int64_t m, int R, const double* G, double* res);
 // sort / scan primitives (rocPRIM)
 size_t sort_pairs_u64_u32(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* kin, uint64_t* kout,

@@ -75,6 +75,22 @@ __device__ __forceinline__ double readlane_f64(double v, int l) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// Row-blocked mapping of an m x R row-major array onto a 1-D grid: each
+// block iteration covers rows_per = blockDim.x / R rows, thread -> (row
+// offset rl, column c); threads past rows_per * R have rl == rows_per and
+// take no rows.  Avoids 64-bit division in element loops.
+struct RowMap {
+    int rows_per, rl, c;
+    __device__ RowMap(int R) {
+        rows_per = blockDim.x / R;
+        rl = threadIdx.x / R;
+        c = threadIdx.x - rl * R;
+    }
+    __device__ int64_t first() const { return (int64_t)blockIdx.x * rows_per + rl; }
+    __device__ int64_t step() const { return (int64_t)gridDim.x * rows_per; }
+    __device__ bool on() const { return rl < rows_per; }
+};
+
 __device__ __forceinline__ float readlane_f32(float v, int l) {
     return __uint_as_float(readlane_u32(__float_as_uint(v), l));
 }

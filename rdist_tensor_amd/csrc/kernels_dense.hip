@@ -5,76 +5,14 @@
 // common.h:696-746 (parallel_dsymm, compute_DAGAT), distmat.hpp:85-151.
 #include "common.hpp"
 #include "device.hpp"
+#include "dsymm.hpp"
 
 namespace exafac {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kGramRows = 64;      // rows staged per block iteration
-constexpr int kGramMaxBlocks = 512;
 
 // ---------------------------------------------------------------- Gram
-// Stage 1: block b accumulates the rows of chunks b, b+B, ... into part[b]
-// (upper triangle).  Each thread owns a 4 x 4 tile of outputs (tiles with
-// ta > tb are skipped) and one of G = 256 / ceil(R/4)^2 row groups; the row
-// groups are combined in LDS in group order.
-__global__ __launch_bounds__(kThreads) void gram_partial(const double* __restrict__ A, int64_t m, int R,
-                                                         double* __restrict__ part) {
-    __shared__ double tile[kGramRows * kMaxRank];
-    const int R2 = R * R;
-    const int nt = (R + 3) / 4, tpg = nt * nt, G = kThreads / tpg;
-    const int g = threadIdx.x / tpg, t = threadIdx.x % tpg, ta = t / nt, tb = t % nt;
-    const bool active = g < G && ta <= tb;
-    double acc[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int k = 0; k < 4; k++) acc[i][k] = 0.0;
-    const int64_t nchunks = (m + kGramRows - 1) / kGramRows;
-    for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-        const int64_t r0 = ch * kGramRows;
-        const int rows = (int)((m - r0) < kGramRows ? (m - r0) : kGramRows);
-        __syncthreads();
-        for (int e = threadIdx.x; e < rows * R; e += kThreads) tile[e] = A[r0 * R + e];
-        __syncthreads();
-        if (active) {
-            for (int r = g; r < rows; r += G) {
-                double va[4], vb[4];
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int ca = 4 * ta + i, cb = 4 * tb + i;
-                    va[i] = ca < R ? tile[r * R + ca] : 0.0;
-                    vb[i] = cb < R ? tile[r * R + cb] : 0.0;
-                }
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-#pragma unroll
-                    for (int k = 0; k < 4; k++) acc[i][k] += va[i] * vb[k];
-            }
-        }
-    }
-    // combine the row groups: red[g][a][b]
-    __syncthreads();
-    double* red = tile;   // G * R2 <= kThreads * 16 = kGramRows * kMaxRank doubles
-    if (active) {
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int a = 4 * ta + i, b = 4 * tb + k;
-                if (a < R && b < R) red[g * R2 + a * R + b] = acc[i][k];
-            }
-    }
-    __syncthreads();
-    for (int o = threadIdx.x; o < R2; o += kThreads) {
-        const int a = o / R, b = o % R;
-        if (a > b) continue;
-        double s = 0.0;
-        for (int q = 0; q < G; q++) s += red[q * R2 + o];
-        part[(int64_t)blockIdx.x * R2 + o] = s;
-    }
-}
-
 // Stage 2: G[a][b] = sum over the partials (upper triangle), one wavefront per
 // output entry: lane l sums partials l, l+64, ... in order, then a fixed
 // butterfly combines the lanes.  Lower triangle mirrored.
@@ -90,6 +28,67 @@ __global__ __launch_bounds__(kThreads) void gram_final(const double* __restrict_
     for (int p = lane; p < nparts; p += 64) s += part[(int64_t)p * R2 + src];
     s = wave_sum(s);
     if (lane == 0) G[o] = s;
+}
+
+// ---------------------------------------------------------------- Gram (FP64 MFMA)
+// Block partials of A^T A on v_mfma_f64_16x16x4f64 (FP64 products and sums;
+// TA = double, or float converted exactly).  Operand layout: lane l holds
+// X[r0 + l/16][16 t + l%16] for column tile t, which is both the A operand of
+// tile (t, .) and the B operand of tile (., t); the accumulator holds
+// C[4 i + l/16][l%16], i < 4.  Each wavefront takes 16-row chunks w, w+4, ...
+// of the block's rows (four k-steps of 4 rows); the four wavefronts' tiles
+// are added in wave order in LDS and the block writes its upper triangle.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename TA, int NT>
+__global__ __launch_bounds__(kThreads) void gram_mfma64_partial(const TA* __restrict__ A, int64_t m, int R,
+                                                                double* __restrict__ part) {
+    __shared__ double red[kMaxRank * kMaxRank];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int kr = lane / 16, kc = lane % 16;
+    f64x4 acc[NT][NT];
+#pragma unroll
+    for (int ta = 0; ta < NT; ta++)
+#pragma unroll
+        for (int tb = 0; tb < NT; tb++) acc[ta][tb] = f64x4{0.0, 0.0, 0.0, 0.0};
+    const int64_t nchunks = (m + 15) / 16;
+    for (int64_t ch = (int64_t)blockIdx.x * 4 + wave; ch < nchunks; ch += (int64_t)gridDim.x * 4) {
+        double x[4][NT];
+#pragma unroll
+        for (int st = 0; st < 4; st++) {
+            const int64_t r = ch * 16 + 4 * st + kr;
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const int c = 16 * t + kc;
+                x[st][t] = (r < m && c < R) ? (double)A[r * R + c] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int st = 0; st < 4; st++)
+#pragma unroll
+            for (int ta = 0; ta < NT; ta++)
+#pragma unroll
+                for (int tb = ta; tb < NT; tb++)
+                    acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[st][ta], x[st][tb], acc[ta][tb], 0, 0, 0);
+    }
+    for (int w = 0; w < 4; w++) {
+        if (wave == w) {
+#pragma unroll
+            for (int ta = 0; ta < NT; ta++)
+#pragma unroll
+                for (int tb = ta; tb < NT; tb++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const int a = 16 * ta + 4 * i + kr, b = 16 * tb + kc;
+                        if (a < R && b < R) red[a * R + b] = w == 0 ? acc[ta][tb][i] : red[a * R + b] + acc[ta][tb][i];
+                    }
+        }
+        __syncthreads();
+    }
+    for (int o = threadIdx.x; o < R * R; o += kThreads) {
+        const int a = o / R, b = o % R;
+        if (a <= b) part[(int64_t)blockIdx.x * R * R + o] = red[o];
+    }
 }
 
 // ---------------------------------------------------------------- Jacobi
@@ -439,66 +438,52 @@ __global__ __launch_bounds__(kJacobiThreads) void sts_compute_m_kernel(int N, in
 // Eigen trees (PartitionTree(R, 1, R) per non-target mode k): leaves are the
 // outer products of the rows of M[offset(k)], internal nodes sum their
 // children; every node is then multiplied elementwise by G_k
-// (sts_cp.hpp:179-187, partition_tree.hpp:117-187).  One block per mode.
+// (sts_cp.hpp:179-187, partition_tree.hpp:117-187).
+//
+// Block (x, k) builds packed elements [64 x, 64 x + 64) of every node of mode
+// k's tree in LDS (node-major), level by level in the heap order the row trees
+// use, and writes them in the output precision TO with the node stride's
+// padding zeroed.
+// kEigCols columns per block keep the LDS tile under 64 KB: 64 for R <= 45,
+// 32 above.
+inline int eig_cols(int R) { return R <= 45 ? 64 : 32; }
+template <typename TO>
 __global__ __launch_bounds__(kThreads) void eig_tree_kernel(int N, int j, int R, const double* __restrict__ M,
                                                             const double* __restrict__ grams, TreeGeom ge,
-                                                            double* __restrict__ eig_nodes) {
-    __shared__ uint16_t pr[kMaxRank * (kMaxRank + 1) / 2];
-    const int k = blockIdx.x;
+                                                            TO* __restrict__ eig_nodes, int S, int kEigCols) {
+    extern __shared__ double T[];   // node_count x kEigCols
+    const int k = blockIdx.y;
     if (k == j || k >= N) return;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int R2 = R * R, P = tri_size(R), S = tri_stride(R);
-    for (int e = tid; e < P; e += nt) {
-        int a = 0;
+    const int P = tri_size(R), R2 = R * R;
+    const int col = threadIdx.x % kEigCols, grp = threadIdx.x / kEigCols, ngrp = blockDim.x / kEigCols;
+    const int e = blockIdx.x * kEigCols + col;
+    int a = 0, b = 0;
+    if (e < P) {
         while (a + 1 < R && tri_index(R, a + 1, a + 1) <= e) a++;
-        pr[e] = (uint16_t)(a | ((a + (e - tri_index(R, a, a))) << 8));
+        b = a + (e - tri_index(R, a, a));
     }
-    __syncthreads();
     const int off = (k + 1 == j) ? k + 2 : k + 1;
     const double* W = M + (int64_t)off * R2;
-    double* T = eig_nodes + (int64_t)k * ge.node_count * S;
-    const int first_leaf = (int)ge.node_count - (int)ge.leaf_count;
-    // leaves: outer products of the rows of W (padding of the node stride zeroed)
-    for (int i = 0; i < (int)ge.leaf_count; i++) {
+    const int nodes = (int)ge.node_count;
+    const int first_leaf = nodes - (int)ge.leaf_count;
+    for (int i = grp; i < (int)ge.leaf_count; i += ngrp) {
         const int c = first_leaf + i;
         const int v = (int)ge.leaf_idx(c);
-        for (int e = tid; e < S; e += nt) {
-            double x = 0.0;
-            if (e < P) {
-                const int a = pr[e] & 0xFF, b = pr[e] >> 8;
-                x = W[v * R + a] * W[v * R + b];
-            }
-            T[(int64_t)c * S + e] = x;
-        }
+        T[c * kEigCols + col] = e < P ? W[v * R + a] * W[v * R + b] : 0.0;
     }
     __syncthreads();
     int start = (int)ge.nodes_before_lfill, end = first_leaf;
     for (int lvl = (int)ge.lfill_level; lvl >= 0; lvl--) {
-        for (int c = start; c < end; c++)
-            for (int e = tid; e < S; e += nt)
-                T[(int64_t)c * S + e] = 0.0 + (T[(int64_t)(2 * c + 1) * S + e] + T[(int64_t)(2 * c + 2) * S + e]);
+        for (int c = start + grp; c < end; c += ngrp)
+            T[c * kEigCols + col] = 0.0 + (T[(2 * c + 1) * kEigCols + col] + T[(2 * c + 2) * kEigCols + col]);
         __syncthreads();
         end = start;
         start = ((start + 1) / 2) - 1;
     }
-    const double* G = grams + (int64_t)k * R2;
-    for (int c = 0; c < (int)ge.node_count; c++)
-        for (int e = tid; e < P; e += nt) {
-            const int a = pr[e] & 0xFF, b = pr[e] >> 8;
-            T[(int64_t)c * S + e] *= G[a * R + b];
-        }
-}
-
-// FP32 copy of the eigen trees (node stride tri_stride32), one block per mode.
-__global__ __launch_bounds__(kThreads) void eig_tree_f32_kernel(int N, int j, int R, int nodes,
-                                                                const double* __restrict__ eig,
-                                                                float* __restrict__ eig32) {
-    const int k = blockIdx.x;
-    if (k == j || k >= N) return;
-    const int P = tri_size(R), S = tri_stride(R), S32 = tri_stride32(R);
-    for (int c = 0; c < nodes; c++)
-        for (int e = threadIdx.x; e < S32; e += blockDim.x)
-            eig32[((int64_t)k * nodes + c) * S32 + e] = e < P ? (float)eig[((int64_t)k * nodes + c) * S + e] : 0.0f;
+    if (e >= S) return;
+    const double g = e < P ? grams[(int64_t)k * R2 + a * R + b] : 0.0;
+    TO* out = eig_nodes + (int64_t)k * nodes * S;
+    for (int c = grp; c < nodes; c += ngrp) out[(int64_t)c * S + e] = (TO)(T[c * kEigCols + col] * g);
 }
 
 // ---------------------------------------------------------------- row-wise
@@ -520,22 +505,6 @@ __global__ __launch_bounds__(kThreads) void dagat_kernel(const double* __restric
     }
 }
 
-// out = Y G (G symmetric, full storage).  One thread per output element.
-__global__ __launch_bounds__(kThreads) void dsymm_kernel(const double* __restrict__ G, const double* __restrict__ Y,
-                                                         int64_t m, int R, double* __restrict__ out) {
-    __shared__ double g[kMaxRank * kMaxRank];
-    for (int e = threadIdx.x; e < R * R; e += blockDim.x) g[e] = G[e];
-    __syncthreads();
-    const int64_t total = m * R;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
-        int64_t i = o / R;
-        int c = (int)(o % R);
-        const double* y = Y + i * R;
-        double acc = 0.0;
-        for (int b = 0; b < R; b++) acc += y[b] * g[b * R + c];
-        out[o] = acc;
-    }
-}
 
 // Column sums of squares, stage 1: part[block][c].
 __global__ __launch_bounds__(kThreads) void col_sumsq_partial(const double* __restrict__ U, int64_t m, int R,
@@ -577,9 +546,10 @@ __global__ __launch_bounds__(kThreads) void col_sumsq_final(const double* __rest
 
 __global__ __launch_bounds__(kThreads) void scale_cols_kernel(double* __restrict__ U, int64_t m, int R,
                                                               const double* __restrict__ sigma) {
-    const int64_t total = m * R;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x)
-        U[o] = U[o] / sigma[o % R];
+    const RowMap rm(R);
+    if (!rm.on()) return;
+    const double sc = sigma[rm.c];
+    for (int64_t i = rm.first(); i < m; i += rm.step()) U[i * R + rm.c] = U[i * R + rm.c] / sc;
 }
 
 __global__ __launch_bounds__(kThreads) void cos_init_kernel(double* __restrict__ U, int64_t m, int R) {
@@ -669,12 +639,27 @@ __global__ void pack_upper_kernel(const double* __restrict__ M, int R, double* _
 
 namespace k {
 
-void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part) {
-    int nb = (int)grid_for(m, kGramRows, kGramMaxBlocks);
+template <typename TA>
+static void gram_mfma64(hipStream_t s, const TA* A, int64_t m, int R, double* G, DevBuf& part) {
+    const int nb = (int)grid_for((m + 15) / 16, 4, 1024);
     part.ensure((size_t)nb * R * R * sizeof(double));
-    hipLaunchKernelGGL(gram_partial, dim3(nb), dim3(kThreads), 0, s, A, m, R, part.as<double>());
+    const dim3 grid(nb), block(kThreads);
+    switch ((R + 15) / 16) {
+        case 1: hipLaunchKernelGGL((gram_mfma64_partial<TA, 1>), grid, block, 0, s, A, m, R, part.as<double>()); break;
+        case 2: hipLaunchKernelGGL((gram_mfma64_partial<TA, 2>), grid, block, 0, s, A, m, R, part.as<double>()); break;
+        case 3: hipLaunchKernelGGL((gram_mfma64_partial<TA, 3>), grid, block, 0, s, A, m, R, part.as<double>()); break;
+        default: hipLaunchKernelGGL((gram_mfma64_partial<TA, 4>), grid, block, 0, s, A, m, R, part.as<double>()); break;
+    }
     EX_HIP(hipGetLastError());
     gram_final_launch(s, part.as<double>(), nb, R, G);
+}
+
+void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part) {
+    gram_mfma64<double>(s, A, m, R, G, part);
+}
+
+void gram_f32in(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part) {
+    gram_mfma64<float>(s, A, m, R, G, part);
 }
 
 void gram_final_launch(hipStream_t s, const double* part, int nb, int R, double* G) {
@@ -716,8 +701,7 @@ void dagat(hipStream_t s, const double* A, int64_t m, int R, const double* G, do
 }
 
 void dsymm(hipStream_t s, const double* G, const double* Y, int64_t m, int R, double* out) {
-    hipLaunchKernelGGL(dsymm_kernel, dim3(grid_for(m * R, kThreads, 8192)), dim3(kThreads), 0, s, G, Y, m, R, out);
-    EX_HIP(hipGetLastError());
+    dsymm_tile<double>(s, G, Y, m, R, out);
 }
 
 void col_sumsq(hipStream_t s, const double* U, int64_t m, int R, double* sigma, DevBuf& part) {
@@ -773,14 +757,20 @@ void sts_compute_m(hipStream_t s, int N, int j, int R, const double* grams, doub
 void eig_tree_build(hipStream_t s, int N, int j, int R, const double* M, const double* grams, double* eig_nodes) {
     TreeGeom ge;
     ge.init(R, 1);
-    hipLaunchKernelGGL(eig_tree_kernel, dim3(N), dim3(kThreads), 0, s, N, j, R, M, grams, ge, eig_nodes);
+    const int S = tri_stride(R), cols = eig_cols(R);
+    const size_t lds = (size_t)ge.node_count * cols * sizeof(double);
+    hipLaunchKernelGGL(eig_tree_kernel<double>, dim3((S + cols - 1) / cols, N), dim3(kThreads), lds, s, N, j, R, M,
+                       grams, ge, eig_nodes, S, cols);
     EX_HIP(hipGetLastError());
 }
 
-void eig_tree_to_f32(hipStream_t s, int N, int j, int R, const double* eig, float* eig32) {
+void eig_tree_build32(hipStream_t s, int N, int j, int R, const double* M, const double* grams, float* eig_nodes) {
     TreeGeom ge;
     ge.init(R, 1);
-    hipLaunchKernelGGL(eig_tree_f32_kernel, dim3(N), dim3(kThreads), 0, s, N, j, R, (int)ge.node_count, eig, eig32);
+    const int S = tri_stride32(R), cols = eig_cols(R);
+    const size_t lds = (size_t)ge.node_count * cols * sizeof(double);
+    hipLaunchKernelGGL(eig_tree_kernel<float>, dim3((S + cols - 1) / cols, N), dim3(kThreads), lds, s, N, j, R, M,
+                       grams, ge, eig_nodes, S, cols);
     EX_HIP(hipGetLastError());
 }
 

@@ -8,6 +8,7 @@
 //   C[4 (l / 16) + i][l % 16], i < 4.
 #include "common.hpp"
 #include "device.hpp"
+#include "dsymm.hpp"
 
 namespace exafac {
 namespace {
@@ -375,10 +376,10 @@ struct DesignArgs32 {
 __global__ __launch_bounds__(kThreads) void design_f32(const uint32_t* __restrict__ sd, const double* __restrict__ wd,
                                                        int64_t Ju, DesignArgs32 a, float* __restrict__ dg,
                                                        float* __restrict__ dm) {
-    const int64_t total = Ju * a.R;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t c = o / a.R;
-        const int r = (int)(o % a.R);
+    const RowMap rm(a.R);
+    if (!rm.on()) return;
+    const int r = rm.c;
+    for (int64_t c = rm.first(); c < Ju; c += rm.step()) {
         float d = 1.0f;
         for (int k = 0; k < a.N; k++) {
             if (k == a.j) continue;
@@ -386,25 +387,36 @@ __global__ __launch_bounds__(kThreads) void design_f32(const uint32_t* __restric
         }
         const float sw = (float)sqrt(wd[c]);
         d *= sw;
-        dg[o] = d;
-        dm[o] = d * sw;
+        dg[c * a.R + r] = d;
+        dm[c * a.R + r] = d * sw;
     }
 }
 
-__global__ __launch_bounds__(kThreads) void dsymm_f32(const float* __restrict__ G, const float* __restrict__ Y, int64_t m,
-                                                      int R, float* __restrict__ out) {
-    __shared__ float g[kMaxRank * kMaxRank];
-    for (int e = threadIdx.x; e < R * R; e += blockDim.x) g[e] = G[e];
-    __syncthreads();
-    const int64_t total = m * R;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = o / R;
-        const int c = (int)(o % R);
-        const float* y = Y + i * R;
-        float acc = 0.0f;
-        for (int b = 0; b < R; b++) acc += y[b] * g[b * R + c];
-        out[o] = acc;
+
+// Column norms of U = Y Gi from the Gram of Y: sigma_c^2 = (Gi (Y^T Y) Gi)_cc;
+// Ms = Gi diag(1 / sigma) in FP32, so that Y Ms is U with unit columns.
+__global__ __launch_bounds__(kThreads) void sigma_from_gram_kernel(const double* __restrict__ GY,
+                                                                   const double* __restrict__ Gi, int R,
+                                                                   double* __restrict__ sigma,
+                                                                   float* __restrict__ Ms) {
+    __shared__ double t[kMaxRank * kMaxRank];   // GY Gi
+    __shared__ double sg[kMaxRank];
+    for (int o = threadIdx.x; o < R * R; o += blockDim.x) {
+        const int a = o / R, c = o % R;
+        double acc = 0.0;
+        for (int b = 0; b < R; b++) acc += GY[a * R + b] * Gi[b * R + c];
+        t[o] = acc;
     }
+    __syncthreads();
+    if (threadIdx.x < R) {
+        const int c = threadIdx.x;
+        double acc = 0.0;
+        for (int a = 0; a < R; a++) acc += Gi[a * R + c] * t[a * R + c];
+        sg[c] = sqrt(acc > 0.0 ? acc : 0.0);
+        sigma[c] = sg[c];
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < R * R; o += blockDim.x) Ms[o] = (float)(Gi[o] / sg[o % R]);
 }
 
 // Column sums of squares of an FP32 matrix, accumulated in FP64.
@@ -433,9 +445,10 @@ __global__ __launch_bounds__(kThreads) void col_sumsq_f32(const float* __restric
 
 __global__ __launch_bounds__(kThreads) void scale_cols_f32(float* __restrict__ U, int64_t m, int R,
                                                            const double* __restrict__ sigma) {
-    const int64_t total = m * R;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x)
-        U[o] = (float)((double)U[o] / sigma[o % R]);
+    const RowMap rm(R);
+    if (!rm.on()) return;
+    const double sc = sigma[rm.c];
+    for (int64_t i = rm.first(); i < m; i += rm.step()) U[i * R + rm.c] = (float)((double)U[i * R + rm.c] / sc);
 }
 
 // res_i = u_i^T G u_i with FP32 rows and an FP64 G, accumulated in FP64.
@@ -527,8 +540,7 @@ void design32(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, i
 }
 
 void dsymm32(hipStream_t s, const float* G, const float* Y, int64_t m, int R, float* out) {
-    hipLaunchKernelGGL(dsymm_f32, dim3(grid_for(m * R, kThreads, 8192)), dim3(kThreads), 0, s, G, Y, m, R, out);
-    EX_HIP(hipGetLastError());
+    dsymm_tile<float>(s, G, Y, m, R, out);
 }
 
 void renormalize32(hipStream_t s, float* U, int64_t m, int R, double* sigma, DevBuf& part) {
@@ -537,6 +549,11 @@ void renormalize32(hipStream_t s, float* U, int64_t m, int R, double* sigma, Dev
     hipLaunchKernelGGL(col_sumsq_f32, dim3(nb), dim3(kThreads), 0, s, U, m, R, part.as<double>());
     col_sumsq_final_launch(s, part.as<double>(), nb, R, sigma);
     hipLaunchKernelGGL(scale_cols_f32, dim3(grid_for(m * R, kThreads, 8192)), dim3(kThreads), 0, s, U, m, R, sigma);
+    EX_HIP(hipGetLastError());
+}
+
+void sigma_from_gram(hipStream_t s, const double* GY, const double* Gi, int R, double* sigma, float* Ms) {
+    hipLaunchKernelGGL(sigma_from_gram_kernel, dim3(1), dim3(kThreads), 0, s, GY, Gi, R, sigma, Ms);
     EX_HIP(hipGetLastError());
 }
 

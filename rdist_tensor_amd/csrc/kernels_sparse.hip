@@ -127,10 +127,10 @@ struct DesignArgs {
 __global__ __launch_bounds__(kThreads) void design_kernel(const uint32_t* __restrict__ sd, const double* __restrict__ wd,
                                                           int64_t Ju, DesignArgs a, double* __restrict__ dg,
                                                           double* __restrict__ dm) {
-    const int64_t total = Ju * a.R;
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c = o / a.R;
-        int r = (int)(o % a.R);
+    const RowMap rm(a.R);
+    if (!rm.on()) return;
+    const int r = rm.c;
+    for (int64_t c = rm.first(); c < Ju; c += rm.step()) {
         double d = 1.0;
         for (int k = 0; k < a.N; k++) {
             if (k == a.j) continue;
@@ -138,8 +138,8 @@ __global__ __launch_bounds__(kThreads) void design_kernel(const uint32_t* __rest
         }
         double sw = sqrt(wd[c]);
         d *= sw;
-        dg[o] = d;
-        dm[o] = d * sw;
+        dg[c * a.R + r] = d;
+        dm[c * a.R + r] = d * sw;
     }
 }
 
@@ -172,10 +172,15 @@ __global__ void total_from_scan(const uint32_t* __restrict__ off, const uint32_t
 
 // One thread per hit h: owner sample c (last off[c] <= h with len > 0), the
 // nonzero p, the output row.
+// Hit h -> (output row, payload).  The payload's high word is the sample c;
+// the low word is the lookup position p (FP64 values, gathered by the
+// reduction) or, with FP32 values, the value's bits, read here where the
+// positions of a fiber are consecutive.
+template <typename V>
 __global__ __launch_bounds__(kThreads) void expand_hits(const uint32_t* __restrict__ off,
                                                         const uint32_t* __restrict__ start, int64_t Ju, int64_t H,
-                                                        const uint32_t* __restrict__ rowj, uint32_t* __restrict__ hrow,
-                                                        uint64_t* __restrict__ hpay) {
+                                                        const uint32_t* __restrict__ rowj, const V* __restrict__ vals,
+                                                        uint32_t* __restrict__ hrow, uint64_t* __restrict__ hpay) {
     for (int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; h < H; h += (int64_t)gridDim.x * blockDim.x) {
         int64_t lo = 0, hi = Ju;   // upper_bound(off, h) - 1
         while (lo < hi) {
@@ -185,7 +190,8 @@ __global__ __launch_bounds__(kThreads) void expand_hits(const uint32_t* __restri
         int64_t c = lo - 1;
         uint32_t p = start[c] + (uint32_t)(h - off[c]);
         hrow[h] = rowj[p];
-        hpay[h] = ((uint64_t)c << 32) | p;
+        const uint32_t low = sizeof(V) == 4 ? __float_as_uint((float)vals[p]) : p;
+        hpay[h] = ((uint64_t)c << 32) | low;
     }
 }
 
@@ -221,7 +227,8 @@ __global__ __launch_bounds__(kThreads) void seg_chunk_reduce(const uint32_t* __r
                 const uint64_t pay = hpay[h];
                 hr[m] = hrow[h];
                 hc[m] = (uint32_t)(pay >> 32);
-                hx[m] = val[(uint32_t)pay];
+                if constexpr (sizeof(V) == 4) hx[m] = __uint_as_float((uint32_t)pay);
+                else hx[m] = val[(uint32_t)pay];
             } else {
                 hr[m] = 0;
                 hc[m] = 0;
@@ -493,8 +500,8 @@ int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const V* vals, const 
     uint32_t* r1 = (uint32_t*)(c2 + o_r1);
     uint64_t* p0 = (uint64_t*)(c2 + o_p0);
     uint64_t* p1 = (uint64_t*)(c2 + o_p1);
-    hipLaunchKernelGGL(expand_hits, dim3(grid_for(H, kThreads, 65536)), dim3(kThreads), 0, s, off, start, Ju, H,
-                       L.rowj.as<uint32_t>(), r0, p0);
+    hipLaunchKernelGGL(expand_hits<V>, dim3(grid_for(H, kThreads, 65536)), dim3(kThreads), 0, s, off, start, Ju, H,
+                       L.rowj.as<uint32_t>(), vals, r0, p0);
     sort_pairs_u32_u64(s, c2 + o_t2, sortH, r0, r1, p0, p1, H, rbits);
     const int64_t nch = (H + kSegChunk - 1) / kSegChunk;
     tmp[3].ensure((size_t)nch * 2 * R * sizeof(V) + (size_t)nch * 2 * 4 + 64);
