@@ -67,19 +67,19 @@ def sts_draw_bytes(dims, R, J, j, s=8):
     return total
 
 
-def mttkrp_bytes(N, R, Ju, H, T, s=8):
-    """Algorithmic bytes of sampled MTTKRP + KRP gather per mode step:
-    sample tuples and weights, gathered factor rows, touched nonzeros
-    (coordinates + value), output rows.  s = bytes per factor / value / output
-    entry (8 FP64, 4 mixed); weights are FP64 on both paths."""
-    return Ju * (4 * (N - 1) + 8) + Ju * (N - 1) * R * s + H * (4 * N + s) + T * R * s
+def mttkrp_bytes(R, Ju, H, T, nnz, nfib, I, s=8):
+    """Algorithmic bytes of one sampled MTTKRP launch group: the fiber marks
+    (Ju keys), the row pass (row_scan_bytes) and the zero fill of the output
+    (I x R).  s = bytes per value / design / output entry (8 FP64, 4 mixed)."""
+    return Ju * 8 + row_scan_bytes(R, Ju, H, T, nnz, nfib, s) + I * R * s
 
 
-def seg_reduce_bytes(R, Ju, H, T, s=8):
-    """Algorithmic bytes of the row-owned reduction kernel: per hit its row
-    key (4), packed (sample, nonzero) index (8) and value (s); each design
-    row once; each touched output row once; segment offsets."""
-    return H * (4 + 8 + s) + Ju * R * s + T * R * s + (T + 1) * 4
+def row_scan_bytes(R, Ju, H, T, nnz, nfib, s=8):
+    """Algorithmic bytes of the sampled-MTTKRP row pass: every nonzero's row
+    and fiber number (4 + 4), the sampled-fiber bitmap once (nfib / 8), per
+    hit its design-row index (4) and value (s), each design row once, each
+    touched output row once."""
+    return nnz * 8 + nfib / 8 + H * (4 + s) + Ju * R * s + T * R * s
 
 
 def run_gpu(args, cfg, ctx, T, U0, precision):
@@ -137,8 +137,9 @@ def summarize(args, cfg, g, vbytes):
     avg_s = st["ms"] / st["count"] * 1e-3
     if name == "sts_draw":
         per = np.mean([sts_draw_bytes(cfg["dims"], R, J, j, s=vbytes) for j in range(N)])
-    elif name == "seg_reduce":
-        per = seg_reduce_bytes(R, cnt["J_u"] / steps, cnt["H"] / steps, cnt["T"] / steps, s=vbytes)
+    elif name == "row_scan":
+        per = row_scan_bytes(R, cnt["J_u"] / steps, cnt["H"] / steps, cnt["T"] / steps, cnt["nnz"] / steps,
+                             cnt["nfib"] / steps, s=vbytes)
     else:
         per = float("nan")
     achieved = per / avg_s / 1e9
@@ -148,7 +149,8 @@ def summarize(args, cfg, g, vbytes):
     mt = spans.get("sampled_mttkrp")
     mt_gbs = None
     if mt and cnt.get("mode_steps"):
-        per_mt = mttkrp_bytes(N, R, cnt["J_u"] / steps, cnt["H"] / steps, cnt["T"] / steps, s=vbytes)
+        per_mt = mttkrp_bytes(R, cnt["J_u"] / steps, cnt["H"] / steps, cnt["T"] / steps, cnt["nnz"] / steps,
+                              cnt["nfib"] / steps, cnt["I"] / steps, s=vbytes)
         mt_gbs = per_mt / (mt["ms"] / mt["count"] * 1e-3) / 1e9
     return dict(value=K / g["elapsed"], ms_per_step=g["elapsed"] / K * 1e3, roofline=roof,
                 sampled_mttkrp_gbs=None if mt_gbs is None else round(mt_gbs, 2),

@@ -3,6 +3,7 @@
 //
 // Reference flow: ALS_Optimizer::execute_ALS_rounds (als_optimizer.hpp:40-124)
 // over AccumulatorStationary::execute_ALS_step (accumulator_stationary.hpp:139-295).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -236,8 +237,10 @@ struct ex_als {
     DevBuf sd, wd, kd, dg, dm, G, Gi, Gi32, GY, Ms, Y, scal;
     DevBuf tmp[4];
     DevBuf part;
+    DevBuf cnt;                  // u64[2]: hits (nonzeros iterated) and touched rows, summed over steps
+    uint64_t cnt_seen[2] = {};   // the part of cnt already reported to timing counters
+    uint64_t cnt_base = 0;       // hits at the start of the current execute_ALS_rounds
     double t_region[7] = {};
-    double nonzeros = 0.0;
     std::string stats = "{}";
 };
 
@@ -345,29 +348,9 @@ void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const do
 void tensor_build_lookups(ex_tensor* t) {
     if (t->built) return;
     hipStream_t st = t->ctx->c.stream;
-    const int64_t nnz = t->nnz;
-    for (int mode = 0; mode < t->N; mode++) {
-        ModeLookup& L = t->lk[mode];
-        L.mode = mode;
-        L.kl = make_key_layout(t->N, mode, t->dims);
-        EX_CHECK(L.kl.bits <= 64, EX_ERR_UNSUPPORTED,
-                 "coordinates of the non-target modes need " + std::to_string(L.kl.bits) + " > 64 key bits");
-        size_t sort_bytes = k::sort_pairs_u64_u32(st, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nnz, L.kl.bits);
-        t->tmp[0].ensure((size_t)nnz * 16 + 512);
-        t->tmp[1].ensure((size_t)nnz * 8 + 512);
-        t->tmp[2].ensure(sort_bytes + 256);
-        uint64_t* kin = t->tmp[0].as<uint64_t>();
-        uint64_t* kout = kin + nnz;
-        uint32_t* iin = t->tmp[1].as<uint32_t>();
-        uint32_t* iout = iin + nnz;
-        k::keys_from_coords(st, t->coords.as<uint32_t>(), nnz, L.kl, kin, iin);
-        if (nnz) k::sort_pairs_u64_u32(st, t->tmp[2].p, t->tmp[2].bytes, kin, kout, iin, iout, nnz, L.kl.bits);
-        L.rowj.ensure((size_t)nnz * 4);
-        L.val.ensure((size_t)nnz * 8);
-        k::gather_lookup(st, iout, t->coords.as<uint32_t>(), t->vals.as<double>(), nnz, t->N, mode,
-                         L.rowj.as<uint32_t>(), L.val.as<double>());
-        L.nfib = k::fiber_index(st, kout, nnz, L.fkey, L.fptr, t->tmp[3]);
-    }
+    for (int mode = 0; mode < t->N; mode++)
+        k::build_lookup(st, t->coords.as<uint32_t>(), t->vals.as<double>(), t->nnz, t->N, mode, t->dims, t->lk[mode],
+                        t->tmp);
     EX_HIP(hipStreamSynchronize(st));
     for (auto& b : t->tmp) b.release();
     t->built = true;
@@ -444,34 +427,28 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     // Region 4: sampled MTTKRP (:252-274)
     a->Y.ensure((size_t)Ij * R * vb);
     ModeLookup& L = a->t->lk[j];
-    if (mixed && !L.val32.p) {
-        L.val32.ensure((size_t)(a->t->nnz > 0 ? a->t->nnz : 1) * 4);
-        k::to_f32(st, L.val.as<double>(), a->t->nnz, L.val32.as<float>());
+    if (mixed && !L.rval32.p) {
+        L.rval32.ensure((size_t)(L.nnz > 0 ? L.nnz : 1) * 4);
+        k::to_f32(st, L.rval.as<double>(), L.nnz, L.rval32.as<float>());
     }
     Span s_mt(a->ctx, "sampled_mttkrp");
-    int64_t Tn = 0;
-    hipEvent_t seg_ev[2] = {nullptr, nullptr};
+    hipEvent_t scan_ev[2] = {nullptr, nullptr};
     if (a->ctx->timing) {
-        seg_ev[0] = a->ctx->get_event();
-        seg_ev[1] = a->ctx->get_event();
+        scan_ev[0] = a->ctx->get_event();
+        scan_ev[1] = a->ctx->get_event();
     }
-    const int64_t H =
-        mixed ? k::sampled_mttkrp<float>(st, L, L.val32.as<float>(), a->kd.as<uint64_t>(), Ju, a->dm.as<float>(), R,
-                                         a->Y.as<float>(), Ij, a->tmp, a->ctx->timing ? &Tn : nullptr,
-                                         a->ctx->timing ? seg_ev : nullptr)
-              : k::sampled_mttkrp<double>(st, L, L.val.as<double>(), a->kd.as<uint64_t>(), Ju, a->dm.as<double>(),
-                                          R, a->Y.as<double>(), Ij, a->tmp, a->ctx->timing ? &Tn : nullptr,
-                                          a->ctx->timing ? seg_ev : nullptr);
+    // hits and touched rows accumulate on the device (a->cnt)
+    uint64_t* cnt = a->cnt.as<uint64_t>();
+    if (mixed)
+        k::sampled_mttkrp<float>(st, L, L.rval32.as<float>(), a->kd.as<uint64_t>(), Ju, a->dm.as<float>(), R,
+                                 a->Y.as<float>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr);
+    else
+        k::sampled_mttkrp<double>(st, L, L.rval.as<double>(), a->kd.as<uint64_t>(), Ju, a->dm.as<double>(), R,
+                                  a->Y.as<double>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr);
     s_mt.end();
     if (a->ctx->timing) {
-        if (H > 0) a->ctx->pending.push_back({"seg_reduce", {seg_ev[0], seg_ev[1]}});
-        else { a->ctx->pool.push_back(seg_ev[0]); a->ctx->pool.push_back(seg_ev[1]); }
-    }
-    a->nonzeros += (double)H;
-    if (a->ctx->timing) {
+        a->ctx->pending.push_back({"row_scan", {scan_ev[0], scan_ev[1]}});
         a->ctx->add_counter("J_u", (double)Ju);
-        a->ctx->add_counter("H", (double)H);
-        a->ctx->add_counter("T", (double)Tn);
         a->ctx->add_counter("mode_steps", 1.0);
     }
     ev.mark(3, st);
@@ -504,7 +481,18 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     a->t_region[3] += ev.secs(2, 3);
     a->t_region[5] += ev.secs(3, 4);
     a->t_region[6] += ev.secs(4, 5);
-    if (a->ctx->timing) a->ctx->resolve();
+    if (a->ctx->timing) {
+        uint64_t c[2];
+        d2h(st, c, a->cnt.p, sizeof(c));
+        a->ctx->add_counter("H", (double)(c[0] - a->cnt_seen[0]));
+        a->ctx->add_counter("T", (double)(c[1] - a->cnt_seen[1]));
+        a->cnt_seen[0] = c[0];
+        a->cnt_seen[1] = c[1];
+        a->ctx->add_counter("nnz", (double)L.nnz);
+        a->ctx->add_counter("nfib", (double)L.nfib);
+        a->ctx->add_counter("I", (double)Ij);
+        a->ctx->resolve();
+    }
 }
 
 double als_fit_impl(ex_als* a) {
@@ -675,23 +663,37 @@ int ex_sampled_mttkrp(ex_tensor* t, int mode, int64_t Ju, const uint32_t* sample
         tensor_build_lookups(t);
         hipStream_t st = t->ctx->c.stream;
         const int N = t->N;
-        for (int64_t c = 0; c < Ju; c++)
-            for (int k2 = 0; k2 < N; k2++)
-                EX_CHECK(k2 == mode || samples[c * N + k2] < (uint64_t)t->dims[k2], EX_ERR_ARG, "sample index out of range");
-        DevBuf ds, dd, dk, dy;
+        const KeyLayout& kl = t->lk[mode].kl;
+        std::vector<uint64_t> keys((size_t)Ju);
+        for (int64_t c = 0; c < Ju; c++) {
+            uint64_t key = 0;
+            for (int k2 = 0; k2 < N; k2++) {
+                if (k2 == mode) continue;
+                EX_CHECK(samples[c * N + k2] < (uint64_t)t->dims[k2], EX_ERR_ARG, "sample index out of range");
+                key |= (uint64_t)samples[c * N + k2] << kl.shift[k2];
+            }
+            keys[(size_t)c] = key;
+        }
+        std::vector<uint64_t> sorted = keys;
+        std::sort(sorted.begin(), sorted.end());
+        EX_CHECK(std::adjacent_find(sorted.begin(), sorted.end()) == sorted.end(), EX_ERR_ARG,
+                 "samples must be deduplicated: two rows share every non-target coordinate");
+        DevBuf dd, dk, dy, dc;
         DevBuf tmp[4];
-        ds.ensure((size_t)(Ju > 0 ? Ju : 1) * N * 4);
         dd.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
         dk.ensure((size_t)(Ju > 0 ? Ju : 1) * 8);
+        dc.ensure(16);
         const int64_t Ij = t->dims[mode];
         dy.ensure((size_t)Ij * R * 8);
-        h2d(st, ds.p, samples, (size_t)Ju * N * 4);
+        h2d(st, dk.p, keys.data(), (size_t)Ju * 8);
         h2d(st, dd.p, design, (size_t)Ju * R * 8);
-        k::keys_from_coords(st, ds.as<uint32_t>(), Ju, t->lk[mode].kl, dk.as<uint64_t>(), nullptr);
-        int64_t H = k::sampled_mttkrp<double>(st, t->lk[mode], t->lk[mode].val.as<double>(), dk.as<uint64_t>(), Ju,
-                                              dd.as<double>(), (int)R, dy.as<double>(), Ij, tmp);
+        EX_HIP(hipMemsetAsync(dc.p, 0, 16, st));
+        k::sampled_mttkrp<double>(st, t->lk[mode], t->lk[mode].rval.as<double>(), dk.as<uint64_t>(), Ju,
+                                  dd.as<double>(), (int)R, dy.as<double>(), Ij, tmp, dc.as<uint64_t>());
         d2h(st, Y, dy.p, (size_t)Ij * R * 8);
-        if (hits) *hits = H;
+        uint64_t c[2];
+        d2h(st, c, dc.p, sizeof(c));
+        if (hits) *hits = (int64_t)c[0];
     });
 }
 
@@ -925,6 +927,9 @@ int ex_als_create(ex_ctx* ctx, ex_tensor* t, ex_factors* f, ex_sampler* s, int d
         a->s = s;
         a->dist = distribution;
         a->precision = precision;
+        a->cnt.ensure(16);
+        EX_HIP(hipMemsetAsync(a->cnt.p, 0, 16, ctx->c.stream));
+        EX_HIP(hipStreamSynchronize(ctx->c.stream));
         *out = a.release();
     });
 }
@@ -966,7 +971,8 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
         EX_CHECK(epoch >= 1, EX_ERR_ARG, "epoch_interval must be positive");
         hipStream_t st = a->ctx->c.stream;
         for (double& x : a->t_region) x = 0.0;
-        a->nonzeros = 0.0;
+        uint64_t c0[2];
+        d2h(st, c0, a->cnt.p, sizeof(c0));
         double als_total = 0.0, fit_total = 0.0;
         std::vector<int64_t> rl{0};
         std::vector<double> fits{0.0}, als_times{0.0}, fit_times{0.0};
@@ -986,10 +992,13 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
                 fit_times.push_back(fit_total);
             }
         }
+        uint64_t c1[2];
+        d2h(st, c1, a->cnt.p, sizeof(c1));
+        const double nonzeros = (double)(c1[0] - c0[0]);
         std::string js = "{";
         js += "\"als_total_time\": " + json_num(als_total);
         js += ", \"als_times\": " + json_list(als_times);
-        js += ", \"avg_nonzeros_iterated\": " + dstat(a->nonzeros);
+        js += ", \"avg_nonzeros_iterated\": " + dstat(nonzeros);
         js += ", \"dense_reduce_time\": " + dstat(a->t_region[4]);
         js += ", \"design_matrix_prep_time\": " + dstat(a->t_region[2]);
         js += ", \"fit_computation_time\": " + json_num(fit_total);
@@ -1004,7 +1013,7 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
         js += ", \"sampler\": \"" + std::string(a->s->kind == EX_SAMPLER_STS_CP ? "sts_cp" : "cp_arls_lev") + "\"";
         js += ", \"sampler_update_time\": " + dstat(a->t_region[6]);
         js += ", \"spmm_time\": " + dstat(a->t_region[3]);
-        js += ", \"sum_nonzeros_iterated\": " + json_num(a->nonzeros);
+        js += ", \"sum_nonzeros_iterated\": " + json_num(nonzeros);
         js += ", \"device\": \"" + device_label(a->ctx) + "\"";
         js += "}";
         a->stats = js;

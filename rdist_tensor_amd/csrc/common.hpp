@@ -107,16 +107,24 @@ struct KeyLayout {
 };
 KeyLayout make_key_layout(int N, int skip, const int64_t* dims);
 
-// Per-mode sorted nonzero copy (the sampled MTTKRP lookup).
+// Per-mode lookup for the sampled MTTKRP.  A fiber is the set of nonzeros
+// sharing every coordinate but the target mode's; fibers are numbered in
+// ascending key order.  The nonzeros are kept in target-row order (ties in
+// fiber order) with their fiber numbers, so one pass over them, keeping those
+// whose fiber is sampled, produces each output row in a fixed order.
 struct ModeLookup {
     int mode = -1;
     KeyLayout kl;
     int64_t nfib = 0;
-    DevBuf fkey;   // u64[nfib] distinct fiber keys, ascending
-    DevBuf fptr;   // u32[nfib+1] offsets into rowj/val
-    DevBuf rowj;   // u32[nnz] target-mode coordinate
-    DevBuf val;    // f64[nnz]
-    DevBuf val32;  // f32[nnz], built on first use by the mixed-precision path
+    int64_t nnz = 0;
+    DevBuf fkey;    // u64[nfib] distinct fiber keys, ascending
+    DevBuf rrow;    // u32[nnz] target-mode row, ascending
+    DevBuf rfib;    // u32[nnz] fiber number
+    DevBuf rval;    // f64[nnz]
+    DevBuf rval32;  // f32[nnz], built on first use by the mixed-precision path
+    DevBuf cmap;    // u64[nfib]: (stamp << 32) | design row of the fibers sampled at step `stamp`
+    DevBuf filt;    // u32[2^14]: bit (f mod 2^19) set for this step's sampled fibers f (zero between steps)
+    uint32_t stamp = 0;
 };
 
 struct Ctx {
@@ -178,19 +186,21 @@ void pack_upper(hipStream_t s, const double* M, int R, double* packed);
 // sparse
 void keys_from_coords(hipStream_t s, const uint32_t* coords, int64_t n, const KeyLayout& kl, uint64_t* keys,
                       uint32_t* iota);
-void gather_lookup(hipStream_t s, const uint32_t* perm, const uint32_t* coords, const double* vals, int64_t nnz,
-                   int N, int mode, uint32_t* rowj, double* val);
-int64_t fiber_index(hipStream_t s, const uint64_t* sorted_keys, int64_t nnz, DevBuf& fkey, DevBuf& fptr,
-                    DevBuf& tmp);
+// Builds L (fkey, rrow, rfib, rval, bitmap, cmap) from the COO tensor.
+void build_lookup(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, int mode,
+                  const int64_t* dims, ModeLookup& L, DevBuf* tmp);
 int64_t dedup(hipStream_t s, const uint32_t* samples, const double* weights, int64_t J, int N, const KeyLayout& kl,
               uint32_t* sd, double* wd, uint64_t* kd, DevBuf* tmp);
 void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
             const double* const* U, double* d_gram, double* d_mttkrp);
-// V = double (FP64 path) or float (mixed path): nonzero values, design rows and Y.
+// Y (I_j x R, zero-filled here) = sampled MTTKRP of the Ju deduplicated
+// samples with fiber keys kd and design rows `design`.  V = double (FP64
+// path) or float (mixed path): values (L.rval / L.rval32), design rows and Y.
+// counts (optional, device u64[2]) receives the hits and touched rows.
 template <typename V>
-int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju,
-                       const V* design, int R, V* Y, int64_t I_j, DevBuf* tmp, int64_t* T_out = nullptr,
-                       hipEvent_t* seg_events = nullptr);
+void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju, const V* design,
+                    int R, V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts = nullptr,
+                    hipEvent_t* scan_events = nullptr);
 void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
              const double* sigma, int R, double* out, DevBuf& part);
 // final stages shared with the mixed-precision kernels
@@ -215,6 +225,8 @@ size_t sort_pairs_u64_u32(hipStream_t s, void* temp, size_t temp_bytes, const ui
                           const uint32_t* vin, uint32_t* vout, int64_t n, int bits);
 size_t sort_pairs_u32_u64(hipStream_t s, void* temp, size_t temp_bytes, const uint32_t* kin, uint32_t* kout,
                           const uint64_t* vin, uint64_t* vout, int64_t n, int bits);
+size_t sort_pairs_u32_u32(hipStream_t s, void* temp, size_t temp_bytes, const uint32_t* kin, uint32_t* kout,
+                          const uint32_t* vin, uint32_t* vout, int64_t n, int bits);
 size_t exclusive_scan_u32(hipStream_t s, void* temp, size_t temp_bytes, const uint32_t* in, uint32_t* out, int64_t n);
 size_t exclusive_scan_u64(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* in, uint64_t* out, int64_t n);
 }  // namespace k

@@ -49,36 +49,19 @@ __global__ __launch_bounds__(kThreads) void keys_kernel(const uint32_t* __restri
     }
 }
 
-__global__ __launch_bounds__(kThreads) void gather_lookup_kernel(const uint32_t* __restrict__ perm,
-                                                                 const uint32_t* __restrict__ coords,
-                                                                 const double* __restrict__ vals, int64_t nnz, int N,
-                                                                 int mode, uint32_t* __restrict__ rowj,
-                                                                 double* __restrict__ val) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t p = perm[i];
-        rowj[i] = coords[(int64_t)p * N + mode];
-        val[i] = vals[p];
-    }
-}
-
 __global__ __launch_bounds__(kThreads) void head_flags_u64(const uint64_t* __restrict__ keys, int64_t n,
                                                            uint32_t* __restrict__ flags) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
-// Scatter run heads: fkey[pos] = key, fptr[pos] = i; fptr[nruns] = n.
-__global__ __launch_bounds__(kThreads) void fiber_scatter(const uint64_t* __restrict__ keys, int64_t n,
-                                                          const uint32_t* __restrict__ flags,
-                                                          const uint32_t* __restrict__ pos, uint64_t* __restrict__ fkey,
-                                                          uint32_t* __restrict__ fptr) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        if (flags[i]) {
-            fkey[pos[i]] = keys[i];
-            fptr[pos[i]] = (uint32_t)i;
-        }
-        if (i == n - 1) fptr[pos[i] + flags[i]] = (uint32_t)n;
-    }
+// Scatter run heads: fkey[pos] = key.
+__global__ __launch_bounds__(kThreads) void fiber_scatter_keys(const uint64_t* __restrict__ keys, int64_t n,
+                                                               const uint32_t* __restrict__ flags,
+                                                               const uint32_t* __restrict__ pos,
+                                                               uint64_t* __restrict__ fkey) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (flags[i]) fkey[pos[i]] = keys[i];
 }
 
 __global__ void run_count(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, int64_t n,
@@ -143,162 +126,371 @@ __global__ __launch_bounds__(kThreads) void design_kernel(const uint32_t* __rest
     }
 }
 
+// ---------------------------------------------------------------- lookup build
+// Fiber number of every position of the key-sorted array: inclusive scan of
+// the run heads minus one.
+__global__ __launch_bounds__(kThreads) void fiber_numbers(const uint32_t* __restrict__ flags,
+                                                          const uint32_t* __restrict__ pos, int64_t n,
+                                                          uint32_t* __restrict__ fib) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        fib[i] = pos[i] + flags[i] - 1;
+}
+
+// Row-order arrays from the row sort: position i holds original lookup
+// position perm[i].
+__global__ __launch_bounds__(kThreads) void gather_row_order(const uint32_t* __restrict__ perm,
+                                                             const uint32_t* __restrict__ fib_keyorder,
+                                                             const uint32_t* __restrict__ src_of_keypos,
+                                                             const double* __restrict__ vals, int64_t nnz,
+                                                             uint32_t* __restrict__ rfib, double* __restrict__ rval) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t q = perm[i];              // position in key order
+        rfib[i] = fib_keyorder[q];
+        rval[i] = vals[src_of_keypos[q]];        // COO index of that nonzero
+    }
+}
+
+// Target-mode coordinate of each key-ordered position.
+__global__ __launch_bounds__(kThreads) void rows_of_keyorder(const uint32_t* __restrict__ src_of_keypos,
+                                                             const uint32_t* __restrict__ coords, int64_t nnz, int N,
+                                                             int mode, uint32_t* __restrict__ rows,
+                                                             uint32_t* __restrict__ iota) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        rows[i] = coords[(int64_t)src_of_keypos[i] * N + mode];
+        iota[i] = (uint32_t)i;
+    }
+}
+
 // ---------------------------------------------------------------- sampled MTTKRP
-__global__ __launch_bounds__(kThreads) void fiber_lookup(const uint64_t* __restrict__ kd, int64_t Ju,
-                                                         const uint64_t* __restrict__ fkey, int64_t nfib,
-                                                         const uint32_t* __restrict__ fptr,
-                                                         uint32_t* __restrict__ start, uint32_t* __restrict__ len) {
+// Sampled fibers of this step: cmap[f] = (stamp << 32) | design row, where
+// the stamp numbers the step, and bit (f mod 2^19) of a 64 KB filter that the
+// row pass keeps in LDS.  A nonzero is a hit when its fiber's filter bit is
+// set and its cmap entry carries this step's stamp, so cmap never needs
+// clearing.  fsel[c] keeps the fiber number (or ~0u) to clear the filter.
+constexpr int kFilterBits = 1 << 19;
+constexpr int kFilterWords = kFilterBits / 32;
+
+__global__ __launch_bounds__(kThreads) void mark_fibers(const uint64_t* __restrict__ kd, int64_t Ju,
+                                                        const uint64_t* __restrict__ fkey, int64_t nfib,
+                                                        uint32_t stamp, uint64_t* __restrict__ cmap,
+                                                        uint32_t* __restrict__ filt, uint32_t* __restrict__ fsel) {
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < Ju; c += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t key = kd[c];
+        const uint64_t key = kd[c];
         int64_t lo = 0, hi = nfib;
         while (lo < hi) {
-            int64_t mid = lo + (hi - lo) / 2;
+            const int64_t mid = lo + (hi - lo) / 2;
             if (fkey[mid] < key) lo = mid + 1; else hi = mid;
         }
+        uint32_t f = ~0u;
         if (lo < nfib && fkey[lo] == key) {
-            start[c] = fptr[lo];
-            len[c] = fptr[lo + 1] - fptr[lo];
-        } else {
-            start[c] = 0;
-            len[c] = 0;
+            f = (uint32_t)lo;
+            const uint32_t h = f & (kFilterBits - 1);
+            atomicOr(filt + (h >> 5), 1u << (h & 31));
+            cmap[f] = ((uint64_t)stamp << 32) | (uint32_t)c;
         }
+        fsel[c] = f;
     }
 }
 
-__global__ void total_from_scan(const uint32_t* __restrict__ off, const uint32_t* __restrict__ len, int64_t n,
-                                uint64_t* __restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *out = n > 0 ? (uint64_t)off[n - 1] + len[n - 1] : 0;
-}
-
-// One thread per hit h: owner sample c (last off[c] <= h with len > 0), the
-// nonzero p, the output row.
-// Hit h -> (output row, payload).  The payload's high word is the sample c;
-// the low word is the lookup position p (FP64 values, gathered by the
-// reduction) or, with FP32 values, the value's bits, read here where the
-// positions of a fiber are consecutive.
-template <typename V>
-__global__ __launch_bounds__(kThreads) void expand_hits(const uint32_t* __restrict__ off,
-                                                        const uint32_t* __restrict__ start, int64_t Ju, int64_t H,
-                                                        const uint32_t* __restrict__ rowj, const V* __restrict__ vals,
-                                                        uint32_t* __restrict__ hrow, uint64_t* __restrict__ hpay) {
-    for (int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; h < H; h += (int64_t)gridDim.x * blockDim.x) {
-        int64_t lo = 0, hi = Ju;   // upper_bound(off, h) - 1
-        while (lo < hi) {
-            int64_t mid = lo + (hi - lo) / 2;
-            if (off[mid] <= (uint32_t)h) lo = mid + 1; else hi = mid;
-        }
-        int64_t c = lo - 1;
-        uint32_t p = start[c] + (uint32_t)(h - off[c]);
-        hrow[h] = rowj[p];
-        const uint32_t low = sizeof(V) == 4 ? __float_as_uint((float)vals[p]) : p;
-        hpay[h] = ((uint64_t)c << 32) | low;
+// Every set bit belongs to this step's samples, so whole words are cleared.
+__global__ __launch_bounds__(kThreads) void clear_fibers(const uint32_t* __restrict__ fsel, int64_t Ju,
+                                                         uint32_t* __restrict__ filt) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < Ju; c += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t f = fsel[c];
+        if (f != ~0u) filt[(f & (kFilterBits - 1)) >> 5] = 0u;
     }
 }
 
-// Row-owned reduction of the row-sorted hits (sort_lookup.hpp:347-371).
-// One wavefront per chunk of kSegChunk consecutive hits.  A row that begins
-// and ends inside the chunk is summed in hit order and written to Y.  The row
-// running into the chunk from the left leaves its partial sum in carry slot
-// 0; a row that begins in the chunk and runs past its end leaves its partial
-// in slot 1.  seg_fixup adds the pieces in chunk order.  No atomics on Y;
-// the result does not depend on scheduling.  *T counts rows that begin in a
-// chunk (the touched rows).
-constexpr int kSegChunk = 256;
+// Row-owned pass over the nonzeros in row order (sort_lookup.hpp:238-375
+// computes the same sums from the sampled side).  One wavefront per chunk
+// of kScanChunk consecutive nonzeros, four per lane; a nonzero is a hit when
+// its fiber's bit is set.  The hits are taken in order: a row that begins and
+// ends inside the chunk is summed and written to Y.  A row that crosses a
+// chunk boundary leaves partial sums in carry entries: slot 0 of a chunk for
+// the row running in from the left, slot 1 for the row beginning in the chunk
+// and running past its end.  Both slots always carry their row as key (flag
+// 0 when there is no partial), so the carry list is sorted by row and every
+// crossing row's partials are contiguous in it; seg_level reduces that list
+// the same way.  Fixed summation order, no atomics on Y.  counts[0] += hits,
+// counts[1] += rows with a hit that begin here.
+constexpr int kScanChunk = 256;
+constexpr int kHitBatch = 8;   // hits whose loads are issued together
+
+// The filter sits in LDS (64 KB); two 1024-thread workgroups per CU.
+constexpr int kScanThreads = 1024;
 
 template <typename V>
-__global__ __launch_bounds__(kThreads) void seg_chunk_reduce(const uint32_t* __restrict__ hrow,
-                                                             const uint64_t* __restrict__ hpay,
-                                                             const V* __restrict__ val,
-                                                             const V* __restrict__ design, int R, int64_t H,
-                                                             int64_t nch, V* __restrict__ Y,
-                                                             V* __restrict__ carry, uint32_t* __restrict__ cflag,
-                                                             unsigned long long* __restrict__ T) {
+__global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* __restrict__ rrow,
+                                                            const uint32_t* __restrict__ rfib,
+                                                            const V* __restrict__ rval, int64_t nnz, int64_t nch,
+                                                            uint32_t stamp, const uint64_t* __restrict__ cmap,
+                                                            const uint32_t* __restrict__ filt,
+                                                            const V* __restrict__ design, int R, V* __restrict__ Y,
+                                                            V* __restrict__ carry, uint32_t* __restrict__ cflag,
+                                                            uint32_t* __restrict__ ckey,
+                                                            unsigned long long* __restrict__ counts) {
+    constexpr int NT = kScanThreads;
+    __shared__ uint32_t lfilt[kFilterWords];
+    __shared__ unsigned long long wsum[2][NT / 64];
+    for (int i = threadIdx.x; i < kFilterWords; i += NT) lfilt[i] = filt[i];
+    __syncthreads();
     const int lane = threadIdx.x % 64;
-    for (int64_t w = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64; w < nch;
-         w += (int64_t)gridDim.x * (kThreads / 64)) {
-        const int64_t b = w * kSegChunk, e = b + kSegChunk < H ? b + kSegChunk : H;
-        // the chunk's hits, 4 per lane: row, sample index, nonzero value
-        uint32_t hr[kSegChunk / 64], hc[kSegChunk / 64];
-        V hx[kSegChunk / 64];
+    const int lr = lane < R ? lane : 0;
+    constexpr int M = kScanChunk / 64;
+    unsigned long long whits = 0, wheads = 0;   // this wavefront's counts over its chunks
+    for (int64_t w = (int64_t)blockIdx.x * (NT / 64) + threadIdx.x / 64; w < nch;
+         w += (int64_t)gridDim.x * (NT / 64)) {
+        const int64_t b = w * kScanChunk, e = b + kScanChunk < nnz ? b + kScanChunk : nnz;
+        uint32_t hr[M], hc[M];
+        V hx[M];
+        bool hit[M];
 #pragma unroll
-        for (int m = 0; m < kSegChunk / 64; m++) {
-            const int64_t h = b + 64 * m + lane;
-            if (h < e) {
-                const uint64_t pay = hpay[h];
-                hr[m] = hrow[h];
-                hc[m] = (uint32_t)(pay >> 32);
-                if constexpr (sizeof(V) == 4) hx[m] = __uint_as_float((uint32_t)pay);
-                else hx[m] = val[(uint32_t)pay];
-            } else {
-                hr[m] = 0;
-                hc[m] = 0;
-                hx[m] = (V)0;
-            }
-        }
-        uint32_t cur = readlane_u32(hr[0], 0);
-        bool cur_head = (b == 0) || hrow[b - 1] != cur;
-        uint32_t f0 = 0, f1 = 0, heads = 0;
-        V acc = 0;
-        const int lr = lane < R ? lane : 0;
-#pragma unroll
-        for (int m = 0; m < kSegChunk / 64; m++) {
-            const int n = (int)(e - b - 64 * m) < 64 ? (int)(e - b - 64 * m) : 64;
-#pragma unroll 8
-            for (int l = 0; l < n; l++) {
-                const uint32_t r = readlane_u32(hr[m], l);
-                if (r != cur) {
-                    if (cur_head) {
-                        if (lane < R) Y[(int64_t)cur * R + lane] = acc;
-                        heads++;
-                    } else {
-                        if (lane < R) carry[(2 * w) * R + lane] = acc;
-                        f0 = 1;
+        for (int m = 0; m < M; m++) {
+            const int64_t i = b + 64 * m + lane;
+            hr[m] = 0;
+            hc[m] = 0;
+            hx[m] = (V)0;
+            hit[m] = false;
+            if (i < e) {
+                hr[m] = rrow[i];
+                const uint32_t f = rfib[i];
+                const uint32_t h = f & (kFilterBits - 1);
+                if ((lfilt[h >> 5] >> (h & 31)) & 1u) {
+                    const uint64_t cm = cmap[f];
+                    if ((uint32_t)(cm >> 32) == stamp) {
+                        hit[m] = true;
+                        hc[m] = (uint32_t)cm;
+                        hx[m] = rval[i];
                     }
-                    cur = r;
-                    acc = 0;
-                    cur_head = true;
                 }
-                const uint32_t c = readlane_u32(hc[m], l);
-                acc += design[(int64_t)c * R + lr] * readlane_val(hx[m], l);
             }
         }
-        const bool ends = (e == H) || hrow[e] != cur;
-        if (cur_head) heads++;
-        if (ends && cur_head) {
-            if (lane < R) Y[(int64_t)cur * R + lane] = acc;
-        } else if (cur_head) {
-            if (lane < R) carry[(2 * w + 1) * R + lane] = acc;
-            f1 = 1;
-        } else {
-            if (lane < R) carry[(2 * w) * R + lane] = acc;
-            f0 = 1;
+        const uint32_t first_row = readlane_u32(hr[0], 0);
+        const int last = (int)(e - 1 - b);
+        uint32_t last_row = first_row;
+#pragma unroll
+        for (int m = 0; m < M; m++)
+            if (last / 64 == m) last_row = readlane_u32(hr[m], last % 64);
+        const bool first_cont = b > 0 && rrow[b - 1] == first_row;   // first row began before the chunk
+        const bool continues = e < nnz && rrow[e] == last_row;       // last row runs past the chunk
+        uint32_t cur = 0, nheads = 0, nhits = 0;
+        bool active = false, cur_head = false;
+        V acc = (V)0;
+        uint32_t f0 = 0, f1 = 0;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            uint64_t mask = __ballot(hit[m]);
+            nhits += (uint32_t)__popcll(mask);
+            while (mask) {
+                // up to kHitBatch hits: their design rows are loaded together,
+                // then summed in order
+                int ls[kHitBatch];
+                V dv[kHitBatch], xv[kHitBatch];
+                uint32_t rv[kHitBatch];
+#pragma unroll
+                for (int q = 0; q < kHitBatch; q++) {
+                    ls[q] = -1;
+                    if (mask) {
+                        ls[q] = __ffsll((unsigned long long)mask) - 1;
+                        mask &= mask - 1;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kHitBatch; q++) {
+                    dv[q] = (V)0;
+                    xv[q] = (V)0;
+                    rv[q] = 0;
+                    if (ls[q] >= 0) {
+                        const uint32_t c = readlane_u32(hc[m], ls[q]);
+                        dv[q] = design[(int64_t)c * R + lr];
+                        xv[q] = readlane_val(hx[m], ls[q]);
+                        rv[q] = readlane_u32(hr[m], ls[q]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kHitBatch; q++) {
+                    if (ls[q] < 0) break;
+                    const uint32_t r = rv[q];
+                    if (!active || r != cur) {
+                        if (active) {
+                            if (cur_head) {
+                                if (lane < R) Y[(int64_t)cur * R + lane] = acc;
+                                nheads++;
+                            } else {
+                                if (lane < R) carry[(2 * w) * R + lane] = acc;
+                                f0 = 1;
+                            }
+                        }
+                        cur = r;
+                        acc = (V)0;
+                        active = true;
+                        cur_head = !(first_cont && r == first_row);
+                    }
+                    acc += dv[q] * xv[q];
+                }
+            }
+        }
+        const bool last_head = !(first_cont && last_row == first_row);
+        if (active && cur != last_row) {
+            // the last hit row ended inside the chunk
+            if (cur_head) {
+                if (lane < R) Y[(int64_t)cur * R + lane] = acc;
+                nheads++;
+            } else {
+                if (lane < R) carry[(2 * w) * R + lane] = acc;
+                f0 = 1;
+            }
+            active = false;
+            acc = (V)0;
+        }
+        if (continues) {
+            const V part = active ? acc : (V)0;   // active implies cur == last_row here
+            if (last_head) {
+                if (lane < R) carry[(2 * w + 1) * R + lane] = part;
+                f1 = 1;
+                if (active) nheads++;
+            } else {
+                if (lane < R) carry[(2 * w) * R + lane] = part;
+                f0 = 1;
+            }
+        } else if (active) {
+            if (cur_head) {
+                if (lane < R) Y[(int64_t)cur * R + lane] = acc;
+                nheads++;
+            } else {
+                if (lane < R) carry[(2 * w) * R + lane] = acc;
+                f0 = 1;
+            }
         }
         if (lane == 0) {
             cflag[2 * w] = f0;
             cflag[2 * w + 1] = f1;
-            if (T) atomicAdd(T, (unsigned long long)heads);
+            ckey[2 * w] = first_row;
+            ckey[2 * w + 1] = last_row;
         }
+        whits += nhits;
+        wheads += nheads;
+    }
+    // one pair of atomics per workgroup
+    if (lane == 0) {
+        wsum[0][threadIdx.x / 64] = whits;
+        wsum[1][threadIdx.x / 64] = wheads;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && counts) {
+        unsigned long long h = 0, t = 0;
+        for (int q = 0; q < NT / 64; q++) {
+            h += wsum[0][q];
+            t += wsum[1][q];
+        }
+        atomicAdd(counts, h);
+        atomicAdd(counts + 1, t);
     }
 }
 
+// One level of the carry reduction: entries (key, R-vector, flag) sorted by
+// key, kScanChunk entries per wavefront (four per lane), flagged entries
+// summed in order per key.  A key whose flagged entries all lie in the
+// chunk and that does not cross the chunk boundary is final and written to
+// Y; a crossing key leaves its partial in the next level's slot 0 (running
+// in from the left) or slot 1 (running out to the right), with the same
+// layout as the row pass.  With n <= kScanChunk every key is final.
 template <typename V>
-__global__ __launch_bounds__(kThreads) void seg_fixup(const uint32_t* __restrict__ hrow, int64_t H, int64_t nch,
-                                                      const V* __restrict__ carry,
-                                                      const uint32_t* __restrict__ cflag, int R,
-                                                      V* __restrict__ Y) {
+__global__ __launch_bounds__(kThreads) void seg_level(const uint32_t* __restrict__ key, const V* __restrict__ vec,
+                                                      const uint32_t* __restrict__ flag, int64_t n, int R,
+                                                      V* __restrict__ Y, V* __restrict__ out_vec,
+                                                      uint32_t* __restrict__ out_flag, uint32_t* __restrict__ out_key) {
     const int lane = threadIdx.x % 64;
+    const int lr = lane < R ? lane : 0;
+    constexpr int M = kScanChunk / 64;
+    const int64_t nch = (n + kScanChunk - 1) / kScanChunk;
     for (int64_t w = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64; w < nch;
          w += (int64_t)gridDim.x * (kThreads / 64)) {
-        if (!cflag[2 * w + 1]) continue;
-        const int64_t ew = (w + 1) * kSegChunk;   // < H: the row runs past this chunk
-        const uint32_t row = hrow[ew - 1];
-        V total = lane < R ? carry[(2 * w + 1) * R + lane] : (V)0;
-        for (int64_t k = w + 1; k < nch; k++) {
-            if (lane < R) total += carry[(2 * k) * R + lane];
-            const int64_t ek = (k + 1) * kSegChunk;
-            if (ek >= H || hrow[ek] != row) break;
+        const int64_t b = w * kScanChunk, e = b + kScanChunk < n ? b + kScanChunk : n;
+        uint32_t kk[M];
+        bool fl[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int64_t i = b + 64 * m + lane;
+            kk[m] = i < e ? key[i] : 0u;
+            fl[m] = i < e && flag[i] != 0u;
         }
-        if (lane < R) Y[(int64_t)row * R + lane] = total;
+        const uint32_t first_key = readlane_u32(kk[0], 0);
+        const int last = (int)(e - 1 - b);
+        uint32_t last_key = first_key;
+#pragma unroll
+        for (int m = 0; m < M; m++)
+            if (last / 64 == m) last_key = readlane_u32(kk[m], last % 64);
+        const bool first_cont = b > 0 && key[b - 1] == first_key;
+        const bool continues = e < n && key[e] == last_key;
+        uint32_t cur = 0;
+        bool active = false, cur_head = false;
+        V acc = (V)0;
+        uint32_t f0 = 0, f1 = 0;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            uint64_t mask = __ballot(fl[m]);
+            while (mask) {
+                int ls[kHitBatch];
+                V dv[kHitBatch];
+                uint32_t rv[kHitBatch];
+#pragma unroll
+                for (int q = 0; q < kHitBatch; q++) {
+                    ls[q] = -1;
+                    if (mask) {
+                        ls[q] = __ffsll((unsigned long long)mask) - 1;
+                        mask &= mask - 1;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kHitBatch; q++) {
+                    dv[q] = (V)0;
+                    rv[q] = 0;
+                    if (ls[q] >= 0) {
+                        dv[q] = vec[(b + 64 * m + ls[q]) * R + lr];
+                        rv[q] = readlane_u32(kk[m], ls[q]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kHitBatch; q++) {
+                    if (ls[q] < 0) break;
+                    const uint32_t r = rv[q];
+                    if (!active || r != cur) {
+                        if (active) {
+                            if (cur_head) {
+                                if (lane < R) Y[(int64_t)cur * R + lane] = acc;
+                            } else {
+                                if (lane < R) out_vec[(2 * w) * R + lane] = acc;
+                                f0 = 1;
+                            }
+                        }
+                        cur = r;
+                        acc = (V)0;
+                        active = true;
+                        cur_head = !(first_cont && r == first_key);
+                    }
+                    acc += dv[q];
+                }
+            }
+        }
+        if (active) {
+            const bool tail = continues && cur == last_key;
+            if (!tail && cur_head) {
+                if (lane < R) Y[(int64_t)cur * R + lane] = acc;
+            } else if (!tail || !cur_head) {
+                if (lane < R) out_vec[(2 * w) * R + lane] = acc;
+                f0 = 1;
+            } else {
+                if (lane < R) out_vec[(2 * w + 1) * R + lane] = acc;
+                f1 = 1;
+            }
+        }
+        if (lane == 0 && out_flag) {
+            out_flag[2 * w] = f0;
+            out_flag[2 * w + 1] = f1;
+            out_key[2 * w] = first_key;
+            out_key[2 * w + 1] = last_key;
+        }
     }
 }
 
@@ -357,6 +549,15 @@ KeyArgs key_args(const KeyLayout& kl) {
     return ka;
 }
 
+// Compute units of the device the stream runs on.
+int64_t num_cus(hipStream_t s) {
+    int dev = 0;
+    EX_HIP(hipStreamGetDevice(s, &dev));
+    int n = 0;
+    EX_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    return n > 0 ? n : 1;
+}
+
 uint64_t read_u64(hipStream_t s, const uint64_t* d) {
     uint64_t v = 0;
     EX_HIP(hipMemcpyAsync(&v, d, sizeof(v), hipMemcpyDeviceToHost, s));
@@ -376,41 +577,78 @@ void keys_from_coords(hipStream_t s, const uint32_t* coords, int64_t n, const Ke
     EX_HIP(hipGetLastError());
 }
 
-void gather_lookup(hipStream_t s, const uint32_t* perm, const uint32_t* coords, const double* vals, int64_t nnz,
-                   int N, int mode, uint32_t* rowj, double* val) {
-    if (nnz == 0) return;
-    hipLaunchKernelGGL(gather_lookup_kernel, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, perm, coords,
-                       vals, nnz, N, mode, rowj, val);
-    EX_HIP(hipGetLastError());
-}
-
-// Distinct keys of a sorted key array -> fkey (u64) / fptr (u32 offsets).
-int64_t fiber_index(hipStream_t s, const uint64_t* sorted_keys, int64_t nnz, DevBuf& fkey, DevBuf& fptr, DevBuf& tmp) {
+// The lookup of one mode (sort_lookup.hpp:37-102 builds the key-sorted copy):
+// nonzeros sorted by fiber key, fibers numbered, then re-sorted stably by
+// target row.  tmp[0..3] are scratch.
+void build_lookup(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, int mode,
+                  const int64_t* dims, ModeLookup& L, DevBuf* tmp) {
+    L.mode = mode;
+    L.nnz = nnz;
+    L.kl = make_key_layout(N, mode, dims);
+    EX_CHECK(L.kl.bits <= 64, 4,
+             "coordinates of the non-target modes need " + std::to_string(L.kl.bits) + " > 64 key bits");
+    int rbits = 1;
+    while (((int64_t)1 << rbits) < dims[mode]) rbits++;
     if (nnz == 0) {
-        fkey.ensure(8);
-        fptr.ensure(8);
-        EX_HIP(hipMemsetAsync(fptr.p, 0, 4, s));
-        return 0;
+        L.nfib = 0;
+        L.fkey.ensure(8);
+        L.cmap.ensure(8);
+        L.rrow.ensure(8);
+        L.rfib.ensure(8);
+        L.rval.ensure(8);
+        L.filt.ensure((size_t)kFilterWords * 4);
+        EX_HIP(hipMemsetAsync(L.filt.p, 0, (size_t)kFilterWords * 4, s));
+        return;
     }
-    size_t scan_bytes = exclusive_scan_u32(s, nullptr, 0, nullptr, nullptr, nnz);
-    size_t off_flags = 0, off_pos = (size_t)nnz * 4, off_cnt = off_pos + (size_t)nnz * 4,
-           off_tmp = (off_cnt + 8 + 255) & ~(size_t)255;
-    tmp.ensure(off_tmp + scan_bytes);
-    char* base = tmp.as<char>();
-    uint32_t* flags = (uint32_t*)(base + off_flags);
-    uint32_t* pos = (uint32_t*)(base + off_pos);
-    uint64_t* cnt = (uint64_t*)(base + off_cnt);
-    hipLaunchKernelGGL(head_flags_u64, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, sorted_keys, nnz,
-                       flags);
-    exclusive_scan_u32(s, base + off_tmp, scan_bytes, flags, pos, nnz);
+    // key order
+    const size_t sort_k = sort_pairs_u64_u32(s, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nnz, L.kl.bits);
+    const size_t sort_r = sort_pairs_u32_u32(s, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nnz, rbits);
+    const size_t scan_b = exclusive_scan_u32(s, nullptr, 0, nullptr, nullptr, nnz);
+    size_t big = sort_k > sort_r ? sort_k : sort_r;
+    if (scan_b > big) big = scan_b;
+    tmp[0].ensure((size_t)nnz * 16 + 512);   // keys in / out
+    tmp[1].ensure((size_t)nnz * 16 + 512);   // u32 x 4
+    tmp[2].ensure(big + 256);
+    tmp[3].ensure((size_t)nnz * 8 + 512);    // u32 x 2
+    uint64_t* kin = tmp[0].as<uint64_t>();
+    uint64_t* kout = kin + nnz;
+    uint32_t* u = tmp[1].as<uint32_t>();
+    uint32_t* iin = u;             // COO index, then row keys
+    uint32_t* src = u + nnz;       // COO index of each key-ordered position
+    uint32_t* flags = u + 2 * nnz;
+    uint32_t* pos = u + 3 * nnz;
+    uint32_t* fib = tmp[3].as<uint32_t>();
+    uint32_t* perm = fib + nnz;
+    keys_from_coords(s, coords, nnz, L.kl, kin, iin);
+    sort_pairs_u64_u32(s, tmp[2].p, tmp[2].bytes, kin, kout, iin, src, nnz, L.kl.bits);
+    // fibers: run heads of the sorted keys
+    hipLaunchKernelGGL(head_flags_u64, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, kout, nnz, flags);
+    exclusive_scan_u32(s, tmp[2].p, scan_b, flags, pos, nnz);
+    uint64_t* cnt = (uint64_t*)kin;   // keys_in is free now
     hipLaunchKernelGGL(run_count, dim3(1), dim3(64), 0, s, flags, pos, nnz, cnt);
-    int64_t nf = (int64_t)read_u64(s, cnt);
-    fkey.ensure((size_t)nf * 8);
-    fptr.ensure((size_t)(nf + 1) * 4);
-    hipLaunchKernelGGL(fiber_scatter, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, sorted_keys, nnz,
-                       flags, pos, fkey.as<uint64_t>(), fptr.as<uint32_t>());
+    const int64_t nf = (int64_t)read_u64(s, cnt);
+    L.nfib = nf;
+    L.fkey.ensure((size_t)nf * 8);
+    L.cmap.ensure((size_t)nf * 8);
+    L.filt.ensure((size_t)kFilterWords * 4);
+    EX_HIP(hipMemsetAsync(L.cmap.p, 0, (size_t)nf * 8, s));   // stamp 0: never a step's stamp
+    L.stamp = 0;
+    EX_HIP(hipMemsetAsync(L.filt.p, 0, (size_t)kFilterWords * 4, s));
+    hipLaunchKernelGGL(fiber_scatter_keys, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, kout, nnz,
+                       flags, pos, L.fkey.as<uint64_t>());
+    hipLaunchKernelGGL(fiber_numbers, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, flags, pos, nnz, fib);
+    // row order (stable: ties stay in fiber order)
+    uint32_t* rows_k = flags;   // flags / pos are free now
+    uint32_t* iota = pos;
+    hipLaunchKernelGGL(rows_of_keyorder, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, src, coords, nnz,
+                       N, mode, rows_k, iota);
+    L.rrow.ensure((size_t)nnz * 4);
+    L.rfib.ensure((size_t)nnz * 4);
+    L.rval.ensure((size_t)nnz * 8);
+    sort_pairs_u32_u32(s, tmp[2].p, tmp[2].bytes, rows_k, L.rrow.as<uint32_t>(), iota, perm, nnz, rbits);
+    hipLaunchKernelGGL(gather_row_order, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, perm, fib, src,
+                       vals, nnz, L.rfib.as<uint32_t>(), L.rval.as<double>());
     EX_HIP(hipGetLastError());
-    return nf;
 }
 
 // tmp[0..3] are scratch buffers owned by the caller.
@@ -455,74 +693,77 @@ void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int
     DesignArgs a;
     for (int k = 0; k < N; k++) a.U[k] = U[k];
     a.N = N; a.j = j; a.R = R;
-    hipLaunchKernelGGL(design_kernel, dim3(grid_for(Ju * R, kThreads, 16384)), dim3(kThreads), 0, s, sd, wd, Ju, a,
+    hipLaunchKernelGGL(design_kernel, dim3(grid_for(Ju, kThreads / R, 16384)), dim3(kThreads), 0, s, sd, wd, Ju, a,
                        d_gram, d_mttkrp);
     EX_HIP(hipGetLastError());
 }
 
-// Y (I_j x R) is zero-filled here; returns the number of hits H.
 template <typename V>
-int64_t sampled_mttkrp(hipStream_t s, const ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju,
-                       const V* design, int R, V* Y, int64_t I_j, DevBuf* tmp, int64_t* T_out,
-                       hipEvent_t* seg_events) {
+void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju, const V* design,
+                    int R, V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts, hipEvent_t* scan_events) {
     EX_HIP(hipMemsetAsync(Y, 0, (size_t)I_j * R * sizeof(V), s));
-    if (T_out) *T_out = 0;
-    if (Ju == 0 || L.nfib == 0) return 0;
-    // stage 1: fiber ranges and offsets
-    size_t scanJ = exclusive_scan_u32(s, nullptr, 0, nullptr, nullptr, Ju);
-    size_t o = 0;
-    auto take = [&](size_t b) { size_t r = o; o = (o + b + 255) & ~(size_t)255; return r; };
-    size_t o_start = take(Ju * 4), o_len = take(Ju * 4), o_off = take(Ju * 4), o_H = take(8), o_T = take(8),
-           o_tmp = take(scanJ);
-    tmp[1].ensure(o);
-    char* b = tmp[1].as<char>();
-    uint32_t* start = (uint32_t*)(b + o_start);
-    uint32_t* len = (uint32_t*)(b + o_len);
-    uint32_t* off = (uint32_t*)(b + o_off);
-    uint64_t* dH = (uint64_t*)(b + o_H);
-    uint64_t* dT = (uint64_t*)(b + o_T);
-    hipLaunchKernelGGL(fiber_lookup, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, kd, Ju,
-                       L.fkey.as<uint64_t>(), L.nfib, L.fptr.as<uint32_t>(), start, len);
-    exclusive_scan_u32(s, b + o_tmp, scanJ, len, off, Ju);
-    hipLaunchKernelGGL(total_from_scan, dim3(1), dim3(64), 0, s, off, len, Ju, dH);
-    const int64_t H = (int64_t)read_u64(s, dH);
-    if (H == 0) return 0;
-    // stage 2: hits sorted by output row
-    int rbits = 1;
-    while (((int64_t)1 << rbits) < I_j) rbits++;
-    size_t sortH = sort_pairs_u32_u64(s, nullptr, 0, nullptr, nullptr, nullptr, nullptr, H, rbits);
-    size_t o2 = 0;
-    auto take2 = [&](size_t bb) { size_t r = o2; o2 = (o2 + bb + 255) & ~(size_t)255; return r; };
-    size_t o_r0 = take2(H * 4), o_r1 = take2(H * 4), o_p0 = take2(H * 8), o_p1 = take2(H * 8), o_t2 = take2(sortH);
-    tmp[2].ensure(o2);
-    char* c2 = tmp[2].as<char>();
-    uint32_t* r0 = (uint32_t*)(c2 + o_r0);
-    uint32_t* r1 = (uint32_t*)(c2 + o_r1);
-    uint64_t* p0 = (uint64_t*)(c2 + o_p0);
-    uint64_t* p1 = (uint64_t*)(c2 + o_p1);
-    hipLaunchKernelGGL(expand_hits<V>, dim3(grid_for(H, kThreads, 65536)), dim3(kThreads), 0, s, off, start, Ju, H,
-                       L.rowj.as<uint32_t>(), vals, r0, p0);
-    sort_pairs_u32_u64(s, c2 + o_t2, sortH, r0, r1, p0, p1, H, rbits);
-    const int64_t nch = (H + kSegChunk - 1) / kSegChunk;
-    tmp[3].ensure((size_t)nch * 2 * R * sizeof(V) + (size_t)nch * 2 * 4 + 64);
-    V* carry = tmp[3].as<V>();
-    uint32_t* cflag = (uint32_t*)(carry + nch * 2 * R);
-    if (T_out) EX_HIP(hipMemsetAsync(dT, 0, 8, s));
-    if (seg_events) EX_HIP(hipEventRecord(seg_events[0], s));
-    const unsigned nb = grid_for(nch, kThreads / 64, 65536);
-    hipLaunchKernelGGL(seg_chunk_reduce<V>, dim3(nb), dim3(kThreads), 0, s, r1, p1, vals, design, R, H, nch, Y, carry,
-                       cflag, T_out ? (unsigned long long*)dT : nullptr);
-    hipLaunchKernelGGL(seg_fixup<V>, dim3(nb), dim3(kThreads), 0, s, r1, H, nch, carry, cflag, R, Y);
-    if (seg_events) EX_HIP(hipEventRecord(seg_events[1], s));
+    if (counts) EX_HIP(hipMemsetAsync(counts, 0, 16, s));
+    if (scan_events) EX_HIP(hipEventRecord(scan_events[0], s));
+    if (Ju > 0 && L.nfib > 0 && L.nnz > 0) {
+        const int64_t nch = (L.nnz + kScanChunk - 1) / kScanChunk;
+        // a fresh stamp per call; wrapping to 0 would match never-marked
+        // entries, so the map is cleared then
+        if (++L.stamp == 0) {
+            EX_HIP(hipMemsetAsync(L.cmap.p, 0, (size_t)L.nfib * 8, s));
+            L.stamp = 1;
+        }
+        const uint32_t stamp = L.stamp;
+        tmp[1].ensure((size_t)Ju * 4 + 256);
+        uint32_t* fsel = tmp[1].as<uint32_t>();
+        // carry lists: level l has n_l = 2 * chunks(n_{l-1}) entries; two
+        // alternating buffers of (vec, flag, key)
+        auto list_bytes = [&](int64_t n) { return (size_t)n * R * sizeof(V) + (size_t)n * 8 + 512; };
+        const int64_t n1 = 2 * nch;
+        tmp[2].ensure(list_bytes(n1));
+        tmp[3].ensure(list_bytes(2 * ((n1 + kScanChunk - 1) / kScanChunk)));
+        auto views = [&](DevBuf& b, int64_t n, V*& vec, uint32_t*& flag, uint32_t*& key) {
+            vec = b.as<V>();
+            flag = (uint32_t*)(vec + n * R);
+            key = flag + n;
+        };
+        V* vec;
+        uint32_t *flag, *key;
+        views(tmp[2], n1, vec, flag, key);
+        hipLaunchKernelGGL(mark_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, kd, Ju,
+                           L.fkey.as<uint64_t>(), L.nfib, stamp, L.cmap.as<uint64_t>(),
+                           L.filt.as<uint32_t>(), fsel);
+        const dim3 scan_grid(grid_for(nch, kScanThreads / 64, 2 * num_cus(s)));
+        hipLaunchKernelGGL(row_scan_reduce<V>, scan_grid, dim3(kScanThreads), 0, s, L.rrow.as<uint32_t>(), L.rfib.as<uint32_t>(), vals, L.nnz, nch,
+                           stamp, L.cmap.as<uint64_t>(), L.filt.as<uint32_t>(), design, R, Y, vec,
+                           flag, key, (unsigned long long*)counts);
+        // reduce the carry list level by level until one chunk holds it
+        int64_t n = n1;
+        int cur = 2;
+        while (true) {
+            const int64_t ch = (n + kScanChunk - 1) / kScanChunk;
+            V* ovec = nullptr;
+            uint32_t *oflag = nullptr, *okey = nullptr;
+            if (ch > 1) views(tmp[5 - cur], 2 * ch, ovec, oflag, okey);
+            hipLaunchKernelGGL(seg_level<V>, dim3(grid_for(ch, kThreads / 64, 65536)), dim3(kThreads), 0, s, key, vec,
+                               flag, n, R, Y, ovec, oflag, okey);
+            if (ch == 1) break;
+            n = 2 * ch;
+            cur = 5 - cur;
+            views(tmp[cur], n, vec, flag, key);
+            if (tmp[5 - cur].bytes < list_bytes(2 * ((n + kScanChunk - 1) / kScanChunk)))
+                tmp[5 - cur].ensure(list_bytes(2 * ((n + kScanChunk - 1) / kScanChunk)));
+        }
+        hipLaunchKernelGGL(clear_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, fsel, Ju,
+                           L.filt.as<uint32_t>());
+    }
+    if (scan_events) EX_HIP(hipEventRecord(scan_events[1], s));
     EX_HIP(hipGetLastError());
-    if (T_out) *T_out = (int64_t)read_u64(s, dT);
-    return H;
 }
 
-template int64_t sampled_mttkrp<double>(hipStream_t, const ModeLookup&, const double*, const uint64_t*, int64_t,
-                                        const double*, int, double*, int64_t, DevBuf*, int64_t*, hipEvent_t*);
-template int64_t sampled_mttkrp<float>(hipStream_t, const ModeLookup&, const float*, const uint64_t*, int64_t,
-                                       const float*, int, float*, int64_t, DevBuf*, int64_t*, hipEvent_t*);
+template void sampled_mttkrp<double>(hipStream_t, ModeLookup&, const double*, const uint64_t*, int64_t, const double*,
+                                     int, double*, int64_t, DevBuf*, uint64_t*, hipEvent_t*);
+template void sampled_mttkrp<float>(hipStream_t, ModeLookup&, const float*, const uint64_t*, int64_t, const float*,
+                                    int, float*, int64_t, DevBuf*, uint64_t*, hipEvent_t*);
 
 void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
              const double* sigma, int R, double* out, DevBuf& part) {

@@ -1,5 +1,5 @@
-// rocPRIM radix sorts and scans used by the lookup build, the sample
-// deduplication and the row-sorting of sampled-MTTKRP hits.  Kept in their
+// rocPRIM radix sorts and scans used by the lookup build (fiber order and
+// row order) and the sample deduplication.  Kept in their
 // own translation unit: the templates are slow to compile.
 //
 // Each wrapper returns the temporary-storage size it needs; called with
@@ -27,6 +27,17 @@ size_t sort_pairs_u64_u32(hipStream_t s, void* temp, size_t temp_bytes, const ui
 
 size_t sort_pairs_u32_u64(hipStream_t s, void* temp, size_t temp_bytes, const uint32_t* kin, uint32_t* kout,
                           const uint64_t* vin, uint64_t* vout, int64_t n, int bits) {
+    size_t need = 0;
+    EX_HIP(rocprim::radix_sort_pairs(nullptr, need, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits, s));
+    if (temp) {
+        EX_CHECK(temp_bytes >= need, 3, "radix sort scratch too small");
+        EX_HIP(rocprim::radix_sort_pairs(temp, need, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits, s));
+    }
+    return need;
+}
+
+size_t sort_pairs_u32_u32(hipStream_t s, void* temp, size_t temp_bytes, const uint32_t* kin, uint32_t* kout,
+                          const uint32_t* vin, uint32_t* vout, int64_t n, int bits) {
     size_t need = 0;
     EX_HIP(rocprim::radix_sort_pairs(nullptr, need, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)bits, s));
     if (temp) {
