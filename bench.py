@@ -67,19 +67,25 @@ def sts_draw_bytes(dims, R, J, j, s=8):
     return total
 
 
-def mttkrp_bytes(R, Ju, H, T, nnz, nfib, I, s=8):
-    """Algorithmic bytes of one sampled MTTKRP launch group: the fiber marks
-    (Ju keys), the row pass (row_scan_bytes) and the zero fill of the output
-    (I x R).  s = bytes per value / design / output entry (8 FP64, 4 mixed)."""
-    return Ju * 8 + row_scan_bytes(R, Ju, H, T, nnz, nfib, s) + I * R * s
+def mttkrp_bytes(N, R, Ju, H, T, s=8):
+    """Algorithmic bytes of sampled MTTKRP + KRP gather per mode step: sample
+    tuples and weights, gathered factor rows, touched nonzeros (coordinates +
+    value), output rows written.  The lookup's own traffic (the row pass over
+    every nonzero, fiber probes) is implementation cost and not counted.
+    s = bytes per factor / value / output entry (8 FP64, 4 mixed); weights
+    are FP64 on both paths."""
+    return Ju * (4 * (N - 1) + 8) + Ju * (N - 1) * R * s + H * (4 * N + s) + T * R * s
 
 
-def row_scan_bytes(R, Ju, H, T, nnz, nfib, s=8):
-    """Algorithmic bytes of the sampled-MTTKRP row pass: every nonzero's row
-    and fiber number (4 + 4), the sampled-fiber bitmap once (nfib / 8), per
-    hit its design-row index (4) and value (s), each design row once, each
-    touched output row once."""
-    return nnz * 8 + nfib / 8 + H * (4 + s) + Ju * R * s + T * R * s
+def load_traffic(path):
+    """Measured HBM bytes per launch from the committed rocprofv3 PMC summary
+    (a JSON block in the markdown file): {"<span>/<precision>/<config>": bytes}."""
+    try:
+        text = open(path).read()
+        block = text.split("```json", 1)[1].split("```", 1)[0]
+        return json.loads(block)
+    except (OSError, IndexError, ValueError):
+        return {}
 
 
 def run_gpu(args, cfg, ctx, T, U0, precision):
@@ -125,35 +131,41 @@ def run_gpu(args, cfg, ctx, T, U0, precision):
     return dict(elapsed=elapsed, timing=timing, fit0=fit0, fit1=fit1)
 
 
-def summarize(args, cfg, g, vbytes):
-    """Throughput, dominant launch group and its roofline for one GPU run."""
+def summarize(args, cfg, g, vbytes, precision):
+    """Throughput, dominant launch group and its roofline for one GPU run.
+    The candidates are the STS-CP draw (one launch per mode step) and the
+    sampled MTTKRP + KRP gather (one launch group per mode step); the one
+    with the larger total time is the dominant kernel."""
     N, R, J = len(cfg["dims"]), cfg["R"], cfg["J"]
     K = args.steps
     spans = g["timing"]["spans"]
     cnt = g["timing"]["counters"]
     steps = max(cnt.get("mode_steps", 1), 1)
-    dom = max(spans.items(), key=lambda kv: kv[1]["ms"])
-    name, st = dom[0], dom[1]
-    avg_s = st["ms"] / st["count"] * 1e-3
-    if name == "sts_draw":
-        per = np.mean([sts_draw_bytes(cfg["dims"], R, J, j, s=vbytes) for j in range(N)])
-    elif name == "row_scan":
-        per = row_scan_bytes(R, cnt["J_u"] / steps, cnt["H"] / steps, cnt["T"] / steps, cnt["nnz"] / steps,
-                             cnt["nfib"] / steps, s=vbytes)
-    else:
-        per = float("nan")
-    achieved = per / avg_s / 1e9
-    roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 4),
-            "algorithmic_bytes_per_launch": int(per) if per == per else None}
-    mt = spans.get("sampled_mttkrp")
-    mt_gbs = None
-    if mt and cnt.get("mode_steps"):
-        per_mt = mttkrp_bytes(R, cnt["J_u"] / steps, cnt["H"] / steps, cnt["T"] / steps, cnt["nnz"] / steps,
-                              cnt["nfib"] / steps, cnt["I"] / steps, s=vbytes)
-        mt_gbs = per_mt / (mt["ms"] / mt["count"] * 1e-3) / 1e9
-    return dict(value=K / g["elapsed"], ms_per_step=g["elapsed"] / K * 1e3, roofline=roof,
-                sampled_mttkrp_gbs=None if mt_gbs is None else round(mt_gbs, 2),
+    traffic = load_traffic(os.path.join(ROOT, "profiles", "hbm_traffic.md"))
+    roofs = {}
+    if "sts_draw" in spans:
+        st = spans["sts_draw"]
+        avg = st["ms"] / st["count"] * 1e-3
+        per = float(np.mean([sts_draw_bytes(cfg["dims"], R, J, j, s=vbytes) for j in range(N)]))
+        roofs["sts_draw"] = (st["ms"], avg, per)
+    if "sampled_mttkrp" in spans and cnt.get("mode_steps"):
+        ms = spans["sampled_mttkrp"]["ms"] + spans.get("design_gather", {"ms": 0.0})["ms"]
+        avg = ms / steps * 1e-3
+        per = mttkrp_bytes(N, R, cnt["J_u"] / steps, cnt["H"] / steps, cnt["T"] / steps, s=vbytes)
+        roofs["sampled_mttkrp"] = (ms, avg, per)
+    out = {}
+    for name, (tot, avg, per) in roofs.items():
+        achieved = per / avg / 1e9
+        key = f"{name}/{precision}/{args.config}"
+        out[name] = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic.get(key) if args.scale == 1.0 else None,
+                     "avg_launch_ms": round(avg * 1e3, 4), "algorithmic_bytes_per_launch": int(per)}
+    dom = max(roofs, key=lambda n: roofs[n][0]) if roofs else None
+    mt = out.get("sampled_mttkrp")
+    return dict(value=K / g["elapsed"], ms_per_step=g["elapsed"] / K * 1e3,
+                roofline=out.get(dom), roofline_other={k: v for k, v in out.items() if k != dom},
+                sampled_mttkrp_gbs=mt["achieved"] if mt else None,
                 nnz_per_s=round(cnt.get("H", 0) / g["elapsed"], 1),
                 spans_ms_per_sweep={k: round(v["ms"] / K, 4) for k, v in spans.items()},
                 counters_per_mode_step={k: round(v / steps, 1) for k, v in cnt.items()},
@@ -214,7 +226,7 @@ def main():
     res = {}
     for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
         g = run_gpu(args, cfg, ctx, T, U0, prec)
-        res[prec] = summarize(args, cfg, g, 4 if prec == "mixed" else 8)
+        res[prec] = summarize(args, cfg, g, 4 if prec == "mixed" else 8, prec)
     N, R, J = len(cfg["dims"]), cfg["R"], cfg["J"]
     K = args.steps
     head = "mixed" if "mixed" in res else "fp64"
@@ -242,6 +254,7 @@ def main():
         "nnz_per_s": h["nnz_per_s"],
         "fit": h["fit"],
         "roofline": h["roofline"],
+        "roofline_other": h["roofline_other"],
         "cpu_baseline": cpu,
         "device": dev,
     }
