@@ -99,7 +99,31 @@ struct ex_ctx {
             if (c2.first == n) { c2.second += v; return; }
         counters.push_back({n, v});
     }
+    // device-side running totals reported as counters: value = current - start
+    struct Watch {
+        const uint64_t* dev;
+        std::string name;
+        uint64_t start;
+    };
+    std::vector<Watch> watches;
+    void watch(const uint64_t* dev, const std::string& n) {
+        for (auto& w : watches)
+            if (w.dev == dev) return;
+        uint64_t v = 0;
+        EX_HIP(hipMemcpyAsync(&v, dev, 8, hipMemcpyDeviceToHost, c.stream));
+        EX_HIP(hipStreamSynchronize(c.stream));
+        watches.push_back({dev, n, v});
+    }
+    // the caller has synchronised the stream
     void resolve() {
+        for (auto& w : watches) {
+            uint64_t v = 0;
+            EX_HIP(hipMemcpy(&v, w.dev, 8, hipMemcpyDeviceToHost));
+            bool found = false;
+            for (auto& c2 : counters)
+                if (c2.first == w.name) { c2.second = (double)(v - w.start); found = true; }
+            if (!found) counters.push_back({w.name, (double)(v - w.start)});
+        }
         for (auto& p : pending) {
             float ms = 0.f;
             EX_HIP(hipEventElapsedTime(&ms, p.second.first, p.second.second));
@@ -237,9 +261,11 @@ struct ex_als {
     DevBuf sd, wd, kd, dg, dm, G, Gi, Gi32, GY, Ms, Y, scal;
     DevBuf tmp[4];
     DevBuf part;
-    DevBuf cnt;                  // u64[2]: hits (nonzeros iterated) and touched rows, summed over steps
-    uint64_t cnt_seen[2] = {};   // the part of cnt already reported to timing counters
-    uint64_t cnt_base = 0;       // hits at the start of the current execute_ALS_rounds
+    // u64[4]: hits (nonzeros iterated) and touched rows summed over steps, the
+    // J_u sum, and the current step's J_u (device-side: no host read per step)
+    DevBuf cnt;
+    // region events of steps not yet read back: 6 per step
+    std::vector<hipEvent_t> ev_pending, ev_pool;
     double t_region[7] = {};
     std::string stats = "{}";
 };
@@ -356,22 +382,31 @@ void tensor_build_lookups(ex_tensor* t) {
     t->built = true;
 }
 
-struct EventTimer {
-    hipEvent_t ev[8];
-    int n = 0;
-    explicit EventTimer(int count) : n(count) {
-        for (int i = 0; i < n; i++) EX_HIP(hipEventCreate(&ev[i]));
+hipEvent_t als_event(ex_als* a) {
+    if (!a->ev_pool.empty()) {
+        hipEvent_t e = a->ev_pool.back();
+        a->ev_pool.pop_back();
+        return e;
     }
-    ~EventTimer() {
-        for (int i = 0; i < n; i++) (void)hipEventDestroy(ev[i]);
+    hipEvent_t e;
+    EX_HIP(hipEventCreate(&e));
+    return e;
+}
+
+// Adds the region times of every finished step to t_region (waits for them).
+void als_resolve_regions(ex_als* a) {
+    static const int slot[5] = {0, 2, 3, 5, 6};   // R1, R3, R4, R6, R7 (als_optimizer.hpp:21-27)
+    for (size_t b = 0; b + 6 <= a->ev_pending.size(); b += 6) {
+        EX_HIP(hipEventSynchronize(a->ev_pending[b + 5]));
+        for (int r = 0; r < 5; r++) {
+            float ms = 0.f;
+            EX_HIP(hipEventElapsedTime(&ms, a->ev_pending[b + r], a->ev_pending[b + r + 1]));
+            a->t_region[slot[r]] += ms * 1e-3;
+        }
     }
-    void mark(int i, hipStream_t s) { EX_HIP(hipEventRecord(ev[i], s)); }
-    double secs(int a, int b) {
-        float ms = 0.f;
-        EX_HIP(hipEventElapsedTime(&ms, ev[a], ev[b]));
-        return ms * 1e-3;
-    }
-};
+    for (hipEvent_t e : a->ev_pending) a->ev_pool.push_back(e);
+    a->ev_pending.clear();
+}
 
 void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     ex_factors* f = a->f;
@@ -380,25 +415,31 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     const int N = f->N, R = f->R;
     const int64_t Ij = f->dims[j];
     EX_CHECK(J > 0 && J < (int64_t)1 << 31, EX_ERR_ARG, "sample count J must be in [1, 2^31)");
-    EventTimer ev(6);
-    ev.mark(0, st);
+    hipEvent_t ev[6];
+    for (int q = 0; q < 6; q++) ev[q] = als_event(a);
+    auto mark = [&](int q) { EX_HIP(hipEventRecord(ev[q], st)); };
+    mark(0);
     // Region 1: draw samples (accumulator_stationary.hpp:148-153)
     s->samples.ensure((size_t)J * N * 4);
     s->weights.ensure((size_t)J * 8);
     sampler_draw_impl(s, J, j, sweep, nullptr, nullptr, s->samples.as<uint32_t>(), s->weights.as<double>());
-    ev.mark(1, st);
+    mark(1);
     // Region 3: dedup, design, Gram, pinv (:176-250)
     a->sd.ensure((size_t)J * N * 4);
     a->wd.ensure((size_t)J * 8);
     a->kd.ensure((size_t)J * 8);
     const KeyLayout kl = a->t->lk[j].kl;
     Span s_dd(a->ctx, "dedup");
-    const int64_t Ju = k::dedup(st, s->samples.as<uint32_t>(), s->weights.as<double>(), J, N, kl,
-                                a->sd.as<uint32_t>(), a->wd.as<double>(), a->kd.as<uint64_t>(), a->tmp);
+    // J_u stays on the device: every later kernel reads it; J bounds the sizes
+    uint64_t* cnt = a->cnt.as<uint64_t>();
+    const uint64_t* ju = cnt + 3;
+    k::dedup(st, s->samples.as<uint32_t>(), s->weights.as<double>(), J, N, kl, a->sd.as<uint32_t>(),
+             a->wd.as<double>(), a->kd.as<uint64_t>(), a->tmp, cnt + 3, cnt + 2);
+    const int64_t Ju = J;
     const bool mixed = a->precision == EX_MIXED;
     const size_t vb = mixed ? 4 : 8;   // bytes per design / MTTKRP value
-    a->dg.ensure((size_t)(Ju > 0 ? Ju : 1) * R * vb);
-    a->dm.ensure((size_t)(Ju > 0 ? Ju : 1) * R * vb);
+    a->dg.ensure((size_t)J * R * vb);
+    a->dm.ensure((size_t)J * R * vb);
     a->G.ensure((size_t)R * R * 8);
     a->Gi.ensure((size_t)R * R * 8);
     a->Gi32.ensure((size_t)R * R * 4);
@@ -407,23 +448,23 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     if (mixed) {
         f->sync32_all();
         k::design32(st, a->sd.as<uint32_t>(), a->wd.as<double>(), Ju, N, j, R, f->ptrs32, a->dg.as<float>(),
-                    a->dm.as<float>());
+                    a->dm.as<float>(), ju);
     } else {
         f->sync64_all();
         k::design(st, a->sd.as<uint32_t>(), a->wd.as<double>(), Ju, N, j, R, f->ptrs, a->dg.as<double>(),
-                  a->dm.as<double>());
+                  a->dm.as<double>(), ju);
     }
     s_de.end();
     Span s_dp(a->ctx, "design_gram_pinv");
     if (mixed) {
-        k::gram_bf16(st, a->dg.as<float>(), Ju, R, a->G.as<double>(), a->part);
+        k::gram_bf16(st, a->dg.as<float>(), Ju, R, a->G.as<double>(), a->part, ju);
         k::spd_inverse(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>(), a->Gi32.as<float>());
     } else {
-        k::gram(st, a->dg.as<double>(), Ju, R, a->G.as<double>(), a->part);
+        k::gram(st, a->dg.as<double>(), Ju, R, a->G.as<double>(), a->part, ju);
         k::pinv(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>());
     }
     s_dp.end();
-    ev.mark(2, st);
+    mark(2);
     // Region 4: sampled MTTKRP (:252-274)
     a->Y.ensure((size_t)Ij * R * vb);
     ModeLookup& L = a->t->lk[j];
@@ -438,20 +479,24 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
         scan_ev[1] = a->ctx->get_event();
     }
     // hits and touched rows accumulate on the device (a->cnt)
-    uint64_t* cnt = a->cnt.as<uint64_t>();
     if (mixed)
         k::sampled_mttkrp<float>(st, L, L.rval32.as<float>(), a->kd.as<uint64_t>(), Ju, a->dm.as<float>(), R,
-                                 a->Y.as<float>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr);
+                                 a->Y.as<float>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr, ju);
     else
         k::sampled_mttkrp<double>(st, L, L.rval.as<double>(), a->kd.as<uint64_t>(), Ju, a->dm.as<double>(), R,
-                                  a->Y.as<double>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr);
+                                  a->Y.as<double>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr, ju);
     s_mt.end();
     if (a->ctx->timing) {
         a->ctx->pending.push_back({"row_scan", {scan_ev[0], scan_ev[1]}});
-        a->ctx->add_counter("J_u", (double)Ju);
+        a->ctx->watch(cnt + 0, "H");
+        a->ctx->watch(cnt + 1, "T");
+        a->ctx->watch(cnt + 2, "J_u");
         a->ctx->add_counter("mode_steps", 1.0);
+        a->ctx->add_counter("nnz", (double)L.nnz);
+        a->ctx->add_counter("nfib", (double)L.nfib);
+        a->ctx->add_counter("I", (double)Ij);
     }
-    ev.mark(3, st);
+    mark(3);
     // Region 6: U_j = Y G+, renormalise (:280-289)
     Span s_fu(a->ctx, "factor_update");
     if (mixed) {
@@ -471,28 +516,13 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
         f->wrote64(j);
     }
     s_fu.end();
-    ev.mark(4, st);
+    mark(4);
     // Region 7: sampler update (:291-294)
     sampler_update_impl(s, j);
-    ev.mark(5, st);
-    EX_HIP(hipEventSynchronize(ev.ev[5]));
-    a->t_region[0] += ev.secs(0, 1);
-    a->t_region[2] += ev.secs(1, 2);
-    a->t_region[3] += ev.secs(2, 3);
-    a->t_region[5] += ev.secs(3, 4);
-    a->t_region[6] += ev.secs(4, 5);
-    if (a->ctx->timing) {
-        uint64_t c[2];
-        d2h(st, c, a->cnt.p, sizeof(c));
-        a->ctx->add_counter("H", (double)(c[0] - a->cnt_seen[0]));
-        a->ctx->add_counter("T", (double)(c[1] - a->cnt_seen[1]));
-        a->cnt_seen[0] = c[0];
-        a->cnt_seen[1] = c[1];
-        a->ctx->add_counter("nnz", (double)L.nnz);
-        a->ctx->add_counter("nfib", (double)L.nfib);
-        a->ctx->add_counter("I", (double)Ij);
-        a->ctx->resolve();
-    }
+    mark(5);
+    // region times are read back later (als_resolve_regions): no wait here
+    for (int q = 0; q < 6; q++) a->ev_pending.push_back(ev[q]);
+    if (a->ev_pending.size() >= 6 * 64) als_resolve_regions(a);
 }
 
 double als_fit_impl(ex_als* a) {
@@ -561,6 +591,7 @@ int ex_ctx_timing(ex_ctx* ctx, int enable) {
         ctx->timing = enable != 0;
         ctx->totals.clear();
         ctx->counters.clear();
+        ctx->watches.clear();
     });
 }
 
@@ -927,15 +958,26 @@ int ex_als_create(ex_ctx* ctx, ex_tensor* t, ex_factors* f, ex_sampler* s, int d
         a->s = s;
         a->dist = distribution;
         a->precision = precision;
-        a->cnt.ensure(16);
-        EX_HIP(hipMemsetAsync(a->cnt.p, 0, 16, ctx->c.stream));
+        a->cnt.ensure(32);
+        EX_HIP(hipMemsetAsync(a->cnt.p, 0, 32, ctx->c.stream));
         EX_HIP(hipStreamSynchronize(ctx->c.stream));
         *out = a.release();
     });
 }
 
 int ex_als_destroy(ex_als* a) {
-    return guarded([&] { delete a; });
+    return guarded([&] {
+        if (!a) return;
+        (void)hipStreamSynchronize(a->ctx->c.stream);
+        const uint64_t* c0 = a->cnt.as<uint64_t>();
+        auto& w = a->ctx->watches;
+        for (size_t i = 0; i < w.size();)
+            if (w[i].dev >= c0 && w[i].dev < c0 + 4) w.erase(w.begin() + (long)i);
+            else i++;
+        for (hipEvent_t e : a->ev_pending) (void)hipEventDestroy(e);
+        for (hipEvent_t e : a->ev_pool) (void)hipEventDestroy(e);
+        delete a;
+    });
 }
 
 int ex_als_init_ground_truth(ex_als* a) {
@@ -970,6 +1012,7 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
         EX_CHECK(rounds >= 0, EX_ERR_ARG, "negative round count");
         EX_CHECK(epoch >= 1, EX_ERR_ARG, "epoch_interval must be positive");
         hipStream_t st = a->ctx->c.stream;
+        als_resolve_regions(a);
         for (double& x : a->t_region) x = 0.0;
         uint64_t c0[2];
         d2h(st, c0, a->cnt.p, sizeof(c0));
@@ -992,6 +1035,7 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
                 fit_times.push_back(fit_total);
             }
         }
+        als_resolve_regions(a);
         uint64_t c1[2];
         d2h(st, c1, a->cnt.p, sizeof(c1));
         const double nonzeros = (double)(c1[0] - c0[0]);

@@ -138,7 +138,8 @@ struct Ctx {
 // All launchers enqueue on `s` and never synchronise.
 namespace k {
 // dense
-void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part);
+void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part,
+          const uint64_t* m_dev = nullptr);
 // G = A^T A for FP32 A with FP64 products and sums
 void gram_f32in(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part);
 void eigh(hipStream_t s, const double* A, int R, double* lam, double* V, int ncopies = 1);
@@ -189,10 +190,14 @@ void keys_from_coords(hipStream_t s, const uint32_t* coords, int64_t n, const Ke
 // Builds L (fkey, rrow, rfib, rval, bitmap, cmap) from the COO tensor.
 void build_lookup(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, int mode,
                   const int64_t* dims, ModeLookup& L, DevBuf* tmp);
+// Returns J_u, or -1 when Ju_dev is given: the count then stays on the device
+// (Ju_dev) with no host synchronisation, and is added to *Ju_total if given.
+// Kernels taking (bound, *_dev) row counts use min(bound, *dev).
 int64_t dedup(hipStream_t s, const uint32_t* samples, const double* weights, int64_t J, int N, const KeyLayout& kl,
-              uint32_t* sd, double* wd, uint64_t* kd, DevBuf* tmp);
+              uint32_t* sd, double* wd, uint64_t* kd, DevBuf* tmp, uint64_t* Ju_dev = nullptr,
+              uint64_t* Ju_total = nullptr);
 void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
-            const double* const* U, double* d_gram, double* d_mttkrp);
+            const double* const* U, double* d_gram, double* d_mttkrp, const uint64_t* Ju_dev = nullptr);
 // Y (I_j x R, zero-filled here) = sampled MTTKRP of the Ju deduplicated
 // samples with fiber keys kd and design rows `design`.  V = double (FP64
 // path) or float (mixed path): values (L.rval / L.rval32), design rows and Y.
@@ -200,21 +205,22 @@ void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int
 template <typename V>
 void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju, const V* design,
                     int R, V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts = nullptr,
-                    hipEvent_t* scan_events = nullptr);
+                    hipEvent_t* scan_events = nullptr, const uint64_t* Ju_dev = nullptr);
 void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
              const double* sigma, int R, double* out, DevBuf& part);
 // final stages shared with the mixed-precision kernels
 void gram_final_launch(hipStream_t s, const double* part, int nb, int R, double* G);
 void col_sumsq_final_launch(hipStream_t s, const double* part, int nb, int R, double* sigma);
 // mixed precision (FP32 data, BF16 MFMA Grams, FP64 R x R solves)
-void gram_bf16(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part);
+void gram_bf16(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part,
+               const uint64_t* m_dev = nullptr);
 void tree_build_f32(hipStream_t s, const float* U, const TreeGeom& g, int R, float* nodes);
 void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, float* out32);
 void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M);
 void to_f32(hipStream_t s, const double* a, int64_t n, float* b);
 void to_f64(hipStream_t s, const float* a, int64_t n, double* b);
 void design32(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
-              const float* const* U, float* dg, float* dm);
+              const float* const* U, float* dg, float* dm, const uint64_t* Ju_dev = nullptr);
 void dsymm32(hipStream_t s, const float* G, const float* Y, int64_t m, int R, float* out);
 void renormalize32(hipStream_t s, float* U, int64_t m, int R, double* sigma, DevBuf& part);
 // sigma_c = ||(Y Gi)[:, c]|| from GY = Y^T Y; Ms = Gi diag(1 / sigma) (FP32)

@@ -41,8 +41,10 @@ __global__ __launch_bounds__(kThreads) void gram_final(const double* __restrict_
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 template <typename TA, int NT>
-__global__ __launch_bounds__(kThreads) void gram_mfma64_partial(const TA* __restrict__ A, int64_t m, int R,
+__global__ __launch_bounds__(kThreads) void gram_mfma64_partial(const TA* __restrict__ A, int64_t mb,
+                                                                const uint64_t* __restrict__ m_dev, int R,
                                                                 double* __restrict__ part) {
+    const int64_t m = m_dev ? ((int64_t)*m_dev < mb ? (int64_t)*m_dev : mb) : mb;
     __shared__ double red[kMaxRank * kMaxRank];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int kr = lane / 16, kc = lane % 16;
@@ -640,22 +642,23 @@ __global__ void pack_upper_kernel(const double* __restrict__ M, int R, double* _
 namespace k {
 
 template <typename TA>
-static void gram_mfma64(hipStream_t s, const TA* A, int64_t m, int R, double* G, DevBuf& part) {
+static void gram_mfma64(hipStream_t s, const TA* A, int64_t m, int R, double* G, DevBuf& part,
+                        const uint64_t* m_dev = nullptr) {
     const int nb = (int)grid_for((m + 15) / 16, 4, 1024);
     part.ensure((size_t)nb * R * R * sizeof(double));
     const dim3 grid(nb), block(kThreads);
     switch ((R + 15) / 16) {
-        case 1: hipLaunchKernelGGL((gram_mfma64_partial<TA, 1>), grid, block, 0, s, A, m, R, part.as<double>()); break;
-        case 2: hipLaunchKernelGGL((gram_mfma64_partial<TA, 2>), grid, block, 0, s, A, m, R, part.as<double>()); break;
-        case 3: hipLaunchKernelGGL((gram_mfma64_partial<TA, 3>), grid, block, 0, s, A, m, R, part.as<double>()); break;
-        default: hipLaunchKernelGGL((gram_mfma64_partial<TA, 4>), grid, block, 0, s, A, m, R, part.as<double>()); break;
+        case 1: hipLaunchKernelGGL((gram_mfma64_partial<TA, 1>), grid, block, 0, s, A, m, m_dev, R, part.as<double>()); break;
+        case 2: hipLaunchKernelGGL((gram_mfma64_partial<TA, 2>), grid, block, 0, s, A, m, m_dev, R, part.as<double>()); break;
+        case 3: hipLaunchKernelGGL((gram_mfma64_partial<TA, 3>), grid, block, 0, s, A, m, m_dev, R, part.as<double>()); break;
+        default: hipLaunchKernelGGL((gram_mfma64_partial<TA, 4>), grid, block, 0, s, A, m, m_dev, R, part.as<double>()); break;
     }
     EX_HIP(hipGetLastError());
     gram_final_launch(s, part.as<double>(), nb, R, G);
 }
 
-void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part) {
-    gram_mfma64<double>(s, A, m, R, G, part);
+void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part, const uint64_t* m_dev) {
+    gram_mfma64<double>(s, A, m, R, G, part, m_dev);
 }
 
 void gram_f32in(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part) {
@@ -713,7 +716,7 @@ void col_sumsq(hipStream_t s, const double* U, int64_t m, int R, double* sigma, 
 }
 
 void scale_cols(hipStream_t s, double* U, int64_t m, int R, const double* sigma) {
-    hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_for(m * R, kThreads, 8192)), dim3(kThreads), 0, s, U, m, R, sigma);
+    hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_for(m, kThreads / R, 8192)), dim3(kThreads), 0, s, U, m, R, sigma);
     EX_HIP(hipGetLastError());
 }
 

@@ -58,8 +58,10 @@ __device__ __forceinline__ void gram_chunk(const float* __restrict__ A, int64_t 
 // order and the block writes its upper-triangle partial (FP64) for the FP64
 // final reduction.
 template <int NT>
-__global__ __launch_bounds__(kThreads) void gram_bf16_partial(const float* __restrict__ A, int64_t m, int R,
+__global__ __launch_bounds__(kThreads) void gram_bf16_partial(const float* __restrict__ A, int64_t mb,
+                                                              const uint64_t* __restrict__ m_dev, int R,
                                                               double* __restrict__ part) {
+    const int64_t m = m_dev ? ((int64_t)*m_dev < mb ? (int64_t)*m_dev : mb) : mb;
     __shared__ double red[kMaxRank * kMaxRank];
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     f32x4 acc[NT][NT];
@@ -374,8 +376,9 @@ struct DesignArgs32 {
 };
 
 __global__ __launch_bounds__(kThreads) void design_f32(const uint32_t* __restrict__ sd, const double* __restrict__ wd,
-                                                       int64_t Ju, DesignArgs32 a, float* __restrict__ dg,
-                                                       float* __restrict__ dm) {
+                                                       int64_t Jb, const uint64_t* __restrict__ Ju_dev, DesignArgs32 a,
+                                                       float* __restrict__ dg, float* __restrict__ dm) {
+    const int64_t Ju = Ju_dev ? ((int64_t)*Ju_dev < Jb ? (int64_t)*Ju_dev : Jb) : Jb;
     const RowMap rm(a.R);
     if (!rm.on()) return;
     const int r = rm.c;
@@ -483,10 +486,10 @@ namespace k {
         }                                                                                              \
     } while (0)
 
-void gram_bf16(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part) {
+void gram_bf16(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part, const uint64_t* m_dev) {
     const int nb = (int)grid_for((m + 31) / 32, 4, 512);
     part.ensure((size_t)nb * R * R * sizeof(double));
-    NT_DISPATCH(R, gram_bf16_partial, dim3(nb), dim3(kThreads), s, A, m, R, part.as<double>());
+    NT_DISPATCH(R, gram_bf16_partial, dim3(nb), dim3(kThreads), s, A, m, m_dev, R, part.as<double>());
     EX_HIP(hipGetLastError());
     gram_final_launch(s, part.as<double>(), nb, R, G);
 }
@@ -530,12 +533,13 @@ void to_f64(hipStream_t s, const float* a, int64_t n, double* b) {
 }
 
 void design32(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
-              const float* const* U, float* dg, float* dm) {
+              const float* const* U, float* dg, float* dm, const uint64_t* Ju_dev) {
     if (Ju == 0) return;
     DesignArgs32 a;
     for (int q = 0; q < N; q++) a.U[q] = U[q];
     a.N = N; a.j = j; a.R = R;
-    hipLaunchKernelGGL(design_f32, dim3(grid_for(Ju * R, kThreads, 16384)), dim3(kThreads), 0, s, sd, wd, Ju, a, dg, dm);
+    hipLaunchKernelGGL(design_f32, dim3(grid_for(Ju, kThreads / R, 16384)), dim3(kThreads), 0, s, sd, wd, Ju, Ju_dev, a,
+                       dg, dm);
     EX_HIP(hipGetLastError());
 }
 
@@ -548,7 +552,7 @@ void renormalize32(hipStream_t s, float* U, int64_t m, int R, double* sigma, Dev
     part.ensure((size_t)nb * R * sizeof(double));
     hipLaunchKernelGGL(col_sumsq_f32, dim3(nb), dim3(kThreads), 0, s, U, m, R, part.as<double>());
     col_sumsq_final_launch(s, part.as<double>(), nb, R, sigma);
-    hipLaunchKernelGGL(scale_cols_f32, dim3(grid_for(m * R, kThreads, 8192)), dim3(kThreads), 0, s, U, m, R, sigma);
+    hipLaunchKernelGGL(scale_cols_f32, dim3(grid_for(m, kThreads / R, 8192)), dim3(kThreads), 0, s, U, m, R, sigma);
     EX_HIP(hipGetLastError());
 }
 

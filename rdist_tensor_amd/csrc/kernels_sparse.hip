@@ -64,9 +64,21 @@ __global__ __launch_bounds__(kThreads) void fiber_scatter_keys(const uint64_t* _
         if (flags[i]) fkey[pos[i]] = keys[i];
 }
 
+// Number of runs; optionally also added to a running total (one thread).
 __global__ void run_count(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, int64_t n,
-                          uint64_t* __restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *out = n > 0 ? (uint64_t)pos[n - 1] + flags[n - 1] : 0;
+                          uint64_t* __restrict__ out, uint64_t* __restrict__ total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint64_t c = n > 0 ? (uint64_t)pos[n - 1] + flags[n - 1] : 0;
+        *out = c;
+        if (total) *total += c;
+    }
+}
+
+// Effective row count: the host bound, or the device count when given.
+__device__ __forceinline__ int64_t rows_of(int64_t bound, const uint64_t* dev) {
+    if (!dev) return bound;
+    const int64_t d = (int64_t)*dev;
+    return d < bound ? d : bound;
 }
 
 // ---------------------------------------------------------------- dedup
@@ -108,8 +120,10 @@ struct DesignArgs {
 // d_gram = prod_{k != j} U_k[s_k] * sqrt(w);  d_mttkrp = d_gram * sqrt(w)
 // (accumulator_stationary.hpp:210-249).
 __global__ __launch_bounds__(kThreads) void design_kernel(const uint32_t* __restrict__ sd, const double* __restrict__ wd,
-                                                          int64_t Ju, DesignArgs a, double* __restrict__ dg,
+                                                          int64_t Jb, const uint64_t* __restrict__ Ju_dev,
+                                                          DesignArgs a, double* __restrict__ dg,
                                                           double* __restrict__ dm) {
+    const int64_t Ju = rows_of(Jb, Ju_dev);
     const RowMap rm(a.R);
     if (!rm.on()) return;
     const int r = rm.c;
@@ -170,10 +184,12 @@ __global__ __launch_bounds__(kThreads) void rows_of_keyorder(const uint32_t* __r
 constexpr int kFilterBits = 1 << 19;
 constexpr int kFilterWords = kFilterBits / 32;
 
-__global__ __launch_bounds__(kThreads) void mark_fibers(const uint64_t* __restrict__ kd, int64_t Ju,
+__global__ __launch_bounds__(kThreads) void mark_fibers(const uint64_t* __restrict__ kd, int64_t Jb,
+                                                        const uint64_t* __restrict__ Ju_dev,
                                                         const uint64_t* __restrict__ fkey, int64_t nfib,
                                                         uint32_t stamp, uint64_t* __restrict__ cmap,
                                                         uint32_t* __restrict__ filt, uint32_t* __restrict__ fsel) {
+    const int64_t Ju = rows_of(Jb, Ju_dev);
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < Ju; c += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t key = kd[c];
         int64_t lo = 0, hi = nfib;
@@ -193,8 +209,10 @@ __global__ __launch_bounds__(kThreads) void mark_fibers(const uint64_t* __restri
 }
 
 // Every set bit belongs to this step's samples, so whole words are cleared.
-__global__ __launch_bounds__(kThreads) void clear_fibers(const uint32_t* __restrict__ fsel, int64_t Ju,
+__global__ __launch_bounds__(kThreads) void clear_fibers(const uint32_t* __restrict__ fsel, int64_t Jb,
+                                                         const uint64_t* __restrict__ Ju_dev,
                                                          uint32_t* __restrict__ filt) {
+    const int64_t Ju = rows_of(Jb, Ju_dev);
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < Ju; c += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t f = fsel[c];
         if (f != ~0u) filt[(f & (kFilterBits - 1)) >> 5] = 0u;
@@ -625,7 +643,7 @@ void build_lookup(hipStream_t s, const uint32_t* coords, const double* vals, int
     hipLaunchKernelGGL(head_flags_u64, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, kout, nnz, flags);
     exclusive_scan_u32(s, tmp[2].p, scan_b, flags, pos, nnz);
     uint64_t* cnt = (uint64_t*)kin;   // keys_in is free now
-    hipLaunchKernelGGL(run_count, dim3(1), dim3(64), 0, s, flags, pos, nnz, cnt);
+    hipLaunchKernelGGL(run_count, dim3(1), dim3(64), 0, s, flags, pos, nnz, cnt, nullptr);
     const int64_t nf = (int64_t)read_u64(s, cnt);
     L.nfib = nf;
     L.fkey.ensure((size_t)nf * 8);
@@ -653,7 +671,7 @@ void build_lookup(hipStream_t s, const uint32_t* coords, const double* vals, int
 
 // tmp[0..3] are scratch buffers owned by the caller.
 int64_t dedup(hipStream_t s, const uint32_t* samples, const double* weights, int64_t J, int N, const KeyLayout& kl,
-              uint32_t* sd, double* wd, uint64_t* kd, DevBuf* tmp) {
+              uint32_t* sd, double* wd, uint64_t* kd, DevBuf* tmp, uint64_t* Ju_dev, uint64_t* Ju_total) {
     if (J == 0) return 0;
     size_t sort_bytes = sort_pairs_u64_u32(s, nullptr, 0, nullptr, nullptr, nullptr, nullptr, J, kl.bits);
     size_t scan_bytes = exclusive_scan_u32(s, nullptr, 0, nullptr, nullptr, J);
@@ -678,29 +696,31 @@ int64_t dedup(hipStream_t s, const uint32_t* samples, const double* weights, int
     sort_pairs_u64_u32(s, b + o_tmp, sort_bytes, kin, kout, iin, iout, J, kl.bits);
     hipLaunchKernelGGL(head_flags_u64, dim3(grid_for(J, kThreads, 16384)), dim3(kThreads), 0, s, kout, J, fl);
     exclusive_scan_u32(s, b + o_tmp, scan_bytes, fl, pos, J);
-    hipLaunchKernelGGL(run_count, dim3(1), dim3(64), 0, s, fl, pos, J, cnt);
+    if (Ju_dev) cnt = Ju_dev;
+    hipLaunchKernelGGL(run_count, dim3(1), dim3(64), 0, s, fl, pos, J, cnt, Ju_total);
     hipLaunchKernelGGL(dedup_scatter, dim3(grid_for(J, kThreads, 16384)), dim3(kThreads), 0, s, kout, iout, J, fl, pos,
                        hpos, rep, kd);
     hipLaunchKernelGGL(dedup_finish, dim3(grid_for(J, kThreads, 16384)), dim3(kThreads), 0, s, samples, weights, N,
                        hpos, rep, cnt, sd, wd);
     EX_HIP(hipGetLastError());
-    return (int64_t)read_u64(s, cnt);
+    return Ju_dev ? -1 : (int64_t)read_u64(s, cnt);
 }
 
 void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
-            const double* const* U, double* d_gram, double* d_mttkrp) {
+            const double* const* U, double* d_gram, double* d_mttkrp, const uint64_t* Ju_dev) {
     if (Ju == 0) return;
     DesignArgs a;
     for (int k = 0; k < N; k++) a.U[k] = U[k];
     a.N = N; a.j = j; a.R = R;
-    hipLaunchKernelGGL(design_kernel, dim3(grid_for(Ju, kThreads / R, 16384)), dim3(kThreads), 0, s, sd, wd, Ju, a,
-                       d_gram, d_mttkrp);
+    hipLaunchKernelGGL(design_kernel, dim3(grid_for(Ju, kThreads / R, 16384)), dim3(kThreads), 0, s, sd, wd, Ju,
+                       Ju_dev, a, d_gram, d_mttkrp);
     EX_HIP(hipGetLastError());
 }
 
 template <typename V>
 void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju, const V* design,
-                    int R, V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts, hipEvent_t* scan_events) {
+                    int R, V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts, hipEvent_t* scan_events,
+                    const uint64_t* Ju_dev) {
     EX_HIP(hipMemsetAsync(Y, 0, (size_t)I_j * R * sizeof(V), s));
     if (counts) EX_HIP(hipMemsetAsync(counts, 0, 16, s));
     if (scan_events) EX_HIP(hipEventRecord(scan_events[0], s));
@@ -729,7 +749,7 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
         V* vec;
         uint32_t *flag, *key;
         views(tmp[2], n1, vec, flag, key);
-        hipLaunchKernelGGL(mark_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, kd, Ju,
+        hipLaunchKernelGGL(mark_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, kd, Ju, Ju_dev,
                            L.fkey.as<uint64_t>(), L.nfib, stamp, L.cmap.as<uint64_t>(),
                            L.filt.as<uint32_t>(), fsel);
         const dim3 scan_grid(grid_for(nch, kScanThreads / 64, 2 * num_cus(s)));
@@ -753,7 +773,7 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
             if (tmp[5 - cur].bytes < list_bytes(2 * ((n + kScanChunk - 1) / kScanChunk)))
                 tmp[5 - cur].ensure(list_bytes(2 * ((n + kScanChunk - 1) / kScanChunk)));
         }
-        hipLaunchKernelGGL(clear_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, fsel, Ju,
+        hipLaunchKernelGGL(clear_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, fsel, Ju, Ju_dev,
                            L.filt.as<uint32_t>());
     }
     if (scan_events) EX_HIP(hipEventRecord(scan_events[1], s));
@@ -761,9 +781,9 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
 }
 
 template void sampled_mttkrp<double>(hipStream_t, ModeLookup&, const double*, const uint64_t*, int64_t, const double*,
-                                     int, double*, int64_t, DevBuf*, uint64_t*, hipEvent_t*);
+                                     int, double*, int64_t, DevBuf*, uint64_t*, hipEvent_t*, const uint64_t*);
 template void sampled_mttkrp<float>(hipStream_t, ModeLookup&, const float*, const uint64_t*, int64_t, const float*,
-                                    int, float*, int64_t, DevBuf*, uint64_t*, hipEvent_t*);
+                                    int, float*, int64_t, DevBuf*, uint64_t*, hipEvent_t*, const uint64_t*);
 
 void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
              const double* sigma, int R, double* out, DevBuf& part) {
