@@ -77,6 +77,14 @@ def mttkrp_bytes(N, R, Ju, H, T, s=8):
     return Ju * (4 * (N - 1) + 8) + Ju * (N - 1) * R * s + H * (4 * N + s) + T * R * s
 
 
+def exact_mttkrp_bytes(dims, j, nnz, R, s=8):
+    """Algorithmic bytes of one exact MTTKRP (mode j): every nonzero's
+    coordinates and value, each non-target factor read once, the output
+    written once."""
+    N = len(dims)
+    return nnz * (4 * N + s) + sum(dims[k] for k in range(N) if k != j) * R * s + dims[j] * R * s
+
+
 def load_traffic(path):
     """Measured HBM bytes per launch from the committed rocprofv3 PMC summary
     (a JSON block in the markdown file): {"<span>/<precision>/<config>": bytes}."""
@@ -129,6 +137,48 @@ def run_gpu(args, cfg, ctx, T, U0, precision):
     log(f"[bench] {precision}: {args.steps} sweeps in {elapsed:.3f} s; fit {fit0:.6f} -> {fit1:.6f}")
     del A, S, F
     return dict(elapsed=elapsed, timing=timing, fit0=fit0, fit1=fit1)
+
+
+def run_exact(args, cfg, ctx, T, U0, precision, nnz):
+    """ExactALS (no sampling) from the same initial factors: --exact-steps
+    timed sweeps; the exact MTTKRP launch group's roofline."""
+    from rdist_tensor_amd import abi
+    dims = cfg["dims"]
+    R, N = cfg["R"], len(cfg["dims"])
+    prec = abi.EX_MIXED if precision == "mixed" else abi.EX_FP64
+    F = abi.Factors(ctx, dims, R)
+    for k in range(N):
+        F.set(k, U0[k])
+    A = abi.ALS.exact(ctx, T, F, precision=prec)
+    A.init()
+    ctx.sync()
+    fit0 = A.fit()
+    ctx.timing(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for sweep in range(1, args.exact_steps + 1):
+        for j in range(N):
+            A.step(j, 0, sweep)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    timing = json.loads(ctx.timing_json())
+    ctx.timing(False)
+    fit1 = A.fit()
+    log(f"[bench] exact {precision}: {args.exact_steps} sweeps in {elapsed:.3f} s; fit {fit0:.6f} -> {fit1:.6f}")
+    del A, F
+    K = args.exact_steps
+    vb = 4 if precision == "mixed" else 8
+    st = timing["spans"]["exact_mttkrp"]
+    avg = st["ms"] / st["count"] * 1e-3
+    per = float(np.mean([exact_mttkrp_bytes(dims, j, nnz, R, vb) for j in range(N)]))
+    achieved = per / avg / 1e9
+    return dict(value=K / elapsed, ms_per_step=elapsed / K * 1e3,
+                roofline={"kernel": "exact_mttkrp", "bound": "hbm", "achieved": round(achieved, 2),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                          "traffic": None, "avg_launch_ms": round(avg * 1e3, 4),
+                          "algorithmic_bytes_per_launch": int(per)},
+                spans_ms_per_sweep={k: round(v["ms"] / K, 4) for k, v in timing["spans"].items()},
+                fit={"initial": fit0, "final": fit1})
 
 
 def summarize(args, cfg, g, vbytes, precision):
@@ -205,6 +255,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
+    ap.add_argument("--exact-steps", type=int, default=3,
+                    help="ExactALS sweeps per precision reported beside the randomized paths (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -227,6 +279,10 @@ def main():
     for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
         g = run_gpu(args, cfg, ctx, T, U0, prec)
         res[prec] = summarize(args, cfg, g, 4 if prec == "mixed" else 8, prec)
+    exact = {}
+    if args.exact_steps > 0:
+        for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+            exact[prec] = run_exact(args, cfg, ctx, T, U0, prec, len(idx))
     N, R, J = len(cfg["dims"]), cfg["R"], cfg["J"]
     K = args.steps
     head = "mixed" if "mixed" in res else "fp64"
@@ -264,6 +320,8 @@ def main():
                                 "final_fit_fp64": res["fp64"]["fit"]["final"],
                                 "final_fit_abs_diff": abs(res["mixed"]["fit"]["final"] - res["fp64"]["fit"]["final"])}
     out["paths"] = res
+    if exact:
+        out["exact_als"] = exact
     print(json.dumps(out), flush=True)
 
 

@@ -96,6 +96,14 @@ int ex_tensor_build_lookups(ex_tensor* t);
  * *hits receives the number of nonzeros touched (nonzeros_iterated). */
 int ex_sampled_mttkrp(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samples,
                       const double* design, int64_t R, double* Y, int64_t* hits);
+/* How the sampled MTTKRP finds the sampled fibers' nonzeros (no reference
+ * counterpart; the results agree to rounding).  EX_MTTKRP_ROW_PASS reads
+ * every nonzero's row and fiber number in row order; EX_MTTKRP_HIT_LIST
+ * marks the sampled fibers' nonzeros, compacts the marks into a list of hit
+ * positions in row order and visits only those; EX_MTTKRP_AUTO (default)
+ * picks per step from nnz, the fiber count and J_u. */
+enum ex_mttkrp_path { EX_MTTKRP_AUTO = 0, EX_MTTKRP_ROW_PASS = 1, EX_MTTKRP_HIT_LIST = 2 };
+int ex_tensor_mttkrp_path(ex_tensor* t, int path);
 
 /* ---- factors (LowRankTensor) ----------------------------------------- */
 /* LowRankTensor(R, TensorGrid&) -- low_rank_tensor.hpp:16-25: N factor
@@ -150,6 +158,13 @@ int ex_arls_scores(ex_sampler* s, int mode, double* scores);
  * At one rank both distributions perform the same arithmetic. */
 int ex_als_create(ex_ctx* ctx, ex_tensor* t, ex_factors* f, ex_sampler* s,
                   int distribution, int precision, ex_als** out);
+/* ExactALS(SparseTensor&, LowRankTensor&) -- exact_als.hpp:16-51: the
+ * non-randomized optimizer (full MTTKRP over every nonzero, Hadamard product
+ * of Grams, pseudo-inverse).  Returns an ex_als handle used with the same
+ * calls as the randomized optimizers; their J and sweep arguments are ignored
+ * (exact_als.hpp:94).  precision: EX_FP64 or EX_MIXED (FP32 factors and
+ * MTTKRP, FP64 Grams and R x R solves). */
+int ex_exact_als_create(ex_ctx* ctx, ex_tensor* t, ex_factors* f, int precision, ex_als** out);
 int ex_als_destroy(ex_als* a);
 /* ALS_Optimizer::initialize_ground_truth_for_als -- accumulator_stationary.hpp:38-137 */
 int ex_als_init_ground_truth(ex_als* a);

@@ -1011,7 +1011,10 @@ orc_als* orc_als_create(int N, const int64_t* dims, int64_t nnz, const uint32_t*
         a->lk[k] = orc_lookup_create(N, k, dims, nnz, idx, vals);
         if (!a->lk[k]) { orc_als_free(a); return NULL; }
     }
-    if (sampler == ORC_ARLS) {
+    if (sampler == ORC_EXACT) {
+        /* ExactALS keeps the factor Grams (exact_als.hpp:20-50); they are
+         * recomputed from the factors at each step here, same values */
+    } else if (sampler == ORC_ARLS) {
         a->scores = (double**)calloc(N, sizeof(double*));
         for (int k = 0; k < N; k++) {
             a->scores[k] = (double*)malloc(sizeof(double) * dims[k]);
@@ -1036,9 +1039,43 @@ double orc_als_fit(orc_als* a) {
     return orc_fit(a->lk[0], a->N, a->dims, (const double* const*)a->U, a->sigma, a->R, a->tensor_norm);
 }
 
+/* ExactALS::execute_ALS_step (exact_als.hpp:95-141): Hadamard product of the
+ * other factors' Grams, its pseudo-inverse (compute_pinv_square, tolerance
+ * 1e-11), the exact MTTKRP, U_j = Y G+, column renormalisation.  ExactALS
+ * leaves nonzeros_iterated at zero, as here. */
+static void orc_exact_step(orc_als* a, int j) {
+    int N = a->N;
+    int64_t R = a->R, Ij = a->dims[j];
+    double t0 = now_s();
+    double** G = (double**)malloc(sizeof(double*) * N);
+    for (int k = 0; k < N; k++) {
+        G[k] = (double*)malloc(sizeof(double) * R * R);
+        orc_gram(a->U[k], a->dims[k], R, G[k]);
+    }
+    double* P = (double*)malloc(sizeof(double) * R * R);
+    double* Pi = (double*)malloc(sizeof(double) * R * R);
+    orc_chain_had((const double* const*)G, N, j, R, P);
+    orc_pinv(P, R, 1e-11, Pi);
+    double t1 = now_s();
+    a->st.t_design += t1 - t0;
+    double* Y = (double*)calloc((size_t)Ij * R, sizeof(double));
+    orc_exact_mttkrp(a->lk[j], (const double* const*)a->U, R, Y, Ij);
+    double t2 = now_s();
+    a->st.t_spmm += t2 - t1;
+    orc_dsymm(Pi, Y, Ij, R, a->U[j]);
+    orc_renormalize(a->U[j], Ij, R, a->sigma);
+    a->st.t_post += now_s() - t2;
+    for (int k = 0; k < N; k++) free(G[k]);
+    free(G); free(P); free(Pi); free(Y);
+}
+
 void orc_als_step(orc_als* a, int j, int64_t J, uint64_t sweep) {
     int N = a->N;
     int64_t R = a->R, Ij = a->dims[j];
+    if (a->sampler == ORC_EXACT) {
+        orc_exact_step(a, j);
+        return;
+    }
     double t0 = now_s();
     double* de = (double*)malloc(sizeof(double) * J * N);
     double* dr = (double*)malloc(sizeof(double) * J * N);

@@ -85,7 +85,7 @@ double orc_fit(const orc_lookup* l0, int N, const int64_t* dims, const double* c
                const double* sigma, int64_t R, double tensor_norm);
 
 /* ---- whole ALS driver (als_optimizer.hpp:40-124) ---- */
-enum { ORC_ARLS = 0, ORC_STS = 1 };
+enum { ORC_ARLS = 0, ORC_STS = 1, ORC_EXACT = 2 };   /* ORC_EXACT: ExactALS, exact_als.hpp:16-143 */
 typedef struct {
     double t_sampling, t_gather, t_design, t_spmm, t_reduce, t_post, t_update;
     double als_time, fit_time, nonzeros_iterated;

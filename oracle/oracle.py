@@ -371,8 +371,9 @@ class Lookup:
 
 
 class ALS:
-    """Whole randomized CP-ALS (accumulator-stationary, one rank) in the oracle.
-    Factors are updated in place in self.U / self.sigma."""
+    """Whole CP-ALS (accumulator-stationary, one rank) in the oracle: sampler
+    "arls" (CP-ARLS-LEV), "sts" (STS-CP) or "exact" (ExactALS, no sampling;
+    J is ignored).  Factors are updated in place in self.U / self.sigma."""
 
     def __init__(self, idx, vals, dims, U, sampler="sts", seed=0):
         self.idx = u32(idx)
@@ -382,7 +383,7 @@ class ALS:
         self.R = self.U[0].shape[1]
         self.sigma = np.ones(self.R)
         self._uptr = _ptr_array(self.U)
-        kind = 1 if sampler == "sts" else 0
+        kind = {"arls": 0, "sts": 1, "exact": 2}[sampler]
         nnz, N = self.idx.shape
         self.N = N
         self.h = lib().orc_als_create(N, _p(self.dims, _i64), nnz, _p(self.idx, _u32), _p(self.vals),

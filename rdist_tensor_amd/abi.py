@@ -17,6 +17,7 @@ EX_FP64, EX_MIXED = 0, 1
 EX_SAMPLER_CP_ARLS_LEV, EX_SAMPLER_STS_CP = 0, 1
 EX_DIST_ACCUMULATOR_STATIONARY, EX_DIST_TENSOR_STATIONARY = 0, 1
 EX_PRE_NONE, EX_PRE_LOG_COUNT, EX_PRE_ONES = 0, 1, 2
+EX_MTTKRP_AUTO, EX_MTTKRP_ROW_PASS, EX_MTTKRP_HIT_LIST = 0, 1, 2
 
 _vp = C.c_void_p
 _d = C.POINTER(C.c_double)
@@ -53,6 +54,8 @@ SIGNATURES = {
     "ex_sts_compute_m": [_vp, C.c_int, _d],
     "ex_arls_scores": [_vp, C.c_int, _d],
     "ex_als_create": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _pp],
+    "ex_exact_als_create": [_vp, _vp, _vp, C.c_int, _pp],
+    "ex_tensor_mttkrp_path": [_vp, C.c_int],
     "ex_als_destroy": [_vp],
     "ex_als_init_ground_truth": [_vp],
     "ex_als_step": [_vp, C.c_int, C.c_int64, C.c_uint64],
@@ -193,6 +196,9 @@ class Tensor:
         check(lib().ex_tensor_norm(self.h, C.byref(v)))
         return v.value
 
+    def mttkrp_path(self, path):
+        check(lib().ex_tensor_mttkrp_path(self.h, path))
+
     def build_lookups(self):
         check(lib().ex_tensor_build_lookups(self.h))
 
@@ -293,6 +299,16 @@ class ALS:
         self.keep = (tensor, factors, sampler)
         self.h = C.c_void_p()
         check(lib().ex_als_create(ctx.h, tensor.h, factors.h, sampler.h, distribution, precision, C.byref(self.h)))
+
+    @classmethod
+    def exact(cls, ctx, tensor, factors, precision=EX_FP64):
+        """ExactALS (exact_als.hpp:16-143): J and sweep arguments are ignored."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.keep = (tensor, factors)
+        self.h = C.c_void_p()
+        check(lib().ex_exact_als_create(ctx.h, tensor.h, factors.h, precision, C.byref(self.h)))
+        return self
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -332,10 +332,11 @@ struct TensorStationary : ALS_Optimizer {
     }
 };
 
+// exact_als.hpp:16-51; `precision` ("fp64" or "mixed") has no reference
+// counterpart.  execute_ALS_step / execute_ALS_rounds ignore J.
 struct ExactALS : ALS_Optimizer {
-    ExactALS(SparseTensor&, LowRankTensor&) {
-        throw std::runtime_error("ExactALS is not built in this library: it provides the randomized "
-                                 "(CP-ARLS-LEV / STS-CP) ALS path only");
+    ExactALS(SparseTensor& t, LowRankTensor& f, const std::string& prec) {
+        check(ex_exact_als_create(context(), t.t, f.f, precision_code(prec), &a));
     }
 };
 
@@ -384,7 +385,9 @@ PYBIND11_MODULE(py_module, m) {
         .def("execute_ALS_step", &ALS_Optimizer::execute_ALS_step, py::arg("mode"), py::arg("J"),
              py::arg("sweep") = 1);
     py::class_<ExactALS, ALS_Optimizer>(m, "ExactALS")
-        .def(py::init<SparseTensor&, LowRankTensor&>());
+        .def(py::init<SparseTensor&, LowRankTensor&, const std::string&>(), py::arg("ground_truth"),
+             py::arg("low_rank_tensor"), py::arg("precision") = "fp64", py::keep_alive<1, 2>(),
+             py::keep_alive<1, 3>());
     py::class_<TensorStationary, ALS_Optimizer>(m, "TensorStationary")
         .def(py::init<SparseTensor&, LowRankTensor&, Sampler&>(), py::keep_alive<1, 2>(), py::keep_alive<1, 3>(),
              py::keep_alive<1, 4>());

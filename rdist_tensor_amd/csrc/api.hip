@@ -258,6 +258,10 @@ struct ex_als {
     ex_sampler* s = nullptr;
     int dist = 0, precision = 0;
     bool initialized = false;
+    // ExactALS (no sampler): Grams of every factor, N x R x R FP64, current
+    // once grams_valid is set (each step refreshes its own mode's)
+    bool exact = false, grams_valid = false;
+    DevBuf grams;
     DevBuf sd, wd, kd, dg, dm, G, Gi, Gi32, GY, Ms, Y, scal;
     DevBuf tmp[4];
     DevBuf part;
@@ -525,6 +529,97 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     if (a->ev_pending.size() >= 6 * 64) als_resolve_regions(a);
 }
 
+// Grams of every factor into a->grams (FP64 products and sums in both
+// precisions; the mixed path reads the FP32 factors).
+void exact_grams(ex_als* a) {
+    ex_factors* f = a->f;
+    hipStream_t st = a->ctx->c.stream;
+    const int N = f->N, R = f->R;
+    a->grams.ensure((size_t)N * R * R * 8);
+    for (int k2 = 0; k2 < N; k2++) {
+        double* Gk = a->grams.as<double>() + (int64_t)k2 * R * R;
+        if (a->precision == EX_MIXED) {
+            f->sync32(k2);
+            k::gram_f32in(st, f->U32[k2].as<float>(), f->dims[k2], R, Gk, a->part);
+        } else {
+            f->sync64(k2);
+            k::gram(st, f->U[k2].as<double>(), f->dims[k2], R, Gk, a->part);
+        }
+    }
+    a->grams_valid = true;
+}
+
+// ExactALS::execute_ALS_step (exact_als.hpp:95-141): G = Hadamard product of
+// the other factors' Grams, G+ (compute_pinv_square, tolerance 1e-11), the
+// exact MTTKRP Y, U_j = Y G+, column renormalisation into sigma, then the new
+// Gram of U_j.  Region times: R3 = Hadamard + inverse, R4 = MTTKRP, R6 =
+// update, R7 = Gram refresh (the reference leaves them at zero for ExactALS).
+void exact_step_impl(ex_als* a, int j) {
+    ex_factors* f = a->f;
+    hipStream_t st = a->ctx->c.stream;
+    const int N = f->N, R = f->R;
+    const int64_t Ij = f->dims[j];
+    const bool mixed = a->precision == EX_MIXED;
+    if (!a->grams_valid) exact_grams(a);
+    hipEvent_t ev[6];
+    for (int q = 0; q < 6; q++) ev[q] = als_event(a);
+    auto mark = [&](int q) { EX_HIP(hipEventRecord(ev[q], st)); };
+    mark(0);
+    mark(1);
+    a->G.ensure((size_t)R * R * 8);
+    a->Gi.ensure((size_t)R * R * 8);
+    a->Gi32.ensure((size_t)R * R * 4);
+    Span s_gp(a->ctx, "exact_gram_pinv");
+    k::chain_had(st, a->grams.as<double>(), N, j, R, a->G.as<double>());
+    if (mixed) k::spd_inverse(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>(), a->Gi32.as<float>());
+    else k::pinv(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>());
+    s_gp.end();
+    mark(2);
+    ModeLookup& L = a->t->lk[j];
+    const size_t vb = mixed ? 4 : 8;
+    a->Y.ensure((size_t)Ij * R * vb);
+    Span s_mt(a->ctx, "exact_mttkrp");
+    if (mixed) {
+        if (!L.rval32.p) {
+            L.rval32.ensure((size_t)(L.nnz > 0 ? L.nnz : 1) * 4);
+            k::to_f32(st, L.rval.as<double>(), L.nnz, L.rval32.as<float>());
+        }
+        f->sync32_all();
+        k::exact_mttkrp<float>(st, L, L.rval32.as<float>(), f->ptrs32, f->dims, R, a->Y.as<float>(), Ij, a->tmp);
+    } else {
+        f->sync64_all();
+        k::exact_mttkrp<double>(st, L, L.rval.as<double>(), f->ptrs, f->dims, R, a->Y.as<double>(), Ij, a->tmp);
+    }
+    s_mt.end();
+    mark(3);
+    Span s_fu(a->ctx, "factor_update");
+    double* Gj = a->grams.as<double>() + (int64_t)j * R * R;
+    if (mixed) {
+        float* Uj = f->U32[j].as<float>();
+        a->GY.ensure((size_t)R * R * 8);
+        a->Ms.ensure((size_t)R * R * 4);
+        k::gram_f32in(st, a->Y.as<float>(), Ij, R, a->GY.as<double>(), a->part);
+        k::sigma_from_gram(st, a->GY.as<double>(), a->Gi.as<double>(), R, f->sigma.as<double>(), a->Ms.as<float>());
+        k::dsymm32(st, a->Ms.as<float>(), a->Y.as<float>(), Ij, R, Uj);
+        f->wrote32(j);
+    } else {
+        double* Uj = f->U[j].as<double>();
+        k::dsymm(st, a->Gi.as<double>(), a->Y.as<double>(), Ij, R, Uj);
+        k::col_sumsq(st, Uj, Ij, R, f->sigma.as<double>(), a->part);
+        k::scale_cols(st, Uj, Ij, R, f->sigma.as<double>());
+        f->wrote64(j);
+    }
+    s_fu.end();
+    mark(4);
+    Span s_gr(a->ctx, "exact_gram_update");
+    if (mixed) k::gram_f32in(st, f->U32[j].as<float>(), Ij, R, Gj, a->part);
+    else k::gram(st, f->U[j].as<double>(), Ij, R, Gj, a->part);
+    s_gr.end();
+    mark(5);
+    for (int q = 0; q < 6; q++) a->ev_pending.push_back(ev[q]);
+    if (a->ev_pending.size() >= 6 * 64) als_resolve_regions(a);
+}
+
 double als_fit_impl(ex_als* a) {
     ex_factors* f = a->f;
     hipStream_t st = a->ctx->c.stream;
@@ -681,6 +776,15 @@ int ex_tensor_build_lookups(ex_tensor* t) {
     return guarded([&] {
         EX_CHECK(t, EX_ERR_ARG, "null tensor");
         tensor_build_lookups(t);
+    });
+}
+
+int ex_tensor_mttkrp_path(ex_tensor* t, int path) {
+    return guarded([&] {
+        EX_CHECK(t, EX_ERR_ARG, "null tensor");
+        EX_CHECK(path == EX_MTTKRP_AUTO || path == EX_MTTKRP_ROW_PASS || path == EX_MTTKRP_HIT_LIST, EX_ERR_ARG,
+                 "unknown sampled-MTTKRP path");
+        for (int k2 = 0; k2 < kMaxModes; k2++) t->lk[k2].mttkrp_path = path;
     });
 }
 
@@ -965,6 +1069,28 @@ int ex_als_create(ex_ctx* ctx, ex_tensor* t, ex_factors* f, ex_sampler* s, int d
     });
 }
 
+int ex_exact_als_create(ex_ctx* ctx, ex_tensor* t, ex_factors* f, int precision, ex_als** out) {
+    return guarded([&] {
+        EX_CHECK(ctx && t && f && out, EX_ERR_ARG, "null argument");
+        EX_CHECK(t->N == f->N, EX_ERR_ARG, "tensor and factor mode counts differ");
+        for (int k2 = 0; k2 < t->N; k2++) EX_CHECK(t->dims[k2] == f->dims[k2], EX_ERR_ARG, "mode sizes differ");
+        EX_CHECK(precision == EX_FP64 || precision == EX_MIXED, EX_ERR_ARG, "unknown precision");
+        auto a = std::make_unique<ex_als>();
+        a->ctx = ctx;
+        a->t = t;
+        a->f = f;
+        a->exact = true;
+        a->dist = EX_DIST_ACCUMULATOR_STATIONARY;
+        a->precision = precision;
+        a->cnt.ensure(32);
+        EX_HIP(hipMemsetAsync(a->cnt.p, 0, 32, ctx->c.stream));
+        if (precision == EX_MIXED) f->enable32();
+        exact_grams(a.get());
+        EX_HIP(hipStreamSynchronize(ctx->c.stream));
+        *out = a.release();
+    });
+}
+
 int ex_als_destroy(ex_als* a) {
     return guarded([&] {
         if (!a) return;
@@ -993,7 +1119,13 @@ int ex_als_step(ex_als* a, int mode, int64_t J, uint64_t sweep) {
         EX_CHECK(a, EX_ERR_ARG, "null optimizer");
         EX_CHECK(a->initialized, EX_ERR_STATE, "initialize_ground_truth_for_als() has not been called");
         EX_CHECK(mode >= 0 && mode < a->f->N, EX_ERR_ARG, "mode out of range");
-        als_step_impl(a, mode, J, sweep);
+        if (a->exact) {
+            // the factors may have been changed through the LowRankTensor
+            a->grams_valid = false;
+            exact_step_impl(a, mode);
+        } else {
+            als_step_impl(a, mode, J, sweep);
+        }
     });
 }
 
@@ -1014,6 +1146,7 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
         hipStream_t st = a->ctx->c.stream;
         als_resolve_regions(a);
         for (double& x : a->t_region) x = 0.0;
+        a->grams_valid = false;
         uint64_t c0[2];
         d2h(st, c0, a->cnt.p, sizeof(c0));
         double als_total = 0.0, fit_total = 0.0;
@@ -1022,7 +1155,10 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
         for (int64_t round = 1; round <= rounds; round++) {
             EX_HIP(hipStreamSynchronize(st));
             auto t0 = std::chrono::steady_clock::now();
-            for (int j = 0; j < a->f->N; j++) als_step_impl(a, j, J, (uint64_t)round);
+            for (int j = 0; j < a->f->N; j++) {
+                if (a->exact) exact_step_impl(a, j);
+                else als_step_impl(a, j, J, (uint64_t)round);
+            }
             EX_HIP(hipStreamSynchronize(st));
             als_total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (round % epoch == 0) {
@@ -1054,7 +1190,8 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
         js += ", \"precision\": \"" + std::string(a->precision == EX_FP64 ? "fp64" : "mixed") + "\"";
         js += ", \"row_gather_time\": " + dstat(a->t_region[1]);
         js += ", \"rounds\": " + json_list(rl);
-        js += ", \"sampler\": \"" + std::string(a->s->kind == EX_SAMPLER_STS_CP ? "sts_cp" : "cp_arls_lev") + "\"";
+        js += ", \"sampler\": \"" +
+              std::string(a->exact ? "exact" : a->s->kind == EX_SAMPLER_STS_CP ? "sts_cp" : "cp_arls_lev") + "\"";
         js += ", \"sampler_update_time\": " + dstat(a->t_region[6]);
         js += ", \"spmm_time\": " + dstat(a->t_region[3]);
         js += ", \"sum_nonzeros_iterated\": " + json_num(nonzeros);

@@ -125,6 +125,14 @@ struct ModeLookup {
     DevBuf cmap;    // u64[nfib]: (stamp << 32) | design row of the fibers sampled at step `stamp`
     DevBuf filt;    // u32[2^14]: bit (f mod 2^19) set for this step's sampled fibers f (zero between steps)
     uint32_t stamp = 0;
+    // hit-list path
+    DevBuf fptr;    // u32[nfib + 1]: first key-order position of each fiber
+    DevBuf kpos;    // u32[nnz]: row-order position of each key-ordered nonzero
+    DevBuf cpos;    // u32[nnz]: design row of the hit at a row-order position (valid where its bit is set)
+    DevBuf marks;   // u8, whole 4096-position chunks: 1 at this step's hit positions (zero between steps)
+    DevBuf cpart;   // u64 per-workgroup counter partials of the passes
+    int64_t maxfib = 0;    // nonzeros in the longest fiber
+    int mttkrp_path = 0;   // 0: chosen per step from the sizes, 1: full row pass, 2: hit bitmap
 };
 
 struct Ctx {
@@ -150,6 +158,8 @@ void col_sumsq(hipStream_t s, const double* U, int64_t m, int R, double* sigma, 
 void scale_cols(hipStream_t s, double* U, int64_t m, int R, const double* sigma);
 void cos_init(hipStream_t s, double* U, int64_t m, int R);
 void chain_had_norm(hipStream_t s, const double* grams, int N, int R, const double* sigma, double* out);
+// out = elementwise product of grams[k] (k != exclude), k ascending
+void chain_had(hipStream_t s, const double* grams, int N, int exclude, int R, double* out);
 // preprocessing / norms
 void preprocess(hipStream_t s, double* vals, int64_t nnz, int kind);
 void sumsq(hipStream_t s, const double* v, int64_t n, double* out, DevBuf& part);
@@ -201,11 +211,16 @@ void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int
 // Y (I_j x R, zero-filled here) = sampled MTTKRP of the Ju deduplicated
 // samples with fiber keys kd and design rows `design`.  V = double (FP64
 // path) or float (mixed path): values (L.rval / L.rval32), design rows and Y.
-// counts (optional, device u64[2]) receives the hits and touched rows.
+// counts (optional, device u64[2]): the hits and the touched rows are added.
 template <typename V>
 void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju, const V* design,
                     int R, V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts = nullptr,
                     hipEvent_t* scan_events = nullptr, const uint64_t* Ju_dev = nullptr);
+// Y (I_j x R, zero-filled here) = exact MTTKRP: every nonzero x at (i_0..i_N)
+// adds x * prod_{k != mode} U_k[i_k] to row i_mode.  U: the N factors in V.
+template <typename V>
+void exact_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const V* const* U, const int64_t* dims, int R,
+                  V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts = nullptr);
 void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
              const double* sigma, int R, double* out, DevBuf& part);
 // final stages shared with the mixed-precision kernels

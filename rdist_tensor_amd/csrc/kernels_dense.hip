@@ -581,6 +581,19 @@ __global__ __launch_bounds__(kThreads) void chain_had_norm_kernel(const double* 
     }
 }
 
+// chain_had_prod (common.h:521-545): out = elementwise product of the Grams
+// of every mode but `exclude`, multiplied in mode order.
+__global__ __launch_bounds__(kThreads) void chain_had_kernel(const double* __restrict__ grams, int N, int exclude,
+                                                             int R, double* __restrict__ out) {
+    const int R2 = R * R;
+    for (int o = threadIdx.x; o < R2; o += blockDim.x) {
+        double p = 1.0;
+        for (int k = 0; k < N; k++)
+            if (k != exclude) p *= grams[(int64_t)k * R2 + o];
+        out[o] = p;
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void preprocess_kernel(double* __restrict__ v, int64_t n, int kind) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         if (kind == 1) v[i] = log(v[i] + 1.0);
@@ -727,6 +740,11 @@ void cos_init(hipStream_t s, double* U, int64_t m, int R) {
 
 void chain_had_norm(hipStream_t s, const double* grams, int N, int R, const double* sigma, double* out) {
     hipLaunchKernelGGL(chain_had_norm_kernel, dim3(1), dim3(kThreads), 0, s, grams, N, R, sigma, out);
+    EX_HIP(hipGetLastError());
+}
+
+void chain_had(hipStream_t s, const double* grams, int N, int exclude, int R, double* out) {
+    hipLaunchKernelGGL(chain_had_kernel, dim3(1), dim3(kThreads), 0, s, grams, N, exclude, R, out);
     EX_HIP(hipGetLastError());
 }
 

@@ -153,7 +153,7 @@ __device__ __forceinline__ double dpp_x(double v) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
-// Sum over aligned groups of G lanes (16, 32 or 64).  Each step adds the
+// Sum over aligned groups of G lanes (8, 16, 32 or 64).  Each step adds the
 // same two operands on both partner lanes, so every lane of a group ends
 // with the same bits.
 template <int G, typename V>
@@ -161,7 +161,7 @@ __device__ __forceinline__ V group_total(V v) {
     v += dpp_x<0xB1>(v);    // quad_perm [1,0,3,2]
     v += dpp_x<0x4E>(v);    // quad_perm [2,3,0,1]
     v += dpp_x<0x141>(v);   // row_half_mirror: quad 0 <-> quad 1 of each half row
-    v += dpp_x<0x140>(v);   // row_mirror: half rows swapped
+    if (G >= 16) v += dpp_x<0x140>(v);   // row_mirror: half rows swapped
     if (G >= 32) v += __shfl_xor(v, 16, 64);
     if (G >= 64) v += __shfl_xor(v, 32, 64);
     return v;
@@ -180,13 +180,12 @@ template <> struct Vec<double> {
 // LDS of one draw block: the packed-upper (a, b) table, and per sample the
 // running Hadamard product h, the current query and the leaf's q^2.
 constexpr int kDrawThreads = 256;
-constexpr int kMaxSlots = kDrawThreads / 16;
-template <typename TN>
+template <typename TN, int SLOTS>
 struct DrawShared {
     uint16_t pairs[kMaxRank * (kMaxRank + 1) / 2];
-    TN h[kMaxSlots][kMaxRank];
-    TN x[kMaxSlots][kMaxRank];
-    double q2[kMaxSlots][kMaxRank];
+    TN h[SLOTS][kMaxRank];
+    TN x[SLOTS][kMaxRank];
+    double q2[SLOTS][kMaxRank];
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -271,7 +270,7 @@ template <typename TN, int G, int NV>
 __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
     constexpr int W = Vec<TN>::W;
     constexpr int SPW = 64 / G;
-    __shared__ DrawShared<TN> sh;
+    __shared__ DrawShared<TN, (kDrawThreads / 64) * SPW> sh;
     const int R = a.R, N = a.N, j = a.j;
     const int P = tri_size(R);
     const int S = sizeof(TN) == 8 ? tri_stride(R) : tri_stride32(R);
@@ -549,7 +548,7 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
         else DRAW_LAUNCH(double, 64, 17);
     } else {
         if (R <= 16) DRAW_LAUNCH(float, 16, 3);
-        else if (R <= 32) DRAW_LAUNCH(float, 16, 9);
+        else if (R <= 32) DRAW_LAUNCH(float, 8, 17);
         else if (R <= 45) DRAW_LAUNCH(float, 32, 9);
         else DRAW_LAUNCH(float, 64, 9);
     }

@@ -64,6 +64,38 @@ __global__ __launch_bounds__(kThreads) void fiber_scatter_keys(const uint64_t* _
         if (flags[i]) fkey[pos[i]] = keys[i];
 }
 
+// fptr[f] = first key-order position of fiber f; fptr[nfib] = n.
+__global__ __launch_bounds__(kThreads) void fiber_starts(int64_t n, const uint32_t* __restrict__ flags,
+                                                         const uint32_t* __restrict__ pos,
+                                                         uint32_t* __restrict__ fptr) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (flags[i]) fptr[pos[i]] = (uint32_t)i;
+        if (i == n - 1) fptr[pos[i] + flags[i]] = (uint32_t)n;
+    }
+}
+
+// *out = the largest fiber length (nonzeros per fiber); *out starts at 0.
+__global__ __launch_bounds__(kThreads) void max_fiber_len(const uint32_t* __restrict__ fptr, int64_t nfib,
+                                                          unsigned int* __restrict__ out) {
+    uint32_t m = 0;
+    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nfib; f += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t len = fptr[f + 1] - fptr[f];
+        m = len > m ? len : m;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if (threadIdx.x % 64 == 0 && m) atomicMax(out, m);
+}
+
+// kpos[perm[p]] = p: the row-order position of each key-ordered nonzero.
+__global__ __launch_bounds__(kThreads) void invert_perm(const uint32_t* __restrict__ perm, int64_t n,
+                                                        uint32_t* __restrict__ kpos) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        kpos[perm[i]] = (uint32_t)i;
+}
+
 // Number of runs; optionally also added to a running total (one thread).
 __global__ void run_count(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ pos, int64_t n,
                           uint64_t* __restrict__ out, uint64_t* __restrict__ total) {
@@ -233,131 +265,64 @@ __global__ __launch_bounds__(kThreads) void clear_fibers(const uint32_t* __restr
 // counts[1] += rows with a hit that begin here.
 constexpr int kScanChunk = 256;
 constexpr int kHitBatch = 8;   // hits whose loads are issued together
+constexpr int kSegBatch = 32;  // carry entries whose loads are issued together
 
 // The filter sits in LDS (64 KB); two 1024-thread workgroups per CU.
 constexpr int kScanThreads = 1024;
 
+// Exact MTTKRP source (sort_lookup.hpp:440-495): every nonzero is a hit and
+// its row of the Khatri-Rao product is formed from the fiber key, whose
+// field k (bits shift[k] .. shift[k] + width) is the coordinate in mode k.
 template <typename V>
-__global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* __restrict__ rrow,
-                                                            const uint32_t* __restrict__ rfib,
-                                                            const V* __restrict__ rval, int64_t nnz, int64_t nch,
-                                                            uint32_t stamp, const uint64_t* __restrict__ cmap,
-                                                            const uint32_t* __restrict__ filt,
-                                                            const V* __restrict__ design, int R, V* __restrict__ Y,
-                                                            V* __restrict__ carry, uint32_t* __restrict__ cflag,
-                                                            uint32_t* __restrict__ ckey,
-                                                            unsigned long long* __restrict__ counts) {
-    constexpr int NT = kScanThreads;
-    __shared__ uint32_t lfilt[kFilterWords];
-    __shared__ unsigned long long wsum[2][NT / 64];
-    for (int i = threadIdx.x; i < kFilterWords; i += NT) lfilt[i] = filt[i];
-    __syncthreads();
-    const int lane = threadIdx.x % 64;
-    const int lr = lane < R ? lane : 0;
-    constexpr int M = kScanChunk / 64;
-    unsigned long long whits = 0, wheads = 0;   // this wavefront's counts over its chunks
-    for (int64_t w = (int64_t)blockIdx.x * (NT / 64) + threadIdx.x / 64; w < nch;
-         w += (int64_t)gridDim.x * (NT / 64)) {
-        const int64_t b = w * kScanChunk, e = b + kScanChunk < nnz ? b + kScanChunk : nnz;
-        uint32_t hr[M], hc[M];
-        V hx[M];
-        bool hit[M];
-#pragma unroll
-        for (int m = 0; m < M; m++) {
-            const int64_t i = b + 64 * m + lane;
-            hr[m] = 0;
-            hc[m] = 0;
-            hx[m] = (V)0;
-            hit[m] = false;
-            if (i < e) {
-                hr[m] = rrow[i];
-                const uint32_t f = rfib[i];
-                const uint32_t h = f & (kFilterBits - 1);
-                if ((lfilt[h >> 5] >> (h & 31)) & 1u) {
-                    const uint64_t cm = cmap[f];
-                    if ((uint32_t)(cm >> 32) == stamp) {
-                        hit[m] = true;
-                        hc[m] = (uint32_t)cm;
-                        hx[m] = rval[i];
-                    }
-                }
-            }
+struct ExactSrc {
+    const uint64_t* fkey;
+    const V* U[kMaxModes];
+    int shift[kMaxModes];   // -1 for the target mode
+    uint64_t mask[kMaxModes];
+    int N;
+};
+
+// Per-chunk row sums of a wavefront (lane r < R holds column r).  Hits are
+// fed in row order; a row that begins and ends inside the chunk is written
+// to Y, a row crossing a chunk boundary leaves its partial in carry slot 0
+// (running in from the left) or slot 1 (running out to the right).
+template <typename V>
+struct ChunkRows {
+    uint32_t first_row, last_row;   // rows at the chunk's first and last positions
+    bool first_cont, continues;     // first row began before / last row runs past the chunk
+    int64_t w;
+    int lane, R;
+    V* Y;
+    V* carry;
+    uint32_t cur = 0, nheads = 0;
+    bool active = false, cur_head = false;
+    V acc = (V)0;
+    uint32_t f0 = 0, f1 = 0;
+
+    __device__ void flush() {
+        if (cur_head) {
+            if (lane < R) Y[(int64_t)cur * R + lane] = acc;
+            nheads++;
+        } else {
+            if (lane < R) carry[(2 * w) * R + lane] = acc;
+            f0 = 1;
         }
-        const uint32_t first_row = readlane_u32(hr[0], 0);
-        const int last = (int)(e - 1 - b);
-        uint32_t last_row = first_row;
-#pragma unroll
-        for (int m = 0; m < M; m++)
-            if (last / 64 == m) last_row = readlane_u32(hr[m], last % 64);
-        const bool first_cont = b > 0 && rrow[b - 1] == first_row;   // first row began before the chunk
-        const bool continues = e < nnz && rrow[e] == last_row;       // last row runs past the chunk
-        uint32_t cur = 0, nheads = 0, nhits = 0;
-        bool active = false, cur_head = false;
-        V acc = (V)0;
-        uint32_t f0 = 0, f1 = 0;
-#pragma unroll
-        for (int m = 0; m < M; m++) {
-            uint64_t mask = __ballot(hit[m]);
-            nhits += (uint32_t)__popcll(mask);
-            while (mask) {
-                // up to kHitBatch hits: their design rows are loaded together,
-                // then summed in order
-                int ls[kHitBatch];
-                V dv[kHitBatch], xv[kHitBatch];
-                uint32_t rv[kHitBatch];
-#pragma unroll
-                for (int q = 0; q < kHitBatch; q++) {
-                    ls[q] = -1;
-                    if (mask) {
-                        ls[q] = __ffsll((unsigned long long)mask) - 1;
-                        mask &= mask - 1;
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < kHitBatch; q++) {
-                    dv[q] = (V)0;
-                    xv[q] = (V)0;
-                    rv[q] = 0;
-                    if (ls[q] >= 0) {
-                        const uint32_t c = readlane_u32(hc[m], ls[q]);
-                        dv[q] = design[(int64_t)c * R + lr];
-                        xv[q] = readlane_val(hx[m], ls[q]);
-                        rv[q] = readlane_u32(hr[m], ls[q]);
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < kHitBatch; q++) {
-                    if (ls[q] < 0) break;
-                    const uint32_t r = rv[q];
-                    if (!active || r != cur) {
-                        if (active) {
-                            if (cur_head) {
-                                if (lane < R) Y[(int64_t)cur * R + lane] = acc;
-                                nheads++;
-                            } else {
-                                if (lane < R) carry[(2 * w) * R + lane] = acc;
-                                f0 = 1;
-                            }
-                        }
-                        cur = r;
-                        acc = (V)0;
-                        active = true;
-                        cur_head = !(first_cont && r == first_row);
-                    }
-                    acc += dv[q] * xv[q];
-                }
-            }
+    }
+    __device__ void add(uint32_t r, V d, V x) {
+        if (!active || r != cur) {
+            if (active) flush();
+            cur = r;
+            acc = (V)0;
+            active = true;
+            cur_head = !(first_cont && r == first_row);
         }
+        acc += d * x;
+    }
+    __device__ void finish(uint32_t* cflag, uint32_t* ckey) {
         const bool last_head = !(first_cont && last_row == first_row);
         if (active && cur != last_row) {
             // the last hit row ended inside the chunk
-            if (cur_head) {
-                if (lane < R) Y[(int64_t)cur * R + lane] = acc;
-                nheads++;
-            } else {
-                if (lane < R) carry[(2 * w) * R + lane] = acc;
-                f0 = 1;
-            }
+            flush();
             active = false;
             acc = (V)0;
         }
@@ -372,13 +337,7 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
                 f0 = 1;
             }
         } else if (active) {
-            if (cur_head) {
-                if (lane < R) Y[(int64_t)cur * R + lane] = acc;
-                nheads++;
-            } else {
-                if (lane < R) carry[(2 * w) * R + lane] = acc;
-                f0 = 1;
-            }
+            flush();
         }
         if (lane == 0) {
             cflag[2 * w] = f0;
@@ -386,24 +345,465 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
             ckey[2 * w] = first_row;
             ckey[2 * w + 1] = last_row;
         }
-        whits += nhits;
-        wheads += nheads;
     }
-    // one pair of atomics per workgroup
+};
+
+// Feeds the hits held across the wavefront (slot m of lane l = hit 64m + l,
+// in row order) to `st`: up to BATCH hits have their rows (row_of(c), this
+// lane's column) loaded together, then summed in order.
+template <typename V, int M, int BATCH = kHitBatch, typename RowOf>
+__device__ __forceinline__ void feed_hits(ChunkRows<V>& st, const bool (&hit)[M], const uint32_t (&hr)[M],
+                                          const uint32_t (&hc)[M], const V (&hx)[M], RowOf row_of,
+                                          uint32_t& nhits) {
+#pragma unroll
+    for (int m = 0; m < M; m++) {
+        uint64_t mask = __ballot(hit[m]);
+        nhits += (uint32_t)__popcll(mask);
+        while (mask) {
+            int ls[BATCH];
+            V dv[BATCH], xv[BATCH];
+            uint32_t rv[BATCH];
+#pragma unroll
+            for (int q = 0; q < BATCH; q++) {
+                ls[q] = -1;
+                if (mask) {
+                    ls[q] = __ffsll((unsigned long long)mask) - 1;
+                    mask &= mask - 1;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < BATCH; q++) {
+                dv[q] = (V)0;
+                xv[q] = (V)0;
+                rv[q] = 0;
+                if (ls[q] >= 0) {
+                    dv[q] = row_of(readlane_u32(hc[m], ls[q]));
+                    xv[q] = readlane_val(hx[m], ls[q]);
+                    rv[q] = readlane_u32(hr[m], ls[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < BATCH; q++) {
+                if (ls[q] < 0) break;
+                st.add(rv[q], dv[q], xv[q]);
+            }
+        }
+    }
+}
+
+// Sums each wavefront's hit and head counts per workgroup into
+// part[2 * blockIdx.x + {0, 1}] (no atomics: many workgroups adding to one
+// address serialise); sum_counts adds the partials to the counters.
+template <int NT>
+__device__ __forceinline__ void add_counts(unsigned long long whits, unsigned long long wheads,
+                                           unsigned long long* part) {
+    __shared__ unsigned long long wsum[2][NT / 64];
+    const int lane = threadIdx.x % 64;
     if (lane == 0) {
         wsum[0][threadIdx.x / 64] = whits;
         wsum[1][threadIdx.x / 64] = wheads;
     }
     __syncthreads();
-    if (threadIdx.x == 0 && counts) {
+    if (threadIdx.x == 0 && part) {
         unsigned long long h = 0, t = 0;
         for (int q = 0; q < NT / 64; q++) {
             h += wsum[0][q];
             t += wsum[1][q];
         }
-        atomicAdd(counts, h);
-        atomicAdd(counts + 1, t);
+        part[2 * blockIdx.x] = h;
+        part[2 * blockIdx.x + 1] = t;
     }
+}
+
+// counts[0..1] += the per-workgroup partials of nb workgroups (one block).
+__global__ __launch_bounds__(kThreads) void sum_counts(const unsigned long long* __restrict__ part, int nb,
+                                                       uint64_t* __restrict__ counts) {
+    __shared__ unsigned long long red[2][kThreads];
+    unsigned long long h = 0, t = 0;
+    for (int b = threadIdx.x; b < nb; b += kThreads) {
+        h += part[2 * b];
+        t += part[2 * b + 1];
+    }
+    red[0][threadIdx.x] = h;
+    red[1][threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < kThreads; q++) {
+            h += red[0][q];
+            t += red[1][q];
+        }
+        counts[0] += h;
+        counts[1] += t;
+    }
+}
+
+// kExact = false: sampled MTTKRP (hits are the nonzeros of sampled fibers,
+// rows from `design`).  kExact = true: exact MTTKRP from `ex`; the filter,
+// map and design arguments are unused.
+template <typename V, bool kExact>
+__global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* __restrict__ rrow,
+                                                            const uint32_t* __restrict__ rfib,
+                                                            const V* __restrict__ rval, int64_t nnz, int64_t nch,
+                                                            uint32_t stamp, const uint64_t* __restrict__ cmap,
+                                                            const uint32_t* __restrict__ filt,
+                                                            const V* __restrict__ design, ExactSrc<V> ex, int R,
+                                                            V* __restrict__ Y, V* __restrict__ carry,
+                                                            uint32_t* __restrict__ cflag, uint32_t* __restrict__ ckey,
+                                                            unsigned long long* __restrict__ counts) {
+    constexpr int NT = kScanThreads;
+    __shared__ uint32_t lfilt[kExact ? 1 : kFilterWords];
+    if constexpr (!kExact) {
+        for (int i = threadIdx.x; i < kFilterWords; i += NT) lfilt[i] = filt[i];
+        __syncthreads();
+    }
+    const int lane = threadIdx.x % 64;
+    const int lr = lane < R ? lane : 0;
+    constexpr int M = kScanChunk / 64;
+    unsigned long long whits = 0, wheads = 0;   // this wavefront's counts over its chunks
+    const int64_t step = (int64_t)gridDim.x * (NT / 64);
+    // rows and fiber numbers of the next chunk, with the rows just before
+    // and after it, are loaded while the current chunk is processed
+    uint32_t nrow[M], nfib[M], nprev = 0, nnext = 0;
+    auto load_raw = [&](int64_t w2) {
+        const int64_t b2 = w2 * kScanChunk, e2 = b2 + kScanChunk;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int64_t i = b2 + 64 * m + lane;
+            nrow[m] = i < nnz ? rrow[i] : 0u;
+            nfib[m] = i < nnz ? rfib[i] : 0u;
+        }
+        nprev = b2 > 0 ? rrow[b2 - 1] : 0u;
+        nnext = e2 < nnz ? rrow[e2] : 0u;
+    };
+    auto row_of = [ex, design, R, lr](uint32_t c) -> V {
+        if constexpr (kExact) {
+            // product of the fiber's factor rows in mode order
+            const uint64_t key = ex.fkey[c];
+            V p = (V)1;
+#pragma unroll
+            for (int k = 0; k < kMaxModes; k++)
+                if (k < ex.N && ex.shift[k] >= 0)
+                    p *= ex.U[k][(int64_t)((key >> ex.shift[k]) & ex.mask[k]) * R + lr];
+            return p;
+        } else {
+            return design[(int64_t)c * R + lr];
+        }
+    };
+    int64_t w = (int64_t)blockIdx.x * (NT / 64) + threadIdx.x / 64;
+    if (w < nch) load_raw(w);
+    for (; w < nch; w += step) {
+        const int64_t b = w * kScanChunk, e = b + kScanChunk < nnz ? b + kScanChunk : nnz;
+        uint32_t hr[M], hc[M], fr[M];
+        V hx[M];
+        bool hit[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            hr[m] = nrow[m];
+            fr[m] = nfib[m];
+        }
+        const uint32_t prev_row = nprev, next_row = nnext;
+        if (w + step < nch) load_raw(w + step);
+        if constexpr (kExact) {
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const int64_t i = b + 64 * m + lane;
+                hit[m] = i < e;
+                hc[m] = fr[m];
+                hx[m] = hit[m] ? rval[i] : (V)0;
+            }
+        } else {
+            // filter probes first, then every map entry and value of the
+            // probe-positive nonzeros in one batch of independent loads
+            bool pos[M];
+            uint64_t cm[M];
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const int64_t i = b + 64 * m + lane;
+                const uint32_t h = fr[m] & (kFilterBits - 1);
+                pos[m] = i < e && ((lfilt[h >> 5] >> (h & 31)) & 1u);
+            }
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                const int64_t i = b + 64 * m + lane;
+                cm[m] = pos[m] ? cmap[fr[m]] : 0ull;
+                hx[m] = pos[m] ? rval[i] : (V)0;
+            }
+#pragma unroll
+            for (int m = 0; m < M; m++) {
+                hit[m] = pos[m] && (uint32_t)(cm[m] >> 32) == stamp;
+                hc[m] = (uint32_t)cm[m];
+            }
+        }
+        ChunkRows<V> st;
+        st.first_row = readlane_u32(hr[0], 0);
+        const int last = (int)(e - 1 - b);
+        st.last_row = st.first_row;
+#pragma unroll
+        for (int m = 0; m < M; m++)
+            if (last / 64 == m) st.last_row = readlane_u32(hr[m], last % 64);
+        st.first_cont = b > 0 && prev_row == st.first_row;
+        st.continues = e < nnz && next_row == st.last_row;
+        st.w = w;
+        st.lane = lane;
+        st.R = R;
+        st.Y = Y;
+        st.carry = carry;
+        uint32_t nhits = 0;
+        feed_hits<V, M>(st, hit, hr, hc, hx, row_of, nhits);
+        st.finish(cflag, ckey);
+        whits += nhits;
+        wheads += st.nheads;
+    }
+    add_counts<NT>(whits, wheads, counts);
+}
+
+// ---------------------------------------------------------------- hit list
+// Sampled MTTKRP driven by marks over the row-ordered nonzeros: the
+// nonzeros of every sampled fiber are marked (mark byte set, design row
+// recorded at that position), the marks are compacted into the list of hit
+// positions in row order, and the row sums run over that list.  Work is
+// proportional to the hits plus one byte per nonzero instead of every
+// nonzero's row and fiber number.
+
+// Wavefront-cooperative 64-ary search of key in the ascending fkey[0, n):
+// returns its index or -1.
+__device__ __forceinline__ int64_t wave_find_key(const uint64_t* __restrict__ fkey, int64_t n, uint64_t key,
+                                                 int lane) {
+    int64_t lo = 0, hi = n;   // fkey[lo] <= key once a probe has succeeded
+    while (hi - lo > 64) {
+        const int64_t stride = (hi - lo + 63) / 64;
+        const int64_t at = lo + (int64_t)lane * stride;
+        const bool le = at < hi && fkey[at] <= key;
+        const int cnt = __popcll(__ballot(le));
+        if (cnt == 0) return -1;   // key < fkey[lo]
+        lo = lo + (int64_t)(cnt - 1) * stride;
+        const int64_t h2 = lo + stride;
+        hi = h2 < hi ? h2 : hi;
+    }
+    const int64_t at = lo + lane;
+    const uint64_t hitmask = __ballot(at < hi && fkey[at] == key);
+    return hitmask ? lo + __ffsll((unsigned long long)hitmask) - 1 : -1;
+}
+
+// Nonzeros marked per expansion block.
+constexpr int kExpandBlock = 1024;
+
+// One wavefront per deduplicated sample c < Jb: its fiber (fsel[c], ~0 if
+// the tensor has none) and the number of kExpandBlock-nonzero blocks of that
+// fiber (nblk[c]; 0 past the device count J_u or without a fiber).
+__global__ __launch_bounds__(kThreads) void find_fibers(const uint64_t* __restrict__ kd, int64_t Jb,
+                                                        const uint64_t* __restrict__ Ju_dev,
+                                                        const uint64_t* __restrict__ fkey, int64_t nfib,
+                                                        const uint32_t* __restrict__ fptr,
+                                                        uint32_t* __restrict__ fsel, uint32_t* __restrict__ nblk) {
+    const int64_t Ju = rows_of(Jb, Ju_dev);
+    const int lane = threadIdx.x % 64;
+    for (int64_t c = blockIdx.x * (int64_t)(kThreads / 64) + threadIdx.x / 64; c < Jb;
+         c += (int64_t)gridDim.x * (kThreads / 64)) {
+        int64_t f = -1;
+        if (c < Ju) f = wave_find_key(fkey, nfib, kd[c], lane);
+        if (lane == 0) {
+            fsel[c] = f < 0 ? ~0u : (uint32_t)f;
+            nblk[c] = f < 0 ? 0u : (fptr[f + 1] - fptr[f] + kExpandBlock - 1) / kExpandBlock;
+        }
+    }
+}
+
+// owner[boff[c] + i] = c for i < nblk[c]: the sample of each expansion block
+// (boff = exclusive scan of nblk over the Jb samples).
+__global__ __launch_bounds__(kThreads) void block_owners(const uint32_t* __restrict__ nblk,
+                                                         const uint32_t* __restrict__ boff, int64_t Jb,
+                                                         uint32_t* __restrict__ owner) {
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < Jb; c += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t n = nblk[c], o = boff[c];
+        for (uint32_t i = 0; i < n; i++) owner[o + i] = (uint32_t)c;
+    }
+}
+
+// Marks the sampled fibers' nonzeros, one wavefront per block of
+// kExpandBlock nonzeros of one fiber: for row-order position p of each,
+// cpos[p] = c and marks[p] = 1 (plain byte stores: every position belongs
+// to one fiber, so no two writers meet).  Long fibers are thereby spread
+// over many wavefronts.
+__global__ __launch_bounds__(kThreads) void expand_hits(const uint32_t* __restrict__ fsel,
+                                                        const uint32_t* __restrict__ nblk,
+                                                        const uint32_t* __restrict__ boff, int64_t Jb,
+                                                        const uint32_t* __restrict__ owner,
+                                                        const uint32_t* __restrict__ fptr,
+                                                        const uint32_t* __restrict__ kpos, uint32_t* __restrict__ cpos,
+                                                        uint8_t* __restrict__ marks) {
+    constexpr int PER = kExpandBlock / 64;
+    const int lane = threadIdx.x % 64;
+    const int64_t B = (int64_t)boff[Jb - 1] + nblk[Jb - 1];
+    for (int64_t q = blockIdx.x * (int64_t)(kThreads / 64) + threadIdx.x / 64; q < B;
+         q += (int64_t)gridDim.x * (kThreads / 64)) {
+        const uint32_t c = owner[q];
+        const uint32_t f = fsel[c];
+        const int64_t k0 = (int64_t)fptr[f] + (q - (int64_t)boff[c]) * kExpandBlock;
+        const int64_t k1 = fptr[f + 1];
+        uint32_t pk[PER];
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            const int64_t k = k0 + 64 * t + lane;
+            pk[t] = k < k1 ? kpos[k] : ~0u;
+        }
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            if (pk[t] != ~0u) {
+                cpos[pk[t]] = (uint32_t)c;
+                marks[pk[t]] = 1;
+            }
+        }
+    }
+}
+
+constexpr int kBitThreads = 256;
+constexpr int kMarkVecs = 4;                        // 16-byte mark vectors per lane per chunk
+constexpr int kMarkChunk = 64 * kMarkVecs * 16;     // row-order positions per chunk (4096)
+
+// Exclusive prefix sum over the wavefront (lane order) and the total.
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t& total) {
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    total = readlane_u32(x, 63);
+    return x - v;
+}
+
+// Marks are bytes, 0 or 1, so a 32-bit word's bit count is its mark count.
+__device__ __forceinline__ uint32_t marks_in(const uint4& v) {
+    return __popc(v.x) + __popc(v.y) + __popc(v.z) + __popc(v.w);
+}
+
+// Hits per mark chunk (the mark array holds whole chunks, zero past nnz).
+__global__ __launch_bounds__(kBitThreads) void marks_count(const uint8_t* __restrict__ marks, int64_t nch,
+                                                           uint32_t* __restrict__ ccount) {
+    const int lane = threadIdx.x % 64;
+    for (int64_t w = (int64_t)blockIdx.x * (kBitThreads / 64) + threadIdx.x / 64; w < nch;
+         w += (int64_t)gridDim.x * (kBitThreads / 64)) {
+        const uint4* vp = reinterpret_cast<const uint4*>(marks + w * kMarkChunk) + kMarkVecs * lane;
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int q = 0; q < kMarkVecs; q++) cnt += marks_in(vp[q]);
+        uint32_t total;
+        (void)wave_excl_scan(cnt, lane, total);
+        if (lane == 0) ccount[w] = total;
+    }
+}
+
+// Lists the hit positions in row order: chunk w's hits go to
+// hits[coff[w] ...]; every marked vector read is cleared, so the marks are
+// zero again afterwards.  The last chunk's wavefront stores the hit count.
+__global__ __launch_bounds__(kBitThreads) void marks_compact(uint8_t* __restrict__ marks, int64_t nch,
+                                                             const uint32_t* __restrict__ coff,
+                                                             const uint32_t* __restrict__ ccount,
+                                                             uint32_t* __restrict__ hits,
+                                                             uint64_t* __restrict__ nhits) {
+    const int lane = threadIdx.x % 64;
+    for (int64_t w = (int64_t)blockIdx.x * (kBitThreads / 64) + threadIdx.x / 64; w < nch;
+         w += (int64_t)gridDim.x * (kBitThreads / 64)) {
+        uint4* vp = reinterpret_cast<uint4*>(marks + w * kMarkChunk) + kMarkVecs * lane;
+        uint4 v[kMarkVecs];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int q = 0; q < kMarkVecs; q++) {
+            v[q] = vp[q];
+            cnt += marks_in(v[q]);
+        }
+        uint32_t total;
+        uint32_t o = coff[w] + wave_excl_scan(cnt, lane, total);
+        if (cnt) {
+            const int64_t base = w * kMarkChunk + (int64_t)(kMarkVecs * lane) * 16;
+#pragma unroll
+            for (int q = 0; q < kMarkVecs; q++) {
+                const uint32_t wd[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
+                if (wd[0] | wd[1] | wd[2] | wd[3]) vp[q] = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                for (int d = 0; d < 4; d++) {
+                    uint32_t x = wd[d];
+                    while (x) {
+                        const int bit = __ffs(x) - 1;
+                        x &= x - 1;
+                        hits[o++] = (uint32_t)(base + 16 * q + 4 * d + (bit >> 3));
+                    }
+                }
+            }
+        }
+        if (w == nch - 1 && lane == 0) *nhits = (uint64_t)coff[w] + ccount[w];
+    }
+}
+
+// Row sums over the listed hits: one wavefront per unit of kHitUnit
+// consecutive hits (row order), as the full row pass does per chunk of
+// nonzeros, with the same two carry slots per unit.  The unit count comes
+// from the device hit count (nu is the host bound).
+constexpr int kHitUnit = 64;   // hits per wavefront unit of the hit pass
+
+template <typename V>
+__global__ __launch_bounds__(kBitThreads) void hit_scan_reduce(const uint32_t* __restrict__ hits,
+                                                               const uint64_t* __restrict__ nhits_dev, int64_t nu,
+                                                               const uint32_t* __restrict__ rrow,
+                                                               const uint32_t* __restrict__ cpos,
+                                                               const V* __restrict__ rval,
+                                                               const V* __restrict__ design, int R,
+                                                               V* __restrict__ Y, V* __restrict__ carry,
+                                                               uint32_t* __restrict__ cflag,
+                                                               uint32_t* __restrict__ ckey,
+                                                               unsigned long long* __restrict__ counts) {
+    constexpr int NT = kBitThreads;
+    constexpr int M = kHitUnit / 64;
+    const int lane = threadIdx.x % 64;
+    const int lr = lane < R ? lane : 0;
+    const int64_t H = (int64_t)*nhits_dev;
+    const int64_t nuh = (H + kHitUnit - 1) / kHitUnit;
+    const int64_t units = nuh < nu ? nuh : nu;
+    unsigned long long whits = 0, wheads = 0;
+    auto row_of = [design, R, lr](uint32_t c) -> V { return design[(int64_t)c * R + lr]; };
+    for (int64_t u = (int64_t)blockIdx.x * (NT / 64) + threadIdx.x / 64; u < units;
+         u += (int64_t)gridDim.x * (NT / 64)) {
+        const int64_t b = u * kHitUnit;
+        const int64_t e = b + kHitUnit < H ? b + kHitUnit : H;
+        bool hit[M];
+        uint32_t hr[M], hc[M];
+        V hx[M];
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int64_t i = b + 64 * m + lane;
+            hit[m] = i < e;
+            hr[m] = 0;
+            hc[m] = 0;
+            hx[m] = (V)0;
+            if (hit[m]) {
+                const uint32_t p = hits[i];
+                hr[m] = rrow[p];
+                hc[m] = cpos[p];
+                hx[m] = rval[p];
+            }
+        }
+        ChunkRows<V> st;
+        st.first_row = readlane_u32(hr[0], 0);
+        const int last = (int)(e - 1 - b);
+        st.last_row = st.first_row;
+#pragma unroll
+        for (int m = 0; m < M; m++)
+            if (last / 64 == m) st.last_row = readlane_u32(hr[m], last % 64);
+        st.first_cont = b > 0 && rrow[hits[b - 1]] == st.first_row;
+        st.continues = e < H && rrow[hits[e]] == st.last_row;
+        st.w = u;
+        st.lane = lane;
+        st.R = R;
+        st.Y = Y;
+        st.carry = carry;
+        uint32_t n = 0;
+        feed_hits<V, M, sizeof(V) == 8 ? 16 : 32>(st, hit, hr, hc, hx, row_of, n);
+        st.finish(cflag, ckey);
+        whits += n;
+        wheads += st.nheads;
+    }
+    add_counts<NT>(whits, wheads, counts);
 }
 
 // One level of the carry reduction: entries (key, R-vector, flag) sorted by
@@ -413,14 +813,30 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
 // Y; a crossing key leaves its partial in the next level's slot 0 (running
 // in from the left) or slot 1 (running out to the right), with the same
 // layout as the row pass.  With n <= kScanChunk every key is final.
+// Entries of carry level `level` (1 = the list the hit pass writes) when the
+// first list came from the device hit count H: 2 per unit of kHitUnit
+// hits, then 2 per chunk of the previous level while it spans more than one
+// chunk (a one-chunk level finishes every key).  n_host bounds it.
+__device__ __forceinline__ int64_t carry_count(const uint64_t* H_dev, int level, int64_t n_host) {
+    if (!H_dev) return n_host;
+    int64_t n = 2 * (((int64_t)*H_dev + kHitUnit - 1) / kHitUnit);
+    for (int l = 1; l < level; l++) {
+        const int64_t ch = (n + kScanChunk - 1) / kScanChunk;
+        n = ch > 1 ? 2 * ch : 0;
+    }
+    return n < n_host ? n : n_host;
+}
+
 template <typename V>
 __global__ __launch_bounds__(kThreads) void seg_level(const uint32_t* __restrict__ key, const V* __restrict__ vec,
-                                                      const uint32_t* __restrict__ flag, int64_t n, int R,
+                                                      const uint32_t* __restrict__ flag, int64_t n_host,
+                                                      const uint64_t* __restrict__ H_dev, int level, int R,
                                                       V* __restrict__ Y, V* __restrict__ out_vec,
                                                       uint32_t* __restrict__ out_flag, uint32_t* __restrict__ out_key) {
     const int lane = threadIdx.x % 64;
     const int lr = lane < R ? lane : 0;
     constexpr int M = kScanChunk / 64;
+    const int64_t n = carry_count(H_dev, level, n_host);
     const int64_t nch = (n + kScanChunk - 1) / kScanChunk;
     for (int64_t w = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64; w < nch;
          w += (int64_t)gridDim.x * (kThreads / 64)) {
@@ -449,11 +865,11 @@ __global__ __launch_bounds__(kThreads) void seg_level(const uint32_t* __restrict
         for (int m = 0; m < M; m++) {
             uint64_t mask = __ballot(fl[m]);
             while (mask) {
-                int ls[kHitBatch];
-                V dv[kHitBatch];
-                uint32_t rv[kHitBatch];
+                int ls[kSegBatch];
+                V dv[kSegBatch];
+                uint32_t rv[kSegBatch];
 #pragma unroll
-                for (int q = 0; q < kHitBatch; q++) {
+                for (int q = 0; q < kSegBatch; q++) {
                     ls[q] = -1;
                     if (mask) {
                         ls[q] = __ffsll((unsigned long long)mask) - 1;
@@ -461,7 +877,7 @@ __global__ __launch_bounds__(kThreads) void seg_level(const uint32_t* __restrict
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < kHitBatch; q++) {
+                for (int q = 0; q < kSegBatch; q++) {
                     dv[q] = (V)0;
                     rv[q] = 0;
                     if (ls[q] >= 0) {
@@ -470,7 +886,7 @@ __global__ __launch_bounds__(kThreads) void seg_level(const uint32_t* __restrict
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < kHitBatch; q++) {
+                for (int q = 0; q < kSegBatch; q++) {
                     if (ls[q] < 0) break;
                     const uint32_t r = rv[q];
                     if (!active || r != cur) {
@@ -614,6 +1030,8 @@ void build_lookup(hipStream_t s, const uint32_t* coords, const double* vals, int
         L.rrow.ensure(8);
         L.rfib.ensure(8);
         L.rval.ensure(8);
+        L.fptr.ensure(8);
+        L.kpos.ensure(8);
         L.filt.ensure((size_t)kFilterWords * 4);
         EX_HIP(hipMemsetAsync(L.filt.p, 0, (size_t)kFilterWords * 4, s));
         return;
@@ -654,6 +1072,13 @@ void build_lookup(hipStream_t s, const uint32_t* coords, const double* vals, int
     EX_HIP(hipMemsetAsync(L.filt.p, 0, (size_t)kFilterWords * 4, s));
     hipLaunchKernelGGL(fiber_scatter_keys, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, kout, nnz,
                        flags, pos, L.fkey.as<uint64_t>());
+    L.fptr.ensure((size_t)(nf + 1) * 4);
+    hipLaunchKernelGGL(fiber_starts, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, nnz, flags, pos,
+                       L.fptr.as<uint32_t>());
+    EX_HIP(hipMemsetAsync(cnt, 0, 8, s));
+    hipLaunchKernelGGL(max_fiber_len, dim3(grid_for(nf, kThreads, 4096)), dim3(kThreads), 0, s, L.fptr.as<uint32_t>(),
+                       nf, (unsigned int*)cnt);
+    L.maxfib = (int64_t)read_u64(s, cnt);
     hipLaunchKernelGGL(fiber_numbers, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, flags, pos, nnz, fib);
     // row order (stable: ties stay in fiber order)
     uint32_t* rows_k = flags;   // flags / pos are free now
@@ -666,6 +1091,9 @@ void build_lookup(hipStream_t s, const uint32_t* coords, const double* vals, int
     sort_pairs_u32_u32(s, tmp[2].p, tmp[2].bytes, rows_k, L.rrow.as<uint32_t>(), iota, perm, nnz, rbits);
     hipLaunchKernelGGL(gather_row_order, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, perm, fib, src,
                        vals, nnz, L.rfib.as<uint32_t>(), L.rval.as<double>());
+    L.kpos.ensure((size_t)nnz * 4);
+    hipLaunchKernelGGL(invert_perm, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, perm, nnz,
+                       L.kpos.as<uint32_t>());
     EX_HIP(hipGetLastError());
 }
 
@@ -717,12 +1145,82 @@ void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int
     EX_HIP(hipGetLastError());
 }
 
+// Carry lists of the row pass: level l has n_l = 2 * chunks(n_{l-1})
+// entries of (R-vector, flag, key), in tmp[2] and tmp[3] alternately.
+template <typename V>
+size_t carry_list_bytes(int64_t n, int R) {
+    return (size_t)n * R * sizeof(V) + (size_t)n * 8 + 512;
+}
+
+template <typename V>
+void carry_views(DevBuf& b, int64_t n, int R, V*& vec, uint32_t*& flag, uint32_t*& key) {
+    vec = b.as<V>();
+    flag = (uint32_t*)(vec + (int64_t)n * R);
+    key = flag + n;
+}
+
+// Reserves the first two levels for a row pass over nch chunks.
+template <typename V>
+void carry_reserve(DevBuf* tmp, int64_t nch, int R) {
+    const int64_t n1 = 2 * nch;
+    tmp[2].ensure(carry_list_bytes<V>(n1, R));
+    tmp[3].ensure(carry_list_bytes<V>(2 * ((n1 + kScanChunk - 1) / kScanChunk), R));
+}
+
+// Reduces the level-1 carry list (n1 entries in tmp[2]) level by level until
+// one chunk holds it; every crossing row's total lands in Y.
+template <typename V>
+void carry_reduce(hipStream_t s, DevBuf* tmp, int64_t n1, int R, V* Y, const uint64_t* H_dev = nullptr) {
+    V* vec;
+    uint32_t *flag, *key;
+    carry_views(tmp[2], n1, R, vec, flag, key);
+    int64_t n = n1;
+    int cur = 2;
+    int level = 1;
+    while (true) {
+        const int64_t ch = (n + kScanChunk - 1) / kScanChunk;
+        V* ovec = nullptr;
+        uint32_t *oflag = nullptr, *okey = nullptr;
+        if (ch > 1) carry_views(tmp[5 - cur], 2 * ch, R, ovec, oflag, okey);
+        hipLaunchKernelGGL(seg_level<V>, dim3(grid_for(ch, kThreads / 64, 65536)), dim3(kThreads), 0, s, key, vec,
+                           flag, n, H_dev, level, R, Y, ovec, oflag, okey);
+        if (ch == 1) break;
+        level++;
+        n = 2 * ch;
+        cur = 5 - cur;
+        carry_views(tmp[cur], n, R, vec, flag, key);
+        const size_t need = carry_list_bytes<V>(2 * ((n + kScanChunk - 1) / kScanChunk), R);
+        if (tmp[5 - cur].bytes < need) tmp[5 - cur].ensure(need);
+    }
+}
+
+// Per-workgroup counter partials of a pass over `blocks` workgroups (null
+// when no counters are wanted).
+unsigned long long* count_parts(ModeLookup& L, const uint64_t* counts, unsigned blocks) {
+    if (!counts) return nullptr;
+    L.cpart.ensure((size_t)blocks * 16);
+    return L.cpart.as<unsigned long long>();
+}
+
+// Hit list or full row pass.  The hit list costs one mark byte per
+// nonzero plus a few scattered accesses per hit; the full pass reads every
+// nonzero's row and fiber number and handles the hits inside that pass.
+// The hits are Ju times the mean length of the sampled fibers; leverage
+// sampling favours long fibers (on a 5e7-nonzero Zipf(1.1) tensor the hits
+// were about 50x Ju times the mean fiber length), so the estimate carries
+// that factor, and the hit list is taken when it stays under nnz / 16.
+bool use_hit_list(const ModeLookup& L, int64_t Ju) {
+    if (L.mttkrp_path == 1) return false;
+    if (L.mttkrp_path == 2) return true;
+    const double mean_len = L.nfib > 0 ? (double)L.nnz / (double)L.nfib : 1.0;
+    return (double)Ju * mean_len * 64.0 * 16.0 < (double)L.nnz;
+}
+
 template <typename V>
 void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju, const V* design,
                     int R, V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts, hipEvent_t* scan_events,
                     const uint64_t* Ju_dev) {
     EX_HIP(hipMemsetAsync(Y, 0, (size_t)I_j * R * sizeof(V), s));
-    if (counts) EX_HIP(hipMemsetAsync(counts, 0, 16, s));
     if (scan_events) EX_HIP(hipEventRecord(scan_events[0], s));
     if (Ju > 0 && L.nfib > 0 && L.nnz > 0) {
         const int64_t nch = (L.nnz + kScanChunk - 1) / kScanChunk;
@@ -732,53 +1230,124 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
             EX_HIP(hipMemsetAsync(L.cmap.p, 0, (size_t)L.nfib * 8, s));
             L.stamp = 1;
         }
+        if (use_hit_list(L, Ju)) {
+            const int64_t nchb = (L.nnz + kMarkChunk - 1) / kMarkChunk;
+            if (!L.marks.p) {
+                L.marks.ensure((size_t)nchb * kMarkChunk);
+                EX_HIP(hipMemsetAsync(L.marks.p, 0, (size_t)nchb * kMarkChunk, s));
+                L.cpos.ensure((size_t)L.nnz * 4);
+            }
+            // hits <= every nonzero of Ju fibers of at most maxfib nonzeros
+            const double hb = (double)Ju * (double)L.maxfib;
+            const int64_t Hb = hb < (double)L.nnz ? (int64_t)hb : L.nnz;
+            const int64_t nu = (Hb + kHitUnit - 1) / kHitUnit;
+            // expansion blocks: ceil(len / kExpandBlock) <= 1 + len / kExpandBlock per sample
+            const int64_t Bb = Ju + Hb / kExpandBlock + 1;
+            const size_t sb1 = exclusive_scan_u32(s, nullptr, 0, nullptr, nullptr, nchb);
+            const size_t sb2 = exclusive_scan_u32(s, nullptr, 0, nullptr, nullptr, Ju);
+            const size_t scan_b = sb1 > sb2 ? sb1 : sb2;
+            size_t o = 0;
+            auto take = [&](size_t bytes) { size_t r = o; o = (o + bytes + 255) & ~(size_t)255; return r; };
+            const size_t o_hits = take((size_t)Hb * 4), o_cc = take((size_t)nchb * 4), o_co = take((size_t)nchb * 4),
+                         o_n = take(8), o_fs = take((size_t)Ju * 4), o_nb = take((size_t)Ju * 4),
+                         o_bo = take((size_t)Ju * 4), o_ow = take((size_t)Bb * 4), o_t = take(scan_b);
+            tmp[1].ensure(o);
+            char* tb = tmp[1].as<char>();
+            uint32_t* hitl = (uint32_t*)(tb + o_hits);
+            uint32_t* ccount = (uint32_t*)(tb + o_cc);
+            uint32_t* coff = (uint32_t*)(tb + o_co);
+            uint64_t* nh = (uint64_t*)(tb + o_n);
+            uint32_t* fsel = (uint32_t*)(tb + o_fs);
+            uint32_t* nblk = (uint32_t*)(tb + o_nb);
+            uint32_t* boff = (uint32_t*)(tb + o_bo);
+            uint32_t* owner = (uint32_t*)(tb + o_ow);
+            carry_reserve<V>(tmp, nu, R);
+            V* vec;
+            uint32_t *flag, *key;
+            carry_views(tmp[2], 2 * nu, R, vec, flag, key);
+            // fibers of the samples, expansion blocks, marks
+            hipLaunchKernelGGL(find_fibers, dim3(grid_for(Ju, kThreads / 64, 16384)), dim3(kThreads), 0, s, kd, Ju,
+                               Ju_dev, L.fkey.as<uint64_t>(), L.nfib, L.fptr.as<uint32_t>(), fsel, nblk);
+            exclusive_scan_u32(s, tb + o_t, scan_b, nblk, boff, Ju);
+            hipLaunchKernelGGL(block_owners, dim3(grid_for(Ju, kThreads, 4096)), dim3(kThreads), 0, s, nblk, boff, Ju,
+                               owner);
+            hipLaunchKernelGGL(expand_hits, dim3(grid_for(Bb, kThreads / 64, 8 * num_cus(s))), dim3(kThreads), 0, s,
+                               fsel, nblk, boff, Ju, owner, L.fptr.as<uint32_t>(), L.kpos.as<uint32_t>(),
+                               L.cpos.as<uint32_t>(), L.marks.as<uint8_t>());
+            const dim3 bgrid(grid_for(nchb, kBitThreads / 64, 8 * num_cus(s)));
+            hipLaunchKernelGGL(marks_count, bgrid, dim3(kBitThreads), 0, s, L.marks.as<uint8_t>(), nchb, ccount);
+            exclusive_scan_u32(s, tb + o_t, scan_b, ccount, coff, nchb);
+            hipLaunchKernelGGL(marks_compact, bgrid, dim3(kBitThreads), 0, s, L.marks.as<uint8_t>(), nchb, coff,
+                               ccount, hitl, nh);
+            const dim3 hgrid(grid_for(nu, kBitThreads / 64, 8 * num_cus(s)));
+            unsigned long long* part = count_parts(L, counts, hgrid.x);
+            hipLaunchKernelGGL(hit_scan_reduce<V>, hgrid, dim3(kBitThreads), 0, s, hitl, nh, nu,
+                               L.rrow.as<uint32_t>(), L.cpos.as<uint32_t>(), vals, design, R, Y, vec, flag, key, part);
+            if (counts) hipLaunchKernelGGL(sum_counts, dim3(1), dim3(kThreads), 0, s, part, (int)hgrid.x, counts);
+            carry_reduce<V>(s, tmp, 2 * nu, R, Y, nh);
+            if (scan_events) EX_HIP(hipEventRecord(scan_events[1], s));
+            EX_HIP(hipGetLastError());
+            return;
+        }
         const uint32_t stamp = L.stamp;
         tmp[1].ensure((size_t)Ju * 4 + 256);
         uint32_t* fsel = tmp[1].as<uint32_t>();
-        // carry lists: level l has n_l = 2 * chunks(n_{l-1}) entries; two
-        // alternating buffers of (vec, flag, key)
-        auto list_bytes = [&](int64_t n) { return (size_t)n * R * sizeof(V) + (size_t)n * 8 + 512; };
-        const int64_t n1 = 2 * nch;
-        tmp[2].ensure(list_bytes(n1));
-        tmp[3].ensure(list_bytes(2 * ((n1 + kScanChunk - 1) / kScanChunk)));
-        auto views = [&](DevBuf& b, int64_t n, V*& vec, uint32_t*& flag, uint32_t*& key) {
-            vec = b.as<V>();
-            flag = (uint32_t*)(vec + n * R);
-            key = flag + n;
-        };
+        carry_reserve<V>(tmp, nch, R);
         V* vec;
         uint32_t *flag, *key;
-        views(tmp[2], n1, vec, flag, key);
+        carry_views(tmp[2], 2 * nch, R, vec, flag, key);
         hipLaunchKernelGGL(mark_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, kd, Ju, Ju_dev,
                            L.fkey.as<uint64_t>(), L.nfib, stamp, L.cmap.as<uint64_t>(),
                            L.filt.as<uint32_t>(), fsel);
         const dim3 scan_grid(grid_for(nch, kScanThreads / 64, 2 * num_cus(s)));
-        hipLaunchKernelGGL(row_scan_reduce<V>, scan_grid, dim3(kScanThreads), 0, s, L.rrow.as<uint32_t>(), L.rfib.as<uint32_t>(), vals, L.nnz, nch,
-                           stamp, L.cmap.as<uint64_t>(), L.filt.as<uint32_t>(), design, R, Y, vec,
-                           flag, key, (unsigned long long*)counts);
-        // reduce the carry list level by level until one chunk holds it
-        int64_t n = n1;
-        int cur = 2;
-        while (true) {
-            const int64_t ch = (n + kScanChunk - 1) / kScanChunk;
-            V* ovec = nullptr;
-            uint32_t *oflag = nullptr, *okey = nullptr;
-            if (ch > 1) views(tmp[5 - cur], 2 * ch, ovec, oflag, okey);
-            hipLaunchKernelGGL(seg_level<V>, dim3(grid_for(ch, kThreads / 64, 65536)), dim3(kThreads), 0, s, key, vec,
-                               flag, n, R, Y, ovec, oflag, okey);
-            if (ch == 1) break;
-            n = 2 * ch;
-            cur = 5 - cur;
-            views(tmp[cur], n, vec, flag, key);
-            if (tmp[5 - cur].bytes < list_bytes(2 * ((n + kScanChunk - 1) / kScanChunk)))
-                tmp[5 - cur].ensure(list_bytes(2 * ((n + kScanChunk - 1) / kScanChunk)));
-        }
+        unsigned long long* part = count_parts(L, counts, scan_grid.x);
+        hipLaunchKernelGGL((row_scan_reduce<V, false>), scan_grid, dim3(kScanThreads), 0, s, L.rrow.as<uint32_t>(),
+                           L.rfib.as<uint32_t>(), vals, L.nnz, nch, stamp, L.cmap.as<uint64_t>(),
+                           L.filt.as<uint32_t>(), design, ExactSrc<V>{}, R, Y, vec, flag, key, part);
+        if (counts) hipLaunchKernelGGL(sum_counts, dim3(1), dim3(kThreads), 0, s, part, (int)scan_grid.x, counts);
+        carry_reduce<V>(s, tmp, 2 * nch, R, Y);
         hipLaunchKernelGGL(clear_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, fsel, Ju, Ju_dev,
                            L.filt.as<uint32_t>());
     }
     if (scan_events) EX_HIP(hipEventRecord(scan_events[1], s));
     EX_HIP(hipGetLastError());
 }
+
+template <typename V>
+void exact_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const V* const* U, const int64_t* dims, int R,
+                  V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts) {
+    EX_HIP(hipMemsetAsync(Y, 0, (size_t)I_j * R * sizeof(V), s));
+    if (L.nnz > 0) {
+        const int64_t nch = (L.nnz + kScanChunk - 1) / kScanChunk;
+        ExactSrc<V> ex{};
+        ex.fkey = L.fkey.as<uint64_t>();
+        ex.N = L.kl.N;
+        for (int k = 0; k < L.kl.N; k++) {
+            ex.U[k] = U[k];
+            ex.shift[k] = L.kl.shift[k];
+            int b = 1;
+            while (((int64_t)1 << b) < dims[k]) b++;
+            ex.mask[k] = b >= 64 ? ~(uint64_t)0 : (((uint64_t)1 << b) - 1);
+        }
+        carry_reserve<V>(tmp, nch, R);
+        V* vec;
+        uint32_t *flag, *key;
+        carry_views(tmp[2], 2 * nch, R, vec, flag, key);
+        const dim3 scan_grid(grid_for(nch, kScanThreads / 64, 2 * num_cus(s)));
+        unsigned long long* part = count_parts(L, counts, scan_grid.x);
+        hipLaunchKernelGGL((row_scan_reduce<V, true>), scan_grid, dim3(kScanThreads), 0, s, L.rrow.as<uint32_t>(),
+                           L.rfib.as<uint32_t>(), vals, L.nnz, nch, 0u, nullptr, nullptr, nullptr, ex, R, Y, vec,
+                           flag, key, part);
+        if (counts) hipLaunchKernelGGL(sum_counts, dim3(1), dim3(kThreads), 0, s, part, (int)scan_grid.x, counts);
+        carry_reduce<V>(s, tmp, 2 * nch, R, Y);
+    }
+    EX_HIP(hipGetLastError());
+}
+
+template void exact_mttkrp<double>(hipStream_t, ModeLookup&, const double*, const double* const*,
+                                   const int64_t*, int, double*, int64_t, DevBuf*, uint64_t*);
+template void exact_mttkrp<float>(hipStream_t, ModeLookup&, const float*, const float* const*, const int64_t*,
+                                  int, float*, int64_t, DevBuf*, uint64_t*);
 
 template void sampled_mttkrp<double>(hipStream_t, ModeLookup&, const double*, const uint64_t*, int64_t, const double*,
                                      int, double*, int64_t, DevBuf*, uint64_t*, hipEvent_t*, const uint64_t*);

@@ -75,13 +75,18 @@ def test_dedup(ctx, orc):
         assert np.array_equal(wd_g, wd_o)
 
 
+PATHS = {"auto": 0, "row_pass": 1, "hit_list": 2}
+
+
+@pytest.mark.parametrize("path", list(PATHS))
 @pytest.mark.parametrize("R", [8, 32])
-def test_sampled_mttkrp_fixed_samples(ctx, orc, R):
+def test_sampled_mttkrp_fixed_samples(ctx, orc, R, path):
     from rdist_tensor_amd import abi
     rng = np.random.default_rng(11 + R)
     dims = [300, 200, 100]
     idx, vals = random_tensor(rng, dims, 40000)
     T = abi.Tensor(ctx, idx, vals, dims)
+    T.mttkrp_path(PATHS[path])
     T.build_lookups()
     for mode in range(3):
         lk = orc.Lookup(idx, vals, mode, dims)
@@ -96,6 +101,42 @@ def test_sampled_mttkrp_fixed_samples(ctx, orc, R):
         Yg, Hg = T.sampled_mttkrp(mode, sd, design)
         assert Hg == Ho
         assert rel_err(Yg, Yo) < REL
+
+
+@pytest.mark.parametrize("path", ["row_pass", "hit_list"])
+def test_sampled_mttkrp_long_rows(ctx, orc, path):
+    """Rows far longer than one chunk of either pass (multi-level carries),
+    chunks holding more hits than one staging round, empty rows."""
+    from rdist_tensor_amd import abi
+    from test_gpu_exact import skewed_tensor
+    rng = np.random.default_rng(12)
+    dims = [20000, 300, 200]
+    R = 16
+    idx, vals = skewed_tensor(rng, dims, 300000)
+    T = abi.Tensor(ctx, idx, vals, dims)
+    T.mttkrp_path(PATHS[path])
+    T.build_lookups()
+    rows0 = np.bincount(idx[:, 0], minlength=dims[0])
+    print(f"longest mode-0 row {rows0.max()}, empty rows {(rows0 == 0).sum()}", flush=True)
+    for mode in range(3):
+        lk = orc.Lookup(idx, vals, mode, dims)
+        # most samples from the nonzeros of the heaviest rows, some random
+        heavy = np.flatnonzero(idx[:, 0] < 4)
+        pick = np.concatenate([rng.choice(heavy, 6000), rng.integers(0, len(idx), 3000)])
+        s = idx[pick].copy()
+        sd, _ = orc.dedup(s, np.ones(len(s)), mode, dims)
+        design = rng.standard_normal((len(sd), R))
+        Yo, Ho = lk.sampled_mttkrp(sd, design)
+        Yg, Hg = T.sampled_mttkrp(mode, sd, design)
+        print(f"{path} mode {mode}: Ju {len(sd)} hits {Ho}", flush=True)
+        assert Hg == Ho
+        assert rel_err(Yg, Yo) < REL
+        # a second call with other samples: the marks were left clean
+        sd2, _ = orc.dedup(idx[rng.integers(0, len(idx), 500)], np.ones(500), mode, dims)
+        d2 = rng.standard_normal((len(sd2), R))
+        Yo2, Ho2 = lk.sampled_mttkrp(sd2, d2)
+        Yg2, Hg2 = T.sampled_mttkrp(mode, sd2, d2)
+        assert Hg2 == Ho2 and rel_err(Yg2, Yo2) < REL
 
 
 def test_arls_scores_and_draw(ctx, orc):
@@ -178,8 +219,9 @@ def test_sts_draw_bitexact(ctx, orc, R):
         assert rel_err(wg[ok], wo[ok]) < 1e-11
 
 
+@pytest.mark.parametrize("path", ["auto", "hit_list"])
 @pytest.mark.parametrize("sampler", ["arls", "sts"])
-def test_als_step_and_run_match_oracle(ctx, orc, sampler):
+def test_als_step_and_run_match_oracle(ctx, orc, sampler, path):
     from rdist_tensor_amd import abi
     rng = np.random.default_rng(51)
     dims = [400, 300, 250]
@@ -188,6 +230,7 @@ def test_als_step_and_run_match_oracle(ctx, orc, sampler):
     U = normalized_factors(rng, dims, R)
     kind = abi.EX_SAMPLER_STS_CP if sampler == "sts" else abi.EX_SAMPLER_CP_ARLS_LEV
     T = abi.Tensor(ctx, idx, vals, dims)
+    T.mttkrp_path(PATHS[path])
     F = abi.Factors(ctx, dims, R)
     for k in range(3):
         F.set(k, U[k])
@@ -205,3 +248,33 @@ def test_als_step_and_run_match_oracle(ctx, orc, sampler):
             print(f"{sampler} sweep {sweep} mode {j}: factor rel err {err:.3e}")
             assert err < 1e-9
     assert abs(A.fit() - oa.fit()) < 1e-9
+
+
+@pytest.mark.parametrize("path", ["row_pass", "hit_list"])
+@pytest.mark.parametrize("sampler", ["arls", "sts"])
+def test_als_run_statistics_match_oracle(ctx, orc, sampler, path):
+    """execute_ALS_rounds: fits per epoch and the nonzeros iterated summed
+    over every step (als_optimizer.hpp:89-106) equal the oracle's."""
+    import json
+    from rdist_tensor_amd import abi
+    rng = np.random.default_rng(52)
+    dims = [500, 300, 200]
+    R = 8
+    idx, vals = random_tensor(rng, dims, 40000)
+    U = normalized_factors(rng, dims, R)
+    kind = abi.EX_SAMPLER_STS_CP if sampler == "sts" else abi.EX_SAMPLER_CP_ARLS_LEV
+    T = abi.Tensor(ctx, idx, vals, dims)
+    T.mttkrp_path(PATHS[path])
+    F = abi.Factors(ctx, dims, R)
+    for k in range(3):
+        F.set(k, U[k])
+    S = abi.Sampler(ctx, F, kind, seed=9)
+    A = abi.ALS(ctx, T, F, S)
+    A.init()
+    A.run(3, 2048, 1)
+    st = json.loads(A.stats_json())
+    oa = orc.ALS(idx, vals, dims, U, sampler, seed=9)
+    fits, ost = oa.run(3, 2048, 1)
+    print(f"{sampler}/{path}: nonzeros {st['sum_nonzeros_iterated']} vs {ost['nonzeros_iterated']}")
+    assert st["sum_nonzeros_iterated"] == ost["nonzeros_iterated"]
+    np.testing.assert_allclose(st["fits"][1:], fits, atol=1e-9)

@@ -396,3 +396,48 @@ def test_als_recovers_planted_low_rank(orc):
         fits, st = a.run(10, 400, 5)
         assert fits[-1] > 0.9999, (sampler, fits)
         assert st["nonzeros_iterated"] > 0
+
+
+def test_exact_als_step_is_least_squares_optimum(orc):
+    """ExactALS step (exact_als.hpp:95-141): U_j sigma is the least-squares
+    solution of X_(j) ~ U_j (KRP of the other factors)^T, here solved by
+    numpy's SVD-based lstsq on the dense unfolding; columns unit norm."""
+    rng = np.random.default_rng(35)
+    dims = [9, 8, 7]
+    R = 3
+    idx, vals = random_tensor(rng, dims, 200)
+    dense = np.zeros(dims)
+    dense[tuple(idx.T.astype(np.int64))] = vals
+    U0 = normalized_factors(rng, dims, R)
+    a = orc.ALS(idx, vals, dims, U0, "exact")
+    for j in range(3):
+        others = [k for k in range(3) if k != j]
+        K = np.einsum("ar,br->abr", a.U[others[0]], a.U[others[1]]).reshape(-1, R)
+        Xj = np.moveaxis(dense, j, 0).reshape(dims[j], -1)
+        sol = np.linalg.lstsq(K, Xj.T, rcond=None)[0].T   # I_j x R
+        a.step(j, 0, 1)
+        nrm = np.linalg.norm(sol, axis=0)
+        np.testing.assert_allclose(a.sigma, nrm, rtol=1e-10)
+        np.testing.assert_allclose(a.U[j], sol / nrm, rtol=1e-9, atol=1e-11)
+
+
+def test_exact_als_fit_is_monotone_and_recovers_planted(orc):
+    rng = np.random.default_rng(36)
+    dims = [12, 11, 10]
+    R = 2
+    A = [rng.standard_normal((d, R)) for d in dims]
+    dense = np.einsum("ar,br,cr->abc", *A)
+    idx = np.argwhere(np.ones(dims)).astype(np.uint32)
+    vals = dense[tuple(idx.T.astype(np.int64))]
+    U0 = normalized_factors(rng, dims, R)
+    a = orc.ALS(idx, vals, dims, U0, "exact")
+    # 15 sweeps: later the residual^2 rounds below zero (unfloored fit, NaN)
+    fits, st = a.run(15, 0, 1)
+    assert np.all(np.diff(fits) > -1e-10), fits
+    assert fits[-1] > 0.9999, fits
+    assert st["nonzeros_iterated"] == 0   # ExactALS does not count them
+    # sparse tensor: exact ALS is monotone as well
+    idx, vals = random_tensor(rng, [30, 20, 25], 900)
+    a = orc.ALS(idx, vals, [30, 20, 25], normalized_factors(rng, [30, 20, 25], 4), "exact")
+    fits, _ = a.run(6, 0, 1)
+    assert np.all(np.diff(fits) > -1e-10), fits
