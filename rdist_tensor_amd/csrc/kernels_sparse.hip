@@ -5,6 +5,7 @@
 // Reference semantics: sort_lookup.hpp:37-102 (SortIdxLookup), :238-375
 // (csr_based_spmm), :377-438 (compute_2bmb); als_optimizer.hpp:130-206
 // (deduplicate_design_matrix); accumulator_stationary.hpp:186-249 (design).
+#include <type_traits>
 #include "common.hpp"
 #include "device.hpp"
 
@@ -282,62 +283,139 @@ struct ExactSrc {
     int N;
 };
 
-// Per-chunk row sums of a wavefront (lane r < R holds column r).  Hits are
-// fed in row order; a row that begins and ends inside the chunk is written
-// to Y, a row crossing a chunk boundary leaves its partial in carry slot 0
+// Per-chunk row sums of a wavefront, several hits at a time.  The wavefront
+// is split into S = 64 / G groups of G lanes; in one step group g takes hit
+// k0 + g and lane `sub` of it columns sub*VW .. sub*VW + VW - 1.  Hits come
+// in row order.  A segmented scan across the groups (rows equal => same
+// segment, since rows are sorted) sums each row's contributions inside the
+// step; rows that end inside the step are written at once, the open row
+// (the step's last) is carried into the next step in `acc` (replicated in
+// every group).  A row that begins and ends inside the chunk is written to
+// Y; a row crossing a chunk boundary leaves its partial in carry slot 0
 // (running in from the left) or slot 1 (running out to the right).
-template <typename V>
-struct ChunkRows {
+template <typename V, int G, int VW>
+struct GroupRows {
+    static constexpr int S = 64 / G;
     uint32_t first_row, last_row;   // rows at the chunk's first and last positions
     bool first_cont, continues;     // first row began before / last row runs past the chunk
     int64_t w;
-    int lane, R;
+    int lane, R, g, sub;
     V* Y;
     V* carry;
-    uint32_t cur = 0, nheads = 0;
-    bool active = false, cur_head = false;
-    V acc = (V)0;
-    uint32_t f0 = 0, f1 = 0;
+    uint32_t cur = 0, nheads = 0, f0 = 0, f1 = 0;
+    bool active = false;
+    V acc[VW];
 
-    __device__ void flush() {
-        if (cur_head) {
-            if (lane < R) Y[(int64_t)cur * R + lane] = acc;
-            nheads++;
-        } else {
-            if (lane < R) carry[(2 * w) * R + lane] = acc;
-            f0 = 1;
+    __device__ void init(int lane_, int R_, int64_t w_, V* Y_, V* carry_) {
+        lane = lane_;
+        R = R_;
+        w = w_;
+        Y = Y_;
+        carry = carry_;
+        g = lane / G;
+        sub = lane % G;
+#pragma unroll
+        for (int v = 0; v < VW; v++) acc[v] = (V)0;
+    }
+    __device__ bool head_row(uint32_t r) const { return !(first_cont && r == first_row); }
+    // the calling group's lanes write row r (Y, or carry slot 0 for the
+    // row running in from the left)
+    __device__ void write_row(uint32_t r, const V (&x)[VW]) {
+        V* dst = head_row(r) ? Y + (int64_t)r * R : carry + (2 * w) * R;
+#pragma unroll
+        for (int v = 0; v < VW; v++) {
+            const int col = sub * VW + v;
+            if (col < R) dst[col] = x[v];
         }
     }
-    __device__ void add(uint32_t r, V d, V x) {
-        if (!active || r != cur) {
-            if (active) flush();
-            cur = r;
-            acc = (V)0;
-            active = true;
-            cur_head = !(first_cont && r == first_row);
+    // one step over hits k0 .. k0 + n - 1 (1 <= n <= S); meta(k, r, c, x)
+    // fetches hit k, row_of(c, col0, d) its design values at columns
+    // col0 .. col0 + VW - 1 (zero at columns >= R)
+    template <typename Meta, typename RowOf>
+    __device__ void step(int64_t k0, int n, Meta meta, RowOf row_of) {
+        const bool valid = g < n;
+        uint32_t r = 0xFFFFFFFFu;
+        V p[VW];
+#pragma unroll
+        for (int v = 0; v < VW; v++) p[v] = (V)0;
+        if (valid) {
+            uint32_t c;
+            V x;
+            meta(k0 + g, r, c, x);
+            V d[VW];
+            row_of(c, sub * VW, d);
+#pragma unroll
+            for (int v = 0; v < VW; v++) p[v] = d[v] * x;
         }
-        acc += d * x;
+        // segmented inclusive scan across the groups
+#pragma unroll
+        for (int d = 1; d < S; d <<= 1) {
+            const uint32_t ro = __shfl_up(r, d * G, 64);
+            V po[VW];
+#pragma unroll
+            for (int v = 0; v < VW; v++) po[v] = __shfl_up(p[v], d * G, 64);
+            if (g >= d && ro == r) {
+#pragma unroll
+                for (int v = 0; v < VW; v++) p[v] += po[v];
+            }
+        }
+        const int L = n - 1;
+        const uint32_t r0 = readlane_u32(r, 0);
+        const bool cont = active && r0 == cur;
+        if (active && !cont) {
+            // the running row ended before this step
+            if (g == 0) write_row(cur, acc);
+            if (head_row(cur)) nheads++;
+            else f0 = 1;
+        }
+        if (cont && r == r0) {
+#pragma unroll
+            for (int v = 0; v < VW; v++) p[v] += acc[v];
+        }
+        const uint32_t rn = __shfl_down(r, G, 64);
+        const bool end = valid && g < L && rn != r;
+        if (end) write_row(r, p);
+        const uint64_t em = __ballot(end && sub == 0);
+        const uint64_t hm = __ballot(end && sub == 0 && head_row(r));
+        nheads += (uint32_t)__popcll(hm);
+        if (em != hm) f0 = 1;
+        // the step's last row stays open
+        cur = readlane_u32(r, L * G);
+#pragma unroll
+        for (int v = 0; v < VW; v++) acc[v] = __shfl(p[v], L * G + sub, 64);
+        active = true;
     }
     __device__ void finish(uint32_t* cflag, uint32_t* ckey) {
-        const bool last_head = !(first_cont && last_row == first_row);
+        const bool last_head = head_row(last_row);
         if (active && cur != last_row) {
             // the last hit row ended inside the chunk
-            flush();
+            if (g == 0) write_row(cur, acc);
+            if (head_row(cur)) nheads++;
+            else f0 = 1;
             active = false;
-            acc = (V)0;
+#pragma unroll
+            for (int v = 0; v < VW; v++) acc[v] = (V)0;
         }
         if (continues) {
-            const V part = active ? acc : (V)0;   // active implies cur == last_row here
+            // active implies cur == last_row here; acc is zero otherwise
+            V* dst = carry + (2 * w + (last_head ? 1 : 0)) * R;
+            if (g == 0) {
+#pragma unroll
+                for (int v = 0; v < VW; v++) {
+                    const int col = sub * VW + v;
+                    if (col < R) dst[col] = acc[v];
+                }
+            }
             if (last_head) {
-                if (lane < R) carry[(2 * w + 1) * R + lane] = part;
                 f1 = 1;
                 if (active) nheads++;
             } else {
-                if (lane < R) carry[(2 * w) * R + lane] = part;
                 f0 = 1;
             }
         } else if (active) {
-            flush();
+            if (g == 0) write_row(cur, acc);
+            if (head_row(cur)) nheads++;
+            else f0 = 1;
         }
         if (lane == 0) {
             cflag[2 * w] = f0;
@@ -348,48 +426,9 @@ struct ChunkRows {
     }
 };
 
-// Feeds the hits held across the wavefront (slot m of lane l = hit 64m + l,
-// in row order) to `st`: up to BATCH hits have their rows (row_of(c), this
-// lane's column) loaded together, then summed in order.
-template <typename V, int M, int BATCH = kHitBatch, typename RowOf>
-__device__ __forceinline__ void feed_hits(ChunkRows<V>& st, const bool (&hit)[M], const uint32_t (&hr)[M],
-                                          const uint32_t (&hc)[M], const V (&hx)[M], RowOf row_of,
-                                          uint32_t& nhits) {
-#pragma unroll
-    for (int m = 0; m < M; m++) {
-        uint64_t mask = __ballot(hit[m]);
-        nhits += (uint32_t)__popcll(mask);
-        while (mask) {
-            int ls[BATCH];
-            V dv[BATCH], xv[BATCH];
-            uint32_t rv[BATCH];
-#pragma unroll
-            for (int q = 0; q < BATCH; q++) {
-                ls[q] = -1;
-                if (mask) {
-                    ls[q] = __ffsll((unsigned long long)mask) - 1;
-                    mask &= mask - 1;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < BATCH; q++) {
-                dv[q] = (V)0;
-                xv[q] = (V)0;
-                rv[q] = 0;
-                if (ls[q] >= 0) {
-                    dv[q] = row_of(readlane_u32(hc[m], ls[q]));
-                    xv[q] = readlane_val(hx[m], ls[q]);
-                    rv[q] = readlane_u32(hr[m], ls[q]);
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < BATCH; q++) {
-                if (ls[q] < 0) break;
-                st.add(rv[q], dv[q], xv[q]);
-            }
-        }
-    }
-}
+// Columns per lane: four FP32 or two FP64 values.
+template <typename V>
+constexpr int group_vw() { return sizeof(V) == 4 ? 4 : 2; }
 
 // Sums each wavefront's hit and head counts per workgroup into
 // part[2 * blockIdx.x + {0, 1}] (no atomics: many workgroups adding to one
@@ -437,10 +476,23 @@ __global__ __launch_bounds__(kThreads) void sum_counts(const unsigned long long*
     }
 }
 
+// Lane's mask bits below it.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Orders LDS accesses of the lanes of one wavefront around a hand-off.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // kExact = false: sampled MTTKRP (hits are the nonzeros of sampled fibers,
 // rows from `design`).  kExact = true: exact MTTKRP from `ex`; the filter,
-// map and design arguments are unused.
-template <typename V, bool kExact>
+// map and design arguments are unused.  Each chunk's hits are listed in LDS
+// in order, then summed G lanes per hit (GroupRows).
+template <typename V, bool kExact, int G>
 __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* __restrict__ rrow,
                                                             const uint32_t* __restrict__ rfib,
                                                             const V* __restrict__ rval, int64_t nnz, int64_t nch,
@@ -451,13 +503,16 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
                                                             uint32_t* __restrict__ cflag, uint32_t* __restrict__ ckey,
                                                             unsigned long long* __restrict__ counts) {
     constexpr int NT = kScanThreads;
+    constexpr int VW = group_vw<V>();
+    constexpr int S = 64 / G;
     __shared__ uint32_t lfilt[kExact ? 1 : kFilterWords];
+    __shared__ uint32_t lrow[NT / 64][kScanChunk], lcol[NT / 64][kScanChunk];
+    __shared__ V lval[NT / 64][kScanChunk];
     if constexpr (!kExact) {
         for (int i = threadIdx.x; i < kFilterWords; i += NT) lfilt[i] = filt[i];
         __syncthreads();
     }
-    const int lane = threadIdx.x % 64;
-    const int lr = lane < R ? lane : 0;
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
     constexpr int M = kScanChunk / 64;
     unsigned long long whits = 0, wheads = 0;   // this wavefront's counts over its chunks
     const int64_t step = (int64_t)gridDim.x * (NT / 64);
@@ -475,21 +530,36 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
         nprev = b2 > 0 ? rrow[b2 - 1] : 0u;
         nnext = e2 < nnz ? rrow[e2] : 0u;
     };
-    auto row_of = [ex, design, R, lr](uint32_t c) -> V {
+    auto row_of = [ex, design, R](uint32_t c, int col0, V (&d)[VW]) {
         if constexpr (kExact) {
             // product of the fiber's factor rows in mode order
             const uint64_t key = ex.fkey[c];
-            V p = (V)1;
 #pragma unroll
-            for (int k = 0; k < kMaxModes; k++)
-                if (k < ex.N && ex.shift[k] >= 0)
-                    p *= ex.U[k][(int64_t)((key >> ex.shift[k]) & ex.mask[k]) * R + lr];
-            return p;
+            for (int v = 0; v < VW; v++) d[v] = (V)1;
+#pragma unroll
+            for (int k = 0; k < kMaxModes; k++) {
+                if (k < ex.N && ex.shift[k] >= 0) {
+                    const V* u = ex.U[k] + (int64_t)((key >> ex.shift[k]) & ex.mask[k]) * R;
+#pragma unroll
+                    for (int v = 0; v < VW; v++)
+                        if (col0 + v < R) d[v] *= u[col0 + v];
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < VW; v++)
+                if (col0 + v >= R) d[v] = (V)0;
         } else {
-            return design[(int64_t)c * R + lr];
+            const V* u = design + (int64_t)c * R;
+#pragma unroll
+            for (int v = 0; v < VW; v++) d[v] = col0 + v < R ? u[col0 + v] : (V)0;
         }
     };
-    int64_t w = (int64_t)blockIdx.x * (NT / 64) + threadIdx.x / 64;
+    auto meta = [&](int64_t k, uint32_t& r, uint32_t& c, V& x) {
+        r = lrow[wv][k];
+        c = lcol[wv][k];
+        x = lval[wv][k];
+    };
+    int64_t w = (int64_t)blockIdx.x * (NT / 64) + wv;
     if (w < nch) load_raw(w);
     for (; w < nch; w += step) {
         const int64_t b = w * kScanChunk, e = b + kScanChunk < nnz ? b + kScanChunk : nnz;
@@ -534,7 +604,8 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
                 hc[m] = (uint32_t)cm[m];
             }
         }
-        ChunkRows<V> st;
+        GroupRows<V, G, VW> st;
+        st.init(lane, R, w, Y, carry);
         st.first_row = readlane_u32(hr[0], 0);
         const int last = (int)(e - 1 - b);
         st.last_row = st.first_row;
@@ -543,15 +614,24 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
             if (last / 64 == m) st.last_row = readlane_u32(hr[m], last % 64);
         st.first_cont = b > 0 && prev_row == st.first_row;
         st.continues = e < nnz && next_row == st.last_row;
-        st.w = w;
-        st.lane = lane;
-        st.R = R;
-        st.Y = Y;
-        st.carry = carry;
-        uint32_t nhits = 0;
-        feed_hits<V, M>(st, hit, hr, hc, hx, row_of, nhits);
+        // list the chunk's hits in order
+        uint32_t nh = 0;
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const uint64_t mask = __ballot(hit[m]);
+            if (hit[m]) {
+                const uint32_t k = nh + lanes_below(mask);
+                lrow[wv][k] = hr[m];
+                lcol[wv][k] = hc[m];
+                lval[wv][k] = hx[m];
+            }
+            nh += (uint32_t)__popcll(mask);
+        }
+        wave_lds_sync();
+        for (uint32_t k0 = 0; k0 < nh; k0 += S) st.step(k0, nh - k0 < (uint32_t)S ? (int)(nh - k0) : S, meta, row_of);
         st.finish(cflag, ckey);
-        whits += nhits;
+        wave_lds_sync();
+        whits += nh;
         wheads += st.nheads;
     }
     add_counts<NT>(whits, wheads, counts);
@@ -737,12 +817,12 @@ __global__ __launch_bounds__(kBitThreads) void marks_compact(uint8_t* __restrict
 }
 
 // Row sums over the listed hits: one wavefront per unit of kHitUnit
-// consecutive hits (row order), as the full row pass does per chunk of
-// nonzeros, with the same two carry slots per unit.  The unit count comes
-// from the device hit count (nu is the host bound).
-constexpr int kHitUnit = 64;   // hits per wavefront unit of the hit pass
+// consecutive hits (row order), G lanes per hit (GroupRows), with the same
+// two carry slots per unit as the full row pass.  The unit count comes from
+// the device hit count (nu is the host bound).
+constexpr int kHitUnit = 256;   // hits per wavefront unit of the hit pass
 
-template <typename V>
+template <typename V, int G>
 __global__ __launch_bounds__(kBitThreads) void hit_scan_reduce(const uint32_t* __restrict__ hits,
                                                                const uint64_t* __restrict__ nhits_dev, int64_t nu,
                                                                const uint32_t* __restrict__ rrow,
@@ -754,53 +834,37 @@ __global__ __launch_bounds__(kBitThreads) void hit_scan_reduce(const uint32_t* _
                                                                uint32_t* __restrict__ ckey,
                                                                unsigned long long* __restrict__ counts) {
     constexpr int NT = kBitThreads;
-    constexpr int M = kHitUnit / 64;
+    constexpr int VW = group_vw<V>();
+    constexpr int S = 64 / G;
     const int lane = threadIdx.x % 64;
-    const int lr = lane < R ? lane : 0;
     const int64_t H = (int64_t)*nhits_dev;
     const int64_t nuh = (H + kHitUnit - 1) / kHitUnit;
     const int64_t units = nuh < nu ? nuh : nu;
     unsigned long long whits = 0, wheads = 0;
-    auto row_of = [design, R, lr](uint32_t c) -> V { return design[(int64_t)c * R + lr]; };
+    auto row_of = [design, R](uint32_t c, int col0, V (&d)[VW]) {
+        const V* u = design + (int64_t)c * R;
+#pragma unroll
+        for (int v = 0; v < VW; v++) d[v] = col0 + v < R ? u[col0 + v] : (V)0;
+    };
+    auto meta = [&](int64_t k, uint32_t& r, uint32_t& c, V& x) {
+        const uint32_t p = hits[k];
+        r = rrow[p];
+        c = cpos[p];
+        x = rval[p];
+    };
     for (int64_t u = (int64_t)blockIdx.x * (NT / 64) + threadIdx.x / 64; u < units;
          u += (int64_t)gridDim.x * (NT / 64)) {
         const int64_t b = u * kHitUnit;
         const int64_t e = b + kHitUnit < H ? b + kHitUnit : H;
-        bool hit[M];
-        uint32_t hr[M], hc[M];
-        V hx[M];
-#pragma unroll
-        for (int m = 0; m < M; m++) {
-            const int64_t i = b + 64 * m + lane;
-            hit[m] = i < e;
-            hr[m] = 0;
-            hc[m] = 0;
-            hx[m] = (V)0;
-            if (hit[m]) {
-                const uint32_t p = hits[i];
-                hr[m] = rrow[p];
-                hc[m] = cpos[p];
-                hx[m] = rval[p];
-            }
-        }
-        ChunkRows<V> st;
-        st.first_row = readlane_u32(hr[0], 0);
-        const int last = (int)(e - 1 - b);
-        st.last_row = st.first_row;
-#pragma unroll
-        for (int m = 0; m < M; m++)
-            if (last / 64 == m) st.last_row = readlane_u32(hr[m], last % 64);
+        GroupRows<V, G, VW> st;
+        st.init(lane, R, u, Y, carry);
+        st.first_row = rrow[hits[b]];
+        st.last_row = rrow[hits[e - 1]];
         st.first_cont = b > 0 && rrow[hits[b - 1]] == st.first_row;
         st.continues = e < H && rrow[hits[e]] == st.last_row;
-        st.w = u;
-        st.lane = lane;
-        st.R = R;
-        st.Y = Y;
-        st.carry = carry;
-        uint32_t n = 0;
-        feed_hits<V, M, sizeof(V) == 8 ? 16 : 32>(st, hit, hr, hc, hx, row_of, n);
+        for (int64_t k0 = b; k0 < e; k0 += S) st.step(k0, e - k0 < S ? (int)(e - k0) : S, meta, row_of);
         st.finish(cflag, ckey);
-        whits += n;
+        whits += (unsigned long long)(e - b);
         wheads += st.nheads;
     }
     add_counts<NT>(whits, wheads, counts);
@@ -1194,6 +1258,28 @@ void carry_reduce(hipStream_t s, DevBuf* tmp, int64_t n1, int R, V* Y, const uin
     }
 }
 
+// Lanes per hit of the grouped row sums: a power of two covering R columns
+// at group_vw values per lane.
+template <typename V>
+int group_lanes(int R) {
+    const int need = (R + group_vw<V>() - 1) / group_vw<V>();
+    int G = 2;
+    while (G < need) G *= 2;
+    return G;
+}
+
+// f(std::integral_constant<int, G>) for the runtime lane count G.
+template <typename F>
+void with_group(int G, F f) {
+    switch (G) {
+        case 2: f(std::integral_constant<int, 2>{}); break;
+        case 4: f(std::integral_constant<int, 4>{}); break;
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        case 16: f(std::integral_constant<int, 16>{}); break;
+        default: f(std::integral_constant<int, 32>{}); break;
+    }
+}
+
 // Per-workgroup counter partials of a pass over `blocks` workgroups (null
 // when no counters are wanted).
 unsigned long long* count_parts(ModeLookup& L, const uint64_t* counts, unsigned blocks) {
@@ -1281,8 +1367,11 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
                                ccount, hitl, nh);
             const dim3 hgrid(grid_for(nu, kBitThreads / 64, 8 * num_cus(s)));
             unsigned long long* part = count_parts(L, counts, hgrid.x);
-            hipLaunchKernelGGL(hit_scan_reduce<V>, hgrid, dim3(kBitThreads), 0, s, hitl, nh, nu,
-                               L.rrow.as<uint32_t>(), L.cpos.as<uint32_t>(), vals, design, R, Y, vec, flag, key, part);
+            with_group(group_lanes<V>(R), [&](auto gc) {
+                hipLaunchKernelGGL((hit_scan_reduce<V, decltype(gc)::value>), hgrid, dim3(kBitThreads), 0, s, hitl,
+                                   nh, nu, L.rrow.as<uint32_t>(), L.cpos.as<uint32_t>(), vals, design, R, Y, vec,
+                                   flag, key, part);
+            });
             if (counts) hipLaunchKernelGGL(sum_counts, dim3(1), dim3(kThreads), 0, s, part, (int)hgrid.x, counts);
             carry_reduce<V>(s, tmp, 2 * nu, R, Y, nh);
             if (scan_events) EX_HIP(hipEventRecord(scan_events[1], s));
@@ -1301,9 +1390,12 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
                            L.filt.as<uint32_t>(), fsel);
         const dim3 scan_grid(grid_for(nch, kScanThreads / 64, 2 * num_cus(s)));
         unsigned long long* part = count_parts(L, counts, scan_grid.x);
-        hipLaunchKernelGGL((row_scan_reduce<V, false>), scan_grid, dim3(kScanThreads), 0, s, L.rrow.as<uint32_t>(),
-                           L.rfib.as<uint32_t>(), vals, L.nnz, nch, stamp, L.cmap.as<uint64_t>(),
-                           L.filt.as<uint32_t>(), design, ExactSrc<V>{}, R, Y, vec, flag, key, part);
+        with_group(group_lanes<V>(R), [&](auto gc) {
+            hipLaunchKernelGGL((row_scan_reduce<V, false, decltype(gc)::value>), scan_grid, dim3(kScanThreads), 0, s,
+                               L.rrow.as<uint32_t>(), L.rfib.as<uint32_t>(), vals, L.nnz, nch, stamp,
+                               L.cmap.as<uint64_t>(), L.filt.as<uint32_t>(), design, ExactSrc<V>{}, R, Y, vec, flag,
+                               key, part);
+        });
         if (counts) hipLaunchKernelGGL(sum_counts, dim3(1), dim3(kThreads), 0, s, part, (int)scan_grid.x, counts);
         carry_reduce<V>(s, tmp, 2 * nch, R, Y);
         hipLaunchKernelGGL(clear_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, fsel, Ju, Ju_dev,
@@ -1335,9 +1427,11 @@ void exact_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const V* const* U
         carry_views(tmp[2], 2 * nch, R, vec, flag, key);
         const dim3 scan_grid(grid_for(nch, kScanThreads / 64, 2 * num_cus(s)));
         unsigned long long* part = count_parts(L, counts, scan_grid.x);
-        hipLaunchKernelGGL((row_scan_reduce<V, true>), scan_grid, dim3(kScanThreads), 0, s, L.rrow.as<uint32_t>(),
-                           L.rfib.as<uint32_t>(), vals, L.nnz, nch, 0u, nullptr, nullptr, nullptr, ex, R, Y, vec,
-                           flag, key, part);
+        with_group(group_lanes<V>(R), [&](auto gc) {
+            hipLaunchKernelGGL((row_scan_reduce<V, true, decltype(gc)::value>), scan_grid, dim3(kScanThreads), 0, s,
+                               L.rrow.as<uint32_t>(), L.rfib.as<uint32_t>(), vals, L.nnz, nch, 0u, nullptr, nullptr,
+                               nullptr, ex, R, Y, vec, flag, key, part);
+        });
         if (counts) hipLaunchKernelGGL(sum_counts, dim3(1), dim3(kThreads), 0, s, part, (int)scan_grid.x, counts);
         carry_reduce<V>(s, tmp, 2 * nch, R, Y);
     }

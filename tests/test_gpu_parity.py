@@ -79,7 +79,7 @@ PATHS = {"auto": 0, "row_pass": 1, "hit_list": 2}
 
 
 @pytest.mark.parametrize("path", list(PATHS))
-@pytest.mark.parametrize("R", [8, 32])
+@pytest.mark.parametrize("R", [8, 25, 32, 64])
 def test_sampled_mttkrp_fixed_samples(ctx, orc, R, path):
     from rdist_tensor_amd import abi
     rng = np.random.default_rng(11 + R)
