@@ -233,7 +233,7 @@ def run_cpu_baseline(cfg, idx, vals, U0, budget_s=30.0):
     setup = time.time() - t0
     N = len(cfg["dims"])
     done, spent = 0, 0.0
-    while spent < budget_s and done < 3 * N:
+    while spent < budget_s and done < 6 * N:
         t1 = time.perf_counter()
         a.step(done % N, cfg["J"], 1 + done // N)
         spent += time.perf_counter() - t1
