@@ -288,7 +288,10 @@ def main():
     head = "mixed" if "mixed" in res else "fp64"
     h = res[head]
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and len(idx) > 100_000_000:
+        cpu = {"value": None, "unit": "sweeps/s", "cores": None, "kind": "port",
+               "sample": "not run: the oracle's lookups for this many nonzeros exceed a bounded sample"}
+    elif not args.no_cpu:
         cpu = run_cpu_baseline(cfg, idx, vals, U0, args.cpu_budget)
     out = {
         "metric": "ALS sweeps/s + sampled-MTTKRP HBM GB/s, mixed vs FP64 GPU at equal fit",
@@ -303,8 +306,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+bf16(mfma)/f64 solves" if head == "mixed" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: STS-CP, dims {cfg['dims']}, nnz {len(idx)}, Zipf {cfg['zipf']}, "
-                               f"lognormal values, R={R}, J={J}, seed {cfg['seed']}, {K} sweeps per precision",
+        "config": {"workload": f"{args.config}: {cfg['sampler']}, dims {cfg['dims']}, nnz {len(idx)}, "
+                               f"Zipf {cfg['zipf']}, {cfg['values']} values, R={R}, J={J}, seed {cfg['seed']}, "
+                               f"{K} sweeps per precision",
                    "precision": head, "sampler": cfg["sampler"]},
         "sampled_mttkrp_gbs": h["sampled_mttkrp_gbs"],
         "nnz_per_s": h["nnz_per_s"],

@@ -210,12 +210,17 @@ __global__ __launch_bounds__(kThreads) void rows_of_keyorder(const uint32_t* __r
 
 // ---------------------------------------------------------------- sampled MTTKRP
 // Sampled fibers of this step: cmap[f] = (stamp << 32) | design row, where
-// the stamp numbers the step, and bit (f mod 2^19) of a 64 KB filter that the
-// row pass keeps in LDS.  A nonzero is a hit when its fiber's filter bit is
-// set and its cmap entry carries this step's stamp, so cmap never needs
-// clearing.  fsel[c] keeps the fiber number (or ~0u) to clear the filter.
+// the stamp numbers the step, and two bits of a 64 KB filter that the row
+// pass keeps in LDS (a Bloom filter with two hashes: f mod 2^19 and the top
+// 19 bits of f times 2654435761, Knuth's multiplicative hash).  A nonzero is
+// a hit when both of its fiber's filter bits are set and its cmap entry
+// carries this step's stamp, so cmap never needs clearing.  fsel[c] keeps
+// the fiber number (or ~0u) to clear the filter.
 constexpr int kFilterBits = 1 << 19;
 constexpr int kFilterWords = kFilterBits / 32;
+
+__device__ __forceinline__ uint32_t filter_h1(uint32_t f) { return f & (kFilterBits - 1); }
+__device__ __forceinline__ uint32_t filter_h2(uint32_t f) { return (f * 2654435761u) >> (32 - 19); }
 
 __global__ __launch_bounds__(kThreads) void mark_fibers(const uint64_t* __restrict__ kd, int64_t Jb,
                                                         const uint64_t* __restrict__ Ju_dev,
@@ -233,8 +238,9 @@ __global__ __launch_bounds__(kThreads) void mark_fibers(const uint64_t* __restri
         uint32_t f = ~0u;
         if (lo < nfib && fkey[lo] == key) {
             f = (uint32_t)lo;
-            const uint32_t h = f & (kFilterBits - 1);
-            atomicOr(filt + (h >> 5), 1u << (h & 31));
+            const uint32_t h1 = filter_h1(f), h2 = filter_h2(f);
+            atomicOr(filt + (h1 >> 5), 1u << (h1 & 31));
+            atomicOr(filt + (h2 >> 5), 1u << (h2 & 31));
             cmap[f] = ((uint64_t)stamp << 32) | (uint32_t)c;
         }
         fsel[c] = f;
@@ -248,7 +254,10 @@ __global__ __launch_bounds__(kThreads) void clear_fibers(const uint32_t* __restr
     const int64_t Ju = rows_of(Jb, Ju_dev);
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < Ju; c += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t f = fsel[c];
-        if (f != ~0u) filt[(f & (kFilterBits - 1)) >> 5] = 0u;
+        if (f != ~0u) {
+            filt[filter_h1(f) >> 5] = 0u;
+            filt[filter_h2(f) >> 5] = 0u;
+        }
     }
 }
 
@@ -328,25 +337,9 @@ struct GroupRows {
             if (col < R) dst[col] = x[v];
         }
     }
-    // one step over hits k0 .. k0 + n - 1 (1 <= n <= S); meta(k, r, c, x)
-    // fetches hit k, row_of(c, col0, d) its design values at columns
-    // col0 .. col0 + VW - 1 (zero at columns >= R)
-    template <typename Meta, typename RowOf>
-    __device__ void step(int64_t k0, int n, Meta meta, RowOf row_of) {
+    // one step over n (1 <= n <= S) fetched hits, group g holding hit g
+    __device__ void combine(uint32_t r, V (&p)[VW], int n) {
         const bool valid = g < n;
-        uint32_t r = 0xFFFFFFFFu;
-        V p[VW];
-#pragma unroll
-        for (int v = 0; v < VW; v++) p[v] = (V)0;
-        if (valid) {
-            uint32_t c;
-            V x;
-            meta(k0 + g, r, c, x);
-            V d[VW];
-            row_of(c, sub * VW, d);
-#pragma unroll
-            for (int v = 0; v < VW; v++) p[v] = d[v] * x;
-        }
         // segmented inclusive scan across the groups
 #pragma unroll
         for (int d = 1; d < S; d <<= 1) {
@@ -384,6 +377,43 @@ struct GroupRows {
 #pragma unroll
         for (int v = 0; v < VW; v++) acc[v] = __shfl(p[v], L * G + sub, 64);
         active = true;
+    }
+    // hits kb .. ke - 1 in steps of S, the loads of NB steps issued
+    // together: meta(k, r, c, x) fetches hit k, pre_of(c) the handle of its
+    // design row (a dependent load may happen here), row_of(h, col0, d) the
+    // row's values at columns col0 .. col0 + VW - 1 (zero at columns >= R).
+    // Each stage is issued for all NB steps before the next stage uses it.
+    template <int NB, typename Meta, typename Pre, typename RowOf>
+    __device__ void run(int64_t kb, int64_t ke, Meta meta, Pre pre_of, RowOf row_of) {
+        for (int64_t k0 = kb; k0 < ke; k0 += (int64_t)NB * S) {
+            uint32_t rr[NB], cc[NB];
+            V xx[NB];
+            uint64_t hh[NB];
+            V dd[NB][VW];
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int64_t k = k0 + (int64_t)b * S + g;
+                meta(k < ke ? k : ke - 1, rr[b], cc[b], xx[b]);
+                if (k >= ke) {
+                    rr[b] = 0xFFFFFFFFu;
+                    xx[b] = (V)0;
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < NB; b++) hh[b] = pre_of(cc[b]);
+#pragma unroll
+            for (int b = 0; b < NB; b++) row_of(hh[b], sub * VW, dd[b]);
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const int64_t s0 = k0 + (int64_t)b * S;
+                if (s0 < ke) {
+                    V p[VW];
+#pragma unroll
+                    for (int v = 0; v < VW; v++) p[v] = dd[b][v] * xx[b];
+                    combine(rr[b], p, ke - s0 < S ? (int)(ke - s0) : S);
+                }
+            }
+        }
     }
     __device__ void finish(uint32_t* cflag, uint32_t* ckey) {
         const bool last_head = head_row(last_row);
@@ -429,6 +459,9 @@ struct GroupRows {
 // Columns per lane: four FP32 or two FP64 values.
 template <typename V>
 constexpr int group_vw() { return sizeof(V) == 4 ? 4 : 2; }
+
+// Grouped steps whose loads are issued together.
+constexpr int kStepBatch = 4;
 
 // Sums each wavefront's hit and head counts per workgroup into
 // part[2 * blockIdx.x + {0, 1}] (no atomics: many workgroups adding to one
@@ -530,16 +563,20 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
         nprev = b2 > 0 ? rrow[b2 - 1] : 0u;
         nnext = e2 < nnz ? rrow[e2] : 0u;
     };
-    auto row_of = [ex, design, R](uint32_t c, int col0, V (&d)[VW]) {
+    // design-row handle: the fiber key (exact) or the design row
+    auto pre_of = [ex](uint32_t c) -> uint64_t {
+        if constexpr (kExact) return ex.fkey[c];
+        else return (uint64_t)c;
+    };
+    auto row_of = [ex, design, R](uint64_t h, int col0, V (&d)[VW]) {
         if constexpr (kExact) {
             // product of the fiber's factor rows in mode order
-            const uint64_t key = ex.fkey[c];
 #pragma unroll
             for (int v = 0; v < VW; v++) d[v] = (V)1;
 #pragma unroll
             for (int k = 0; k < kMaxModes; k++) {
                 if (k < ex.N && ex.shift[k] >= 0) {
-                    const V* u = ex.U[k] + (int64_t)((key >> ex.shift[k]) & ex.mask[k]) * R;
+                    const V* u = ex.U[k] + (int64_t)((h >> ex.shift[k]) & ex.mask[k]) * R;
 #pragma unroll
                     for (int v = 0; v < VW; v++)
                         if (col0 + v < R) d[v] *= u[col0 + v];
@@ -549,7 +586,7 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
             for (int v = 0; v < VW; v++)
                 if (col0 + v >= R) d[v] = (V)0;
         } else {
-            const V* u = design + (int64_t)c * R;
+            const V* u = design + (int64_t)h * R;
 #pragma unroll
             for (int v = 0; v < VW; v++) d[v] = col0 + v < R ? u[col0 + v] : (V)0;
         }
@@ -589,8 +626,8 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
 #pragma unroll
             for (int m = 0; m < M; m++) {
                 const int64_t i = b + 64 * m + lane;
-                const uint32_t h = fr[m] & (kFilterBits - 1);
-                pos[m] = i < e && ((lfilt[h >> 5] >> (h & 31)) & 1u);
+                const uint32_t h1 = filter_h1(fr[m]), h2 = filter_h2(fr[m]);
+                pos[m] = i < e && ((lfilt[h1 >> 5] >> (h1 & 31)) & (lfilt[h2 >> 5] >> (h2 & 31)) & 1u);
             }
 #pragma unroll
             for (int m = 0; m < M; m++) {
@@ -628,7 +665,7 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
             nh += (uint32_t)__popcll(mask);
         }
         wave_lds_sync();
-        for (uint32_t k0 = 0; k0 < nh; k0 += S) st.step(k0, nh - k0 < (uint32_t)S ? (int)(nh - k0) : S, meta, row_of);
+        if (nh > 0) st.template run<kStepBatch>(0, nh, meta, pre_of, row_of);
         st.finish(cflag, ckey);
         wave_lds_sync();
         whits += nh;
@@ -841,8 +878,9 @@ __global__ __launch_bounds__(kBitThreads) void hit_scan_reduce(const uint32_t* _
     const int64_t nuh = (H + kHitUnit - 1) / kHitUnit;
     const int64_t units = nuh < nu ? nuh : nu;
     unsigned long long whits = 0, wheads = 0;
-    auto row_of = [design, R](uint32_t c, int col0, V (&d)[VW]) {
-        const V* u = design + (int64_t)c * R;
+    auto pre_of = [](uint32_t c) -> uint64_t { return (uint64_t)c; };
+    auto row_of = [design, R](uint64_t h, int col0, V (&d)[VW]) {
+        const V* u = design + (int64_t)h * R;
 #pragma unroll
         for (int v = 0; v < VW; v++) d[v] = col0 + v < R ? u[col0 + v] : (V)0;
     };
@@ -862,7 +900,7 @@ __global__ __launch_bounds__(kBitThreads) void hit_scan_reduce(const uint32_t* _
         st.last_row = rrow[hits[e - 1]];
         st.first_cont = b > 0 && rrow[hits[b - 1]] == st.first_row;
         st.continues = e < H && rrow[hits[e]] == st.last_row;
-        for (int64_t k0 = b; k0 < e; k0 += S) st.step(k0, e - k0 < S ? (int)(e - k0) : S, meta, row_of);
+        st.template run<kStepBatch>(b, e, meta, pre_of, row_of);
         st.finish(cflag, ckey);
         whits += (unsigned long long)(e - b);
         wheads += st.nheads;
@@ -1388,7 +1426,7 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
         hipLaunchKernelGGL(mark_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, kd, Ju, Ju_dev,
                            L.fkey.as<uint64_t>(), L.nfib, stamp, L.cmap.as<uint64_t>(),
                            L.filt.as<uint32_t>(), fsel);
-        const dim3 scan_grid(grid_for(nch, kScanThreads / 64, 2 * num_cus(s)));
+        const dim3 scan_grid(grid_for(nch, kScanThreads / 64, num_cus(s)));   // one workgroup per CU (LDS)
         unsigned long long* part = count_parts(L, counts, scan_grid.x);
         with_group(group_lanes<V>(R), [&](auto gc) {
             hipLaunchKernelGGL((row_scan_reduce<V, false, decltype(gc)::value>), scan_grid, dim3(kScanThreads), 0, s,
@@ -1425,7 +1463,7 @@ void exact_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const V* const* U
         V* vec;
         uint32_t *flag, *key;
         carry_views(tmp[2], 2 * nch, R, vec, flag, key);
-        const dim3 scan_grid(grid_for(nch, kScanThreads / 64, 2 * num_cus(s)));
+        const dim3 scan_grid(grid_for(nch, kScanThreads / 64, num_cus(s)));   // one workgroup per CU (LDS)
         unsigned long long* part = count_parts(L, counts, scan_grid.x);
         with_group(group_lanes<V>(R), [&](auto gc) {
             hipLaunchKernelGGL((row_scan_reduce<V, true, decltype(gc)::value>), scan_grid, dim3(kScanThreads), 0, s,

@@ -92,6 +92,9 @@ CONFIGS = {
                           sweeps=20, seed=0, sampler="cp_arls_lev"),
     "sts_1e6x1e6x1e5": dict(dims=[1_000_000, 1_000_000, 100_000], nnz=50_000_000, zipf=[1.1, 1.1, 1.1],
                             values="lognormal", R=32, J=65536, sweeps=30, seed=1, sampler="sts_cp"),
+    "sts_1e7x1e6x1e4x1e2": dict(dims=[10_000_000, 1_000_000, 10_000, 100], nnz=500_000_000,
+                                zipf=[1.2, 1.2, None, None], values="poisson", R=64, J=131072, sweeps=20, seed=2,
+                                sampler="sts_cp"),
 }
 
 

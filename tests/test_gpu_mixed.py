@@ -157,3 +157,32 @@ def test_mixed_factor_copies_stay_consistent(ctx):
     A.step(0, 4096, 2)
     assert np.allclose(F.get(1), V)
     assert np.isfinite(A.fit())
+
+
+def test_four_mode_rank64_shape(ctx):
+    """The 4-mode R = 64 shape of the at-scale STS-CP workload (Zipf(1.2) on
+    modes 0-1, uniform on 2-3, 1 + Poisson(2) values, J = 131,072) with every
+    mode size and the nonzero count divided down: both precisions run, the
+    counters are consistent and the final fits agree within 1e-3."""
+    import json
+    from rdist_tensor_amd import abi, synthetic
+    dims = [100_000, 10_000, 1_000, 100]
+    R, J = 64, 131072
+    idx = synthetic.coords(dims, 1_000_000, 2, [1.2, 1.2, None, None])
+    vals = synthetic.values(len(idx), 2, "poisson")
+    U = synthetic.factors(dims, R, 2)
+    fits = {}
+    for prec in (abi.EX_FP64, abi.EX_MIXED):
+        T = abi.Tensor(ctx, idx, vals, dims)
+        F = abi.Factors(ctx, dims, R)
+        for k in range(4):
+            F.set(k, U[k])
+        S = abi.Sampler(ctx, F, abi.EX_SAMPLER_STS_CP, precision=prec, seed=2)
+        A = abi.ALS(ctx, T, F, S, precision=prec)
+        A.init()
+        A.run(4, J, 2)
+        st = json.loads(A.stats_json())
+        assert 0 < st["sum_nonzeros_iterated"] <= 4 * 4 * len(idx)
+        fits[prec] = st["fits"][1:]
+    print("fp64", fits[abi.EX_FP64], "mixed", fits[abi.EX_MIXED])
+    assert abs(fits[abi.EX_FP64][-1] - fits[abi.EX_MIXED][-1]) < 1e-3
