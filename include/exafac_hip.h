@@ -105,6 +105,28 @@ int ex_sampled_mttkrp(ex_tensor* t, int mode, int64_t Ju, const uint32_t* sample
 enum ex_mttkrp_path { EX_MTTKRP_AUTO = 0, EX_MTTKRP_ROW_PASS = 1, EX_MTTKRP_HIT_LIST = 2 };
 int ex_tensor_mttkrp_path(ex_tensor* t, int path);
 
+/* ---- differentiable sampled MTTKRP ------------------------------------- */
+/* No reference counterpart: the forward is SortIdxLookup::csr_based_spmm
+ * (sort_lookup.hpp:238-375) applied to the design rows of
+ * accumulator_stationary.hpp:210-249, the backward its hand-written adjoint,
+ * for callers that differentiate a sampled least-squares objective (the
+ * torch.autograd.Function in rdist_tensor_amd/autograd.py).  Every array
+ * argument is a DEVICE pointer on the context's device (a torch tensor's
+ * data_ptr()); factors and grads are host arrays of N such pointers
+ * (I_k x R row-major FP64).  samples: Ju x N deduplicated tuples (column
+ * `mode` ignored); weights: Ju.  Both calls wait for their work on the
+ * context stream before returning.
+ * Forward: Y (I_mode x R, overwritten) = sum over the nonzeros x of each
+ * sampled fiber c of x * (prod_{k != mode} U_k[s_c,k] * sqrt(w_c)) * sqrt(w_c).
+ * Backward: given G = dL/dY (I_mode x R), adds dL/dU_k into grads[k] for
+ * every k != mode with grads[k] non-null (grads[mode] is ignored); the
+ * Khatri-Rao rows are recomputed, not stored; rows shared by several
+ * samples are accumulated with atomic adds (order-dependent rounding). */
+int ex_smttkrp_forward(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samples, const double* weights,
+                       const double* const* factors, int64_t R, double* Y);
+int ex_smttkrp_backward(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samples, const double* weights,
+                        const double* const* factors, int64_t R, const double* G, double* const* grads);
+
 /* ---- factors (LowRankTensor) ----------------------------------------- */
 /* LowRankTensor(R, TensorGrid&) -- low_rank_tensor.hpp:16-25: N factor
  * matrices I_k x R (FP64 on the device) and sigma = 1. */

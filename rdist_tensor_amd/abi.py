@@ -6,6 +6,7 @@ CPU fallback.
 """
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -56,6 +57,8 @@ SIGNATURES = {
     "ex_als_create": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _pp],
     "ex_exact_als_create": [_vp, _vp, _vp, C.c_int, _pp],
     "ex_tensor_mttkrp_path": [_vp, C.c_int],
+    "ex_smttkrp_forward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp],
+    "ex_smttkrp_backward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp, _vp],
     "ex_als_destroy": [_vp],
     "ex_als_init_ground_truth": [_vp],
     "ex_als_step": [_vp, C.c_int, C.c_int64, C.c_uint64],
@@ -113,6 +116,13 @@ def u32(a):
 
 class Context:
     def __init__(self, device=0):
+        # PyTorch-ROCm carries its own HIP runtime.  In a process that also
+        # uses PyTorch on the GPU, PyTorch's runtime has to start first (the
+        # other order leaves PyTorch without devices), so it is started here
+        # when the caller has imported torch.
+        torch = sys.modules.get("torch")
+        if torch is not None and torch.cuda.is_available():
+            torch.cuda.init()
         self.h = C.c_void_p()
         check(lib().ex_ctx_create(device, C.byref(self.h)))
 

@@ -641,6 +641,30 @@ double als_fit_impl(ex_als* a) {
     return 1.0 - std::sqrt(h[0] + h[1]) / a->t->norm;
 }
 
+// Keys of device sample tuples for `mode`, checked distinct (the sampled
+// MTTKRP sums each fiber once per sample row).
+void smttkrp_keys(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samples, DevBuf& kd) {
+    hipStream_t st = t->ctx->c.stream;
+    const ModeLookup& L = t->lk[mode];
+    kd.ensure((size_t)(Ju > 0 ? Ju : 1) * 8);
+    k::keys_from_coords(st, samples, Ju, L.kl, kd.as<uint64_t>(), nullptr);
+    std::vector<uint64_t> keys((size_t)Ju);
+    d2h(st, keys.data(), kd.p, (size_t)Ju * 8);
+    std::sort(keys.begin(), keys.end());
+    EX_CHECK(std::adjacent_find(keys.begin(), keys.end()) == keys.end(), EX_ERR_ARG,
+             "samples must be deduplicated: two rows share every non-target coordinate");
+}
+
+void smttkrp_check(ex_tensor* t, int mode, int64_t Ju, const void* samples, const void* weights,
+                   const double* const* factors, int64_t R) {
+    EX_CHECK(t && factors, EX_ERR_ARG, "null argument");
+    EX_CHECK(mode >= 0 && mode < t->N, EX_ERR_ARG, "mode out of range");
+    EX_CHECK(R >= 1 && R <= EX_MAX_RANK, EX_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+    EX_CHECK(Ju >= 0 && Ju < (int64_t)1 << 31, EX_ERR_ARG, "sample count out of range");
+    EX_CHECK(Ju == 0 || (samples && weights), EX_ERR_ARG, "null samples or weights");
+    for (int k2 = 0; k2 < t->N; k2++)
+        EX_CHECK(k2 == mode || factors[k2], EX_ERR_ARG, "null factor pointer");
+}
 }  // namespace
 
 extern "C" {
@@ -831,6 +855,49 @@ int ex_sampled_mttkrp(ex_tensor* t, int mode, int64_t Ju, const uint32_t* sample
         if (hits) *hits = (int64_t)c[0];
     });
 }
+
+// ---------------------------------------------------------------- differentiable sampled MTTKRP
+int ex_smttkrp_forward(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samples, const double* weights,
+                       const double* const* factors, int64_t R, double* Y) {
+    return guarded([&] {
+        smttkrp_check(t, mode, Ju, samples, weights, factors, R);
+        EX_CHECK(Y, EX_ERR_ARG, "null output");
+        EX_HIP(hipSetDevice(t->ctx->c.device));
+        tensor_build_lookups(t);
+        hipStream_t st = t->ctx->c.stream;
+        ModeLookup& L = t->lk[mode];
+        const int64_t Ij = t->dims[mode];
+        DevBuf kd, dg, dm;
+        DevBuf tmp[4];
+        smttkrp_keys(t, mode, Ju, samples, kd);
+        dg.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
+        dm.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
+        k::design(st, samples, weights, Ju, t->N, mode, (int)R, factors, dg.as<double>(), dm.as<double>());
+        k::sampled_mttkrp<double>(st, L, L.rval.as<double>(), kd.as<uint64_t>(), Ju, dm.as<double>(), (int)R, Y,
+                                  Ij, tmp);
+        EX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int ex_smttkrp_backward(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samples, const double* weights,
+                        const double* const* factors, int64_t R, const double* G, double* const* grads) {
+    return guarded([&] {
+        smttkrp_check(t, mode, Ju, samples, weights, factors, R);
+        EX_CHECK(G && grads, EX_ERR_ARG, "null gradient argument");
+        EX_HIP(hipSetDevice(t->ctx->c.device));
+        tensor_build_lookups(t);
+        hipStream_t st = t->ctx->c.stream;
+        DevBuf kd, dD;
+        smttkrp_keys(t, mode, Ju, samples, kd);
+        dD.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
+        double* gp[kMaxModes] = {};
+        for (int k2 = 0; k2 < t->N; k2++) gp[k2] = k2 == mode ? nullptr : grads[k2];
+        k::smttkrp_adjoint(st, t->lk[mode], kd.as<uint64_t>(), samples, weights, Ju, t->N, mode, (int)R, factors, G,
+                           gp, dD.as<double>());
+        EX_HIP(hipStreamSynchronize(st));
+    });
+}
+
 
 // ---------------------------------------------------------------- factors
 int ex_factors_create(ex_ctx* ctx, int N, const int64_t* dims, int64_t R, ex_factors** out) {

@@ -221,6 +221,11 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
 template <typename V>
 void exact_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const V* const* U, const int64_t* dims, int R,
                   V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts = nullptr);
+// Adjoint of the sampled MTTKRP: dD = per-sample fiber sums of x * G[row]
+// (Ju x R, written), then grads[k] (k != j, non-null) += dD * dDesign/dU_k.
+void smttkrp_adjoint(hipStream_t s, const ModeLookup& L, const uint64_t* kd, const uint32_t* sd, const double* wd,
+                     int64_t Ju, int N, int j, int R, const double* const* U, const double* G,
+                     double* const* grads, double* dD);
 void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
              const double* sigma, int R, double* out, DevBuf& part);
 // final stages shared with the mixed-precision kernels

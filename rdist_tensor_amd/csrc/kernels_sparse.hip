@@ -1247,6 +1247,83 @@ void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int
     EX_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- adjoint
+// dD[c] = sum over the nonzeros (row i, value x) of sample c's fiber of
+// x * G[i]: the transpose of the sampled MTTKRP, one wavefront per sample
+// (lane r < R sums column r over the fiber's nonzeros in key order).
+__global__ __launch_bounds__(kThreads) void fiber_adjoint(const uint64_t* __restrict__ kd, int64_t Ju,
+                                                          const uint64_t* __restrict__ fkey, int64_t nfib,
+                                                          const uint32_t* __restrict__ fptr,
+                                                          const uint32_t* __restrict__ kpos,
+                                                          const uint32_t* __restrict__ rrow,
+                                                          const double* __restrict__ rval,
+                                                          const double* __restrict__ G, int R,
+                                                          double* __restrict__ dD) {
+    const int lane = threadIdx.x % 64;
+    for (int64_t c = blockIdx.x * (int64_t)(kThreads / 64) + threadIdx.x / 64; c < Ju;
+         c += (int64_t)gridDim.x * (kThreads / 64)) {
+        const int64_t f = wave_find_key(fkey, nfib, kd[c], lane);
+        double acc = 0.0;
+        if (f >= 0 && lane < R) {
+            const uint32_t k1 = fptr[f + 1];
+            for (uint32_t k = fptr[f]; k < k1; k++) {
+                const uint32_t p = kpos[k];
+                acc += rval[p] * G[(int64_t)rrow[p] * R + lane];
+            }
+        }
+        if (lane < R) dD[c * R + lane] = acc;
+    }
+}
+
+struct GradArgs {
+    double* G[kMaxModes];
+};
+
+// dL/dU_k[s_k] += dD[c] * prod_{k' != j, k} U_k'[s_k'] * sqrt(w) * sqrt(w)
+// for every non-target k: the design rows (prod U * sqrt(w)) * sqrt(w)
+// differentiated, Khatri-Rao rows recomputed.  Rows repeat across samples,
+// so the adds are atomic.
+__global__ __launch_bounds__(kThreads) void scatter_adjoint(const uint32_t* __restrict__ sd,
+                                                            const double* __restrict__ wd, int64_t Ju,
+                                                            DesignArgs a, const double* __restrict__ dD,
+                                                            GradArgs g) {
+    const RowMap rm(a.R);
+    if (!rm.on()) return;
+    const int r = rm.c;
+    for (int64_t c = rm.first(); c < Ju; c += rm.step()) {
+        const double sw = sqrt(wd[c]);
+        const double base = dD[c * a.R + r];
+        for (int k = 0; k < a.N; k++) {
+            if (k == a.j || !g.G[k]) continue;
+            double prod = 1.0;
+            for (int k2 = 0; k2 < a.N; k2++) {
+                if (k2 == a.j || k2 == k) continue;
+                prod *= a.U[k2][(int64_t)sd[c * a.N + k2] * a.R + r];
+            }
+            atomicAdd(g.G[k] + (int64_t)sd[c * a.N + k] * a.R + r, base * (prod * sw * sw));
+        }
+    }
+}
+
+void smttkrp_adjoint(hipStream_t s, const ModeLookup& L, const uint64_t* kd, const uint32_t* sd, const double* wd,
+                     int64_t Ju, int N, int j, int R, const double* const* U, const double* G,
+                     double* const* grads, double* dD) {
+    if (Ju == 0) return;
+    hipLaunchKernelGGL(fiber_adjoint, dim3(grid_for(Ju, kThreads / 64, 16384)), dim3(kThreads), 0, s, kd, Ju,
+                       L.fkey.as<uint64_t>(), L.nfib, L.fptr.as<uint32_t>(), L.kpos.as<uint32_t>(),
+                       L.rrow.as<uint32_t>(), L.rval.as<double>(), G, R, dD);
+    DesignArgs a;
+    GradArgs g;
+    for (int k = 0; k < N; k++) {
+        a.U[k] = U[k];
+        g.G[k] = grads[k];
+    }
+    a.N = N; a.j = j; a.R = R;
+    hipLaunchKernelGGL(scatter_adjoint, dim3(grid_for(Ju, kThreads / R, 16384)), dim3(kThreads), 0, s, sd, wd, Ju,
+                       a, dD, g);
+    EX_HIP(hipGetLastError());
+}
+
 // Carry lists of the row pass: level l has n_l = 2 * chunks(n_{l-1})
 // entries of (R-vector, flag, key), in tmp[2] and tmp[3] alternately.
 template <typename V>

@@ -22,6 +22,7 @@ def orc():
 
 @pytest.fixture(scope="session")
 def ctx():
+    import torch  # noqa: F401  (tests hand torch device tensors to the library: its runtime starts first)
     from rdist_tensor_amd import abi
     c = abi.Context(0)
     print(f"\n[gpu] device: {c.device_name()}", flush=True)
