@@ -76,6 +76,10 @@ std::string json_list(const std::vector<T>& v) {
 
 struct ex_ctx {
     Ctx c;
+    // second stream for work that overlaps the main stream inside a step
+    // (the design Gram and its inverse run beside the sampled MTTKRP)
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // opt-in per-launch-group timing (ex_ctx_timing): events are recorded
     // around named launch groups on the context's stream and resolved after
     // the stream is synchronised
@@ -150,17 +154,18 @@ std::string device_label(const ex_ctx* c) {
 struct Span {
     ex_ctx* c;
     std::string name;
+    hipStream_t st;
     hipEvent_t a = nullptr;
-    Span(ex_ctx* ctx, const char* n) : c(ctx), name(n) {
+    Span(ex_ctx* ctx, const char* n, hipStream_t s = nullptr) : c(ctx), name(n), st(s ? s : ctx->c.stream) {
         if (c->timing) {
             a = c->get_event();
-            EX_HIP(hipEventRecord(a, c->c.stream));
+            EX_HIP(hipEventRecord(a, st));
         }
     }
     void end() {
         if (c->timing && a) {
             hipEvent_t b = c->get_event();
-            EX_HIP(hipEventRecord(b, c->c.stream));
+            EX_HIP(hipEventRecord(b, st));
             c->pending.push_back({name, {a, b}});
             a = nullptr;
         }
@@ -459,15 +464,22 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
                   a->dm.as<double>(), ju);
     }
     s_de.end();
-    Span s_dp(a->ctx, "design_gram_pinv");
+    // the design Gram and its inverse run on the second stream beside the
+    // sampled MTTKRP (which needs only the design rows); the factor update
+    // waits for them
+    hipStream_t aux = a->ctx->aux;
+    EX_HIP(hipEventRecord(a->ctx->ev_fork, st));
+    EX_HIP(hipStreamWaitEvent(aux, a->ctx->ev_fork, 0));
+    Span s_dp(a->ctx, "design_gram_pinv", aux);
     if (mixed) {
-        k::gram_bf16(st, a->dg.as<float>(), Ju, R, a->G.as<double>(), a->part, ju);
-        k::spd_inverse(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>(), a->Gi32.as<float>());
+        k::gram_bf16(aux, a->dg.as<float>(), Ju, R, a->G.as<double>(), a->part, ju);
+        k::spd_inverse(aux, a->G.as<double>(), R, 1e-11, a->Gi.as<double>(), a->Gi32.as<float>());
     } else {
-        k::gram(st, a->dg.as<double>(), Ju, R, a->G.as<double>(), a->part, ju);
-        k::pinv(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>());
+        k::gram(aux, a->dg.as<double>(), Ju, R, a->G.as<double>(), a->part, ju);
+        k::pinv(aux, a->G.as<double>(), R, 1e-11, a->Gi.as<double>());
     }
     s_dp.end();
+    EX_HIP(hipEventRecord(a->ctx->ev_join, aux));
     mark(2);
     // Region 4: sampled MTTKRP (:252-274)
     a->Y.ensure((size_t)Ij * R * vb);
@@ -502,6 +514,7 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     }
     mark(3);
     // Region 6: U_j = Y G+, renormalise (:280-289)
+    EX_HIP(hipStreamWaitEvent(st, a->ctx->ev_join, 0));
     Span s_fu(a->ctx, "factor_update");
     if (mixed) {
         // U_j = Y Gi diag(1 / sigma) in one pass over Y: sigma from the Gram of Y
@@ -682,6 +695,9 @@ int ex_ctx_create(int device, ex_ctx** out) {
         EX_HIP(hipSetDevice(device));
         EX_HIP(hipGetDeviceProperties(&c->c.prop, device));
         EX_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        EX_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+        EX_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+        EX_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         *out = c.release();
     });
 }
@@ -690,6 +706,10 @@ int ex_ctx_destroy(ex_ctx* ctx) {
     return guarded([&] {
         if (!ctx) return;
         (void)hipStreamSynchronize(ctx->c.stream);
+        (void)hipStreamSynchronize(ctx->aux);
+        (void)hipEventDestroy(ctx->ev_fork);
+        (void)hipEventDestroy(ctx->ev_join);
+        (void)hipStreamDestroy(ctx->aux);
         (void)hipStreamDestroy(ctx->c.stream);
         delete ctx;
     });
