@@ -61,6 +61,8 @@ const char* ex_last_error(void);
 /* One context per device: a HIP stream plus scratch.  Replaces the MPI world
  * set up by mpi4py + Grid (grid.hpp:34-108) -- one rank, one GPU. */
 int ex_ctx_create(int device, ex_ctx** out);
+/* Every tensor, factors, sampler and optimizer object of the context must be
+ * destroyed before the context. */
 int ex_ctx_destroy(ex_ctx* ctx);
 int ex_ctx_sync(ex_ctx* ctx);
 /* Device name (e.g. "AMD Instinct MI355X") into buf. */

@@ -7,6 +7,7 @@ CPU fallback.
 import ctypes as C
 import os
 import sys
+import weakref
 
 import numpy as np
 
@@ -114,8 +115,32 @@ def u32(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
 
 
+class _Owned:
+    """A library object of one context: destroyed once, by close() or by
+    garbage collection, and always before its context (Context.close()
+    closes the live objects it created, newest first)."""
+    _destroy = None
+
+    def _register(self, ctx):
+        self.ctx = ctx
+        ctx._children.append(weakref.ref(self))
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h:
+            getattr(lib(), self._destroy)(h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     def __init__(self, device=0):
+        self._children = []
         # PyTorch-ROCm carries its own HIP runtime.  In a process that also
         # uses PyTorch on the GPU, PyTorch's runtime has to start first (the
         # other order leaves PyTorch without devices), so it is started here
@@ -128,6 +153,11 @@ class Context:
 
     def close(self):
         if self.h:
+            for ref in reversed(self._children):
+                obj = ref()
+                if obj is not None:
+                    obj.close()
+            self._children = []
             lib().ex_ctx_destroy(self.h)
             self.h = C.c_void_p()
 
@@ -185,9 +215,10 @@ class Context:
         return sd[:ju.value].copy(), wd[:ju.value].copy()
 
 
-class Tensor:
+class Tensor(_Owned):
+    _destroy = "ex_tensor_destroy"
+
     def __init__(self, ctx, idx, vals, dims, preprocessing=EX_PRE_NONE):
-        self.ctx = ctx
         self.idx = u32(idx)
         self.vals = f64(vals)
         self.dims = np.asarray(dims, np.int64)
@@ -196,10 +227,7 @@ class Tensor:
         self.h = C.c_void_p()
         check(lib().ex_tensor_create(ctx.h, N, _p(self.dims, _i64), nnz, _p(self.idx, _u32), _p(self.vals),
                                      preprocessing, C.byref(self.h)))
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            lib().ex_tensor_destroy(self.h)
+        self._register(ctx)
 
     def norm(self):
         v = C.c_double()
@@ -223,18 +251,16 @@ class Tensor:
         return Y, hits.value
 
 
-class Factors:
+class Factors(_Owned):
+    _destroy = "ex_factors_destroy"
+
     def __init__(self, ctx, dims, R):
-        self.ctx = ctx
         self.dims = np.asarray(dims, np.int64)
         self.N = len(dims)
         self.R = R
         self.h = C.c_void_p()
         check(lib().ex_factors_create(ctx.h, self.N, _p(self.dims, _i64), R, C.byref(self.h)))
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            lib().ex_factors_destroy(self.h)
+        self._register(ctx)
 
     def set(self, mode, U):
         U = f64(U)
@@ -264,17 +290,15 @@ class Factors:
         check(lib().ex_factors_init_deterministic(self.h))
 
 
-class Sampler:
+class Sampler(_Owned):
+    _destroy = "ex_sampler_destroy"
+
     def __init__(self, ctx, factors, kind, precision=EX_FP64, seed=0):
-        self.ctx = ctx
         self.f = factors
         self.kind = kind
         self.h = C.c_void_p()
         check(lib().ex_sampler_create(ctx.h, factors.h, kind, precision, seed, C.byref(self.h)))
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            lib().ex_sampler_destroy(self.h)
+        self._register(ctx)
 
     def update(self, mode):
         check(lib().ex_sampler_update(self.h, mode))
@@ -302,27 +326,25 @@ class Sampler:
         return s
 
 
-class ALS:
+class ALS(_Owned):
+    _destroy = "ex_als_destroy"
+
     def __init__(self, ctx, tensor, factors, sampler, distribution=EX_DIST_ACCUMULATOR_STATIONARY,
                  precision=EX_FP64):
-        self.ctx = ctx
         self.keep = (tensor, factors, sampler)
         self.h = C.c_void_p()
         check(lib().ex_als_create(ctx.h, tensor.h, factors.h, sampler.h, distribution, precision, C.byref(self.h)))
+        self._register(ctx)
 
     @classmethod
     def exact(cls, ctx, tensor, factors, precision=EX_FP64):
         """ExactALS (exact_als.hpp:16-143): J and sweep arguments are ignored."""
         self = cls.__new__(cls)
-        self.ctx = ctx
         self.keep = (tensor, factors)
         self.h = C.c_void_p()
         check(lib().ex_exact_als_create(ctx.h, tensor.h, factors.h, precision, C.byref(self.h)))
+        self._register(ctx)
         return self
-
-    def __del__(self):
-        if getattr(self, "h", None):
-            lib().ex_als_destroy(self.h)
 
     def init(self):
         check(lib().ex_als_init_ground_truth(self.h))

@@ -180,13 +180,21 @@ template <> struct Vec<double> {
 // LDS of one draw block: the packed-upper (a, b) table, and per sample the
 // running Hadamard product h, the current query and the leaf's q^2.
 constexpr int kDrawThreads = 256;
-template <typename TN, int SLOTS>
+template <typename TN, int SLOTS, int RMAX>
 struct DrawShared {
-    uint16_t pairs[kMaxRank * (kMaxRank + 1) / 2];
-    TN h[SLOTS][kMaxRank];
-    TN x[SLOTS][kMaxRank];
-    double q2[SLOTS][kMaxRank];
+    uint16_t pairs[RMAX * (RMAX + 1) / 2];
+    TN h[SLOTS][RMAX];
+    TN x[SLOTS][RMAX];
+    double q2[SLOTS][RMAX];
 };
+
+// Largest rank whose packed triangle fits the NV W-wide vectors of G lanes
+// (at most kMaxRank): the LDS of one instantiation is sized for it.
+constexpr int draw_rmax(int G, int NV, int W) {
+    int r = 1;
+    while (r < kMaxRank && (r + 1) * (r + 2) / 2 <= G * NV * W) r++;
+    return r;
+}
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -270,7 +278,8 @@ template <typename TN, int G, int NV>
 __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
     constexpr int W = Vec<TN>::W;
     constexpr int SPW = 64 / G;
-    __shared__ DrawShared<TN, (kDrawThreads / 64) * SPW> sh;
+    constexpr int RMAX = draw_rmax(G, NV, W);
+    __shared__ DrawShared<TN, (kDrawThreads / 64) * SPW, RMAX> sh;
     const int R = a.R, N = a.N, j = a.j;
     const int P = tri_size(R);
     const int S = sizeof(TN) == 8 ? tri_stride(R) : tri_stride32(R);
