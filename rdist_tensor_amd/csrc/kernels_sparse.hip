@@ -1408,8 +1408,8 @@ unsigned long long* count_parts(ModeLookup& L, const uint64_t* counts, unsigned 
 // nonzero's row and fiber number and handles the hits inside that pass.
 // The hits are Ju times the mean length of the sampled fibers; leverage
 // sampling favours long fibers (on a 5e7-nonzero Zipf(1.1) tensor the hits
-// were about 50x Ju times the mean fiber length), so the estimate carries
-// that factor, and the hit list is taken when it stays under nnz / 16.
+// were about 100x Ju times the mean fiber length), so the estimate carries a
+// factor of 64, and the hit list is taken when it stays under nnz / 16.
 bool use_hit_list(const ModeLookup& L, int64_t Ju) {
     if (L.mttkrp_path == 1) return false;
     if (L.mttkrp_path == 2) return true;
