@@ -245,6 +245,66 @@ def run_cpu_baseline(cfg, idx, vals, U0, budget_s=30.0):
                        f"{setup:.1f} s untimed")
 
 
+def run_batched(args):
+    """Batched CP-ARLS-LEV (mixed precision): independent tensors, one
+    optimizer and context each, all advanced together by ex_als_run_batch
+    (their streams side by side); compared with the same optimizers run one
+    after another.  A step is one sweep of every tensor."""
+    from rdist_tensor_amd import abi, synthetic
+    t0 = time.time()
+    batch = synthetic.make_batch(count=args.batch_count, seed=4, scale=args.scale)
+    log(f"[bench] generated {len(batch)} tensors, {sum(len(b[1]) for b in batch)} nonzeros in "
+        f"{time.time() - t0:.1f} s")
+    J = 8192
+    objs = []
+    t0 = time.time()
+    for dims, idx, vals, R, st in batch:
+        c = abi.Context(0)
+        T = abi.Tensor(c, idx, vals, dims)
+        F = abi.Factors(c, dims, R)
+        for k, u in enumerate(synthetic.factors(dims, R, st)):
+            F.set(k, u)
+        S = abi.Sampler(c, F, abi.EX_SAMPLER_CP_ARLS_LEV, precision=abi.EX_MIXED, seed=st)
+        A = abi.ALS(c, T, F, S, precision=abi.EX_MIXED)
+        A.init()
+        objs.append((c, T, F, S, A))
+    log(f"[bench] {len(objs)} optimizers ready in {time.time() - t0:.1f} s")
+    opts = [o[4] for o in objs]
+    abi.ALS.run_batch(opts, args.warmup, J)
+    batch_s = abi.ALS.run_batch(opts, args.steps, J)
+    log(f"[bench] batched: {args.steps} sweeps of {len(opts)} tensors in {batch_s:.3f} s")
+    seq_sweeps = max(1, args.steps // 5)
+    t1 = time.perf_counter()
+    for o in opts:
+        o.run(seq_sweeps, J, 10 ** 9)
+    seq_s = time.perf_counter() - t1
+    log(f"[bench] one after another: {seq_sweeps} sweeps of {len(opts)} tensors in {seq_s:.3f} s")
+    out = {
+        "metric": "batched CP-ARLS-LEV sweeps/s (one sweep = every tensor of the batch)",
+        "value": round(args.steps / batch_s, 4),
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batch_s / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+bf16(mfma)/f64 solves",
+        "data": "synthetic",
+        "config": {"workload": f"batched_arls: {len(opts)} 3-mode tensors, mode sizes U[500, 50000] x {args.scale}, "
+                               f"nnz = 20 x largest mode, uniform coordinates, U(0,1) values, R in {{8,16,32}}, "
+                               f"J={J}, CP-ARLS-LEV, mixed precision, seed 4"},
+        "one_after_another_sweeps_per_s": round(seq_sweeps / seq_s, 4),
+        "batched_vs_one_after_another": round((args.steps / batch_s) / (seq_sweeps / seq_s), 3),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    for c, *_ in objs:
+        c.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,11 +319,15 @@ def main():
                     help="how the sampled MTTKRP finds the sampled fibers' nonzeros")
     ap.add_argument("--exact-steps", type=int, default=3,
                     help="ExactALS sweeps per precision reported beside the randomized paths (0: skip)")
+    ap.add_argument("--batch-count", type=int, default=256, help="tensors of the batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+    if args.config == "batched_arls":
+        run_batched(args)
+        return
     from rdist_tensor_amd import synthetic
     t0 = time.time()
     cfg, idx, vals = synthetic.make(args.config, args.scale)

@@ -197,6 +197,14 @@ int ex_als_init_ground_truth(ex_als* a);
 int ex_als_step(ex_als* a, int mode, int64_t J, uint64_t sweep);
 /* ALS_Optimizer::execute_ALS_rounds(rounds, J, epoch) -- als_optimizer.hpp:40-124 */
 int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch);
+/* Many independent decompositions at once (no reference counterpart: the
+ * reference runs them one after another).  `rounds` sweeps of every optimizer,
+ * their mode steps interleaved so that each optimizer's context stream
+ * carries its own chain and the streams run side by side; every optimizer
+ * needs its own context.  The results equal those of ex_als_run on each
+ * optimizer alone (no fit evaluations, no statistics).  *seconds (optional)
+ * receives the wall time from the first launch to the last completion. */
+int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, double* seconds);
 /* ALS_Optimizer::compute_exact_fit -- als_optimizer.hpp:208-211,
  * sparse_tensor.hpp:288-356 */
 int ex_als_fit(ex_als* a, double* fit);

@@ -58,6 +58,7 @@ SIGNATURES = {
     "ex_als_create": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _pp],
     "ex_exact_als_create": [_vp, _vp, _vp, C.c_int, _pp],
     "ex_tensor_mttkrp_path": [_vp, C.c_int],
+    "ex_als_run_batch": [_pp, C.c_int, C.c_int64, C.c_int64, _d],
     "ex_smttkrp_forward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp],
     "ex_smttkrp_backward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp, _vp],
     "ex_als_destroy": [_vp],
@@ -348,6 +349,15 @@ class ALS(_Owned):
 
     def init(self):
         check(lib().ex_als_init_ground_truth(self.h))
+
+    @staticmethod
+    def run_batch(optimizers, rounds, J):
+        """`rounds` sweeps of every optimizer (each on its own context), the
+        contexts' streams running side by side; returns the wall seconds."""
+        arr = (C.c_void_p * len(optimizers))(*[o.h.value for o in optimizers])
+        sec = C.c_double()
+        check(lib().ex_als_run_batch(arr, len(optimizers), rounds, J, C.byref(sec)))
+        return sec.value
 
     def step(self, mode, J, sweep):
         check(lib().ex_als_step(self.h, mode, J, sweep))

@@ -5,6 +5,8 @@
 // over AccumulatorStationary::execute_ALS_step (accumulator_stationary.hpp:139-295).
 #include <algorithm>
 #include <chrono>
+#include <exception>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -1221,6 +1223,58 @@ int ex_als_fit(ex_als* a, double* fit) {
         EX_CHECK(a && fit, EX_ERR_ARG, "null argument");
         EX_CHECK(a->initialized, EX_ERR_STATE, "initialize_ground_truth_for_als() has not been called");
         *fit = als_fit_impl(a);
+    });
+}
+
+// Host threads that enqueue the work of ex_als_run_batch.
+constexpr int kBatchThreads = 8;
+
+int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, double* seconds) {
+    return guarded([&] {
+        EX_CHECK(als && n >= 1, EX_ERR_ARG, "no optimizers");
+        EX_CHECK(rounds >= 0, EX_ERR_ARG, "negative round count");
+        int maxN = 0;
+        for (int t = 0; t < n; t++) {
+            EX_CHECK(als[t], EX_ERR_ARG, "null optimizer");
+            EX_CHECK(als[t]->initialized, EX_ERR_STATE, "initialize_ground_truth_for_als() has not been called");
+            for (int u = 0; u < t; u++)
+                EX_CHECK(als[u]->ctx != als[t]->ctx, EX_ERR_ARG, "each optimizer needs its own context");
+            maxN = std::max(maxN, als[t]->f->N);
+        }
+        for (int t = 0; t < n; t++) {
+            als_resolve_regions(als[t]);
+            EX_HIP(hipStreamSynchronize(als[t]->ctx->c.stream));
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        // host threads enqueue the optimizers' mode steps (optimizer t on
+        // thread t mod T), interleaved across each thread's optimizers: every
+        // context's stream carries its own chain and the streams run side by
+        // side, so the launch work of the host is spread over T threads
+        const int T = std::min(n, kBatchThreads);
+        std::vector<std::thread> pool;
+        std::vector<std::exception_ptr> err((size_t)T);
+        for (int th = 0; th < T; th++)
+            pool.emplace_back([&, th] {
+                try {
+                    for (int64_t round = 1; round <= rounds; round++)
+                        for (int j = 0; j < maxN; j++)
+                            for (int t = th; t < n; t += T) {
+                                ex_als* a = als[t];
+                                if (j >= a->f->N) continue;
+                                EX_HIP(hipSetDevice(a->ctx->c.device));
+                                if (a->exact) exact_step_impl(a, j);
+                                else als_step_impl(a, j, J, (uint64_t)round);
+                            }
+                    for (int t = th; t < n; t += T) EX_HIP(hipStreamSynchronize(als[t]->ctx->c.stream));
+                } catch (...) {
+                    err[(size_t)th] = std::current_exception();
+                }
+            });
+        for (auto& p : pool) p.join();
+        for (auto& e : err)
+            if (e) std::rethrow_exception(e);
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        for (int t = 0; t < n; t++) als_resolve_regions(als[t]);
     });
 }
 

@@ -98,6 +98,24 @@ CONFIGS = {
 }
 
 
+def make_batch(count=256, seed=4, size_range=(500, 50_000), nnz_per_max=20, ranks=(8, 16, 32), scale=1.0):
+    """Independent 3-mode tensors of the batched CP-ARLS-LEV workload: mode
+    sizes ~ U[size_range], nnz = nnz_per_max x the largest mode size at
+    uniform coordinates, values U(0,1) stored FP32, R drawn from `ranks`.
+    Returns [(dims, idx, vals, R, seed_t)]; scale shrinks every size."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for t in range(count):
+        dims = [max(2, int(d * scale)) for d in rng.integers(size_range[0], size_range[1] + 1, size=3)]
+        R = int(rng.choice(ranks))
+        nnz = nnz_per_max * max(dims)
+        st = seed * 100_003 + t
+        idx = coords(dims, nnz, st)
+        vals = values(nnz, st, "uniform_f32")
+        out.append((dims, idx, vals, R, st))
+    return out
+
+
 def make(config, scale=1.0):
     c = dict(CONFIGS[config])
     nnz = int(c["nnz"] * scale)

@@ -186,3 +186,36 @@ def test_four_mode_rank64_shape(ctx):
         fits[prec] = st["fits"][1:]
     print("fp64", fits[abi.EX_FP64], "mixed", fits[abi.EX_MIXED])
     assert abs(fits[abi.EX_FP64][-1] - fits[abi.EX_MIXED][-1]) < 1e-3
+
+
+def test_batched_run_equals_single_runs():
+    """ex_als_run_batch over independent tensors (each optimizer on its own
+    context, the streams side by side) gives exactly the factors that each
+    optimizer reaches alone with ex_als_run."""
+    from rdist_tensor_amd import abi, synthetic
+    batch = synthetic.make_batch(count=6, seed=4, size_range=(300, 3000))
+    results = {}
+    for mode in ("single", "batch"):
+        objs = []
+        for dims, idx, vals, R, st in batch:
+            c = abi.Context(0)
+            T = abi.Tensor(c, idx, vals, dims)
+            F = abi.Factors(c, dims, R)
+            for k, u in enumerate(synthetic.factors(dims, R, st)):
+                F.set(k, u)
+            S = abi.Sampler(c, F, abi.EX_SAMPLER_CP_ARLS_LEV, precision=abi.EX_MIXED, seed=st)
+            A = abi.ALS(c, T, F, S, precision=abi.EX_MIXED)
+            A.init()
+            objs.append((c, T, F, S, A))
+        if mode == "single":
+            for c, T, F, S, A in objs:
+                A.run(3, 2048, 100)
+        else:
+            sec = abi.ALS.run_batch([o[4] for o in objs], 3, 2048)
+            print(f"batch of {len(objs)}: {sec * 1e3:.1f} ms")
+        results[mode] = [[o[2].get(k) for k in range(3)] for o in objs]
+        for c, *_ in objs:
+            c.close()
+    for a, b in zip(results["single"], results["batch"]):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
