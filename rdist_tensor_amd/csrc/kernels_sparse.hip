@@ -264,20 +264,21 @@ __global__ __launch_bounds__(kThreads) void clear_fibers(const uint32_t* __restr
 // Row-owned pass over the nonzeros in row order (sort_lookup.hpp:238-375
 // computes the same sums from the sampled side).  One wavefront per chunk
 // of kScanChunk consecutive nonzeros, four per lane; a nonzero is a hit when
-// its fiber's bit is set.  The hits are taken in order: a row that begins and
-// ends inside the chunk is summed and written to Y.  A row that crosses a
-// chunk boundary leaves partial sums in carry entries: slot 0 of a chunk for
-// the row running in from the left, slot 1 for the row beginning in the chunk
-// and running past its end.  Both slots always carry their row as key (flag
-// 0 when there is no partial), so the carry list is sorted by row and every
+// its fiber's filter bits are set and its map entry carries the step's
+// stamp.  The hits are taken in order: a row that begins and ends inside the
+// chunk is summed and written to Y.  A row that crosses a chunk boundary
+// leaves partial sums in carry entries: slot 0 of a chunk for the row
+// running in from the left, slot 1 for the row beginning in the chunk and
+// running past its end.  Both slots always carry their row as key (flag 0
+// when there is no partial), so the carry list is sorted by row and every
 // crossing row's partials are contiguous in it; seg_level reduces that list
-// the same way.  Fixed summation order, no atomics on Y.  counts[0] += hits,
-// counts[1] += rows with a hit that begin here.
+// the same way.  Fixed summation order, no atomics on Y.  The per-workgroup
+// partials of the hit and head counts go to `counts` (sum_counts adds them).
 constexpr int kScanChunk = 256;
-constexpr int kHitBatch = 8;   // hits whose loads are issued together
 constexpr int kSegBatch = 32;  // carry entries whose loads are issued together
 
-// The filter sits in LDS (64 KB); two 1024-thread workgroups per CU.
+// The filter (64 KB) and the chunk's hit lists sit in LDS; one 1024-thread
+// workgroup per CU.
 constexpr int kScanThreads = 1024;
 
 // Exact MTTKRP source (sort_lookup.hpp:440-495): every nonzero is a hit and
