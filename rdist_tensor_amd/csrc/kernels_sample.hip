@@ -137,6 +137,7 @@ struct StsArgs {
     const double* draws_eig;
     uint32_t* samples;
     double* weights;
+    int vec_leaf;           // leaf rows read as W-wide vectors (R % W == 0, aligned factors)
 };
 
 // Lane exchange inside rows of 16 lanes through DPP (CTRL: quad_perm,
@@ -244,6 +245,108 @@ __device__ __forceinline__ double quad_g(const TN* __restrict__ node, const TN (
     return (double)group_total<G>(acc);
 }
 
+// Sum over aligned sub-groups of SG lanes (SG a power of two, uniform over
+// the wavefront): the first log2(SG) steps of group_total.
+template <typename V>
+__device__ __forceinline__ V sub_total(V v, int SG) {
+    if (SG >= 2) v += dpp_x<0xB1>(v);
+    if (SG >= 4) v += dpp_x<0x4E>(v);
+    if (SG >= 8) v += dpp_x<0x141>(v);
+    if (SG >= 16) v += dpp_x<0x140>(v);
+    if (SG >= 32) v += __shfl_xor(v, 16, 64);
+    if (SG >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// Passes over a leaf whose loads are issued together (fewer in FP64, where
+// more would cost occupancy).
+template <typename TN>
+constexpr int leaf_batch() { return sizeof(TN) == 4 ? 8 : 2; }
+
+// q2[r] = (U_blk[r] . x)^2 for r < F (0 for r >= rows) of the leaf block blk
+// (F rows of R, row-major).  SG lanes per row (the power of two covering its
+// VL-wide vectors, at most G), G / SG rows per pass; the loads of
+// leaf_batch passes are in flight together.  VL = W needs R % W == 0 and a
+// W-aligned factor; VL = 1 reads single elements.
+template <typename TN, int G, int VL>
+__device__ __forceinline__ void leaf_q2(const TN* __restrict__ blk, int rows, int F, int R, const TN* x,
+                                        double* q2, int l) {
+    const int nvec = R / VL;
+    int SG = 1;
+    while (SG < nvec && SG < G) SG *= 2;
+    const int RPS = G / SG, sr = l / SG, sl = l % SG;
+    const int nu = (nvec + SG - 1) / SG;
+    constexpr int kLeafBatch = leaf_batch<TN>();
+    for (int r0 = 0; r0 < F; r0 += RPS * kLeafBatch) {
+        TN part[kLeafBatch];
+#pragma unroll
+        for (int t = 0; t < kLeafBatch; t++) part[t] = (TN)0;
+        for (int u = 0; u < nu; u++) {
+            const int v = sl + SG * u;
+            TN xv[VL];
+#pragma unroll
+            for (int i = 0; i < VL; i++) xv[i] = v < nvec ? x[v * VL + i] : (TN)0;
+            TN d[kLeafBatch][VL];
+#pragma unroll
+            for (int t = 0; t < kLeafBatch; t++) {
+                const int r = r0 + t * RPS + sr;
+                const bool ok = r < rows && v < nvec;
+                if constexpr (VL == 1) {
+                    d[t][0] = ok ? blk[(int64_t)r * R + v] : (TN)0;
+                } else {
+                    typedef typename Vec<TN>::type VT;
+                    VT e;
+                    if (ok) e = *reinterpret_cast<const VT*>(blk + (int64_t)r * R + v * VL);
+#pragma unroll
+                    for (int i = 0; i < VL; i++) d[t][i] = ok ? e[i] : (TN)0;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < kLeafBatch; t++)
+#pragma unroll
+                for (int i = 0; i < VL; i++) part[t] += d[t][i] * xv[i];
+        }
+#pragma unroll
+        for (int t = 0; t < kLeafBatch; t++) {
+            const TN q = sub_total(part[t], SG);
+            const int r = r0 + t * RPS + sr;
+            if (sl == 0 && r < F) q2[r] = (double)q * (double)q;
+        }
+    }
+}
+
+// Position in the leaf of a draw at fraction frac of its mass: the first r
+// < F - 1 with frac < (q2[0] + ... + q2[r]) / (q2[0] + ... + q2[F-1]), else
+// F - 1.  Every lane of the group forms the same sequential FP64 prefix and
+// keeps the entries r = k G + l.
+template <int G, int RMAX>
+__device__ __forceinline__ int leaf_pick(const double* q2, int F, double frac, int l, int grp) {
+    constexpr int KMAX = (RMAX + G - 1) / G;
+    double mine[KMAX];
+    double run = 0.0;
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        mine[k] = 0.0;
+#pragma unroll
+        for (int i = 0; i < G; i++) {
+            const int r = k * G + i;
+            if (r < F) {
+                run += q2[r];
+                if (i == l) mine[k] = run;
+            }
+        }
+    }
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
+#pragma unroll
+    for (int k = 0; k < KMAX; k++) {
+        const int r = k * G + l;
+        const bool cond = r < F - 1 && frac < mine[k] / run;
+        const uint64_t bits = (__ballot(cond) >> (grp * G)) & gmask;
+        if (bits) return k * G + __ffsll((unsigned long long)bits) - 1;
+    }
+    return F - 1;
+}
+
 // Root-to-leaf descent of one heap tree (partition_tree.hpp:226-307): masses
 // and cutoffs in FP64, left if u <= low + m_left / m_root.
 template <typename TN, int G, int NV>
@@ -326,31 +429,12 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
         const double frac = (ur - low) / (high - low);
         // q_r = U[leaf F + r] . sh for the leaf's real rows (0 for the rest),
         // q_r^2 into LDS; the cumulative sums run in row order
-        for (int r = l; r < F; r += G) {
-            TN q = 0;
-            if (r < rows) {
-                const TN* ur_row = rows_k + (leaf * F + r) * R;
-                for (int b = 0; b < R; b++) q += ur_row[b] * xs[b];
-            }
-            const double qd = (double)q;
-            q2[r] = qd * qd;
-        }
+        const TN* blk = rows_k + leaf * F * R;
+        if (a.vec_leaf) leaf_q2<TN, G, W>(blk, rows, F, R, xs, q2, l);
+        else leaf_q2<TN, G, 1>(blk, rows, F, R, xs, q2, l);
         wave_lds_sync();
-        double total = 0.0;
-        for (int r = 0; r < F; r++) total += q2[r];
-        int res = F - 1;
-        for (int r0 = 0; r0 < F; r0 += G) {
-            const int r = r0 + l;
-            bool cond = false;
-            if (r < F - 1) {
-                double cum = 0.0;
-                for (int r2 = 0; r2 <= r; r2++) cum += q2[r2];
-                cond = frac < cum / total;
-            }
-            const uint64_t bits = (__ballot(cond) >> (grp * G)) & (G == 64 ? ~0ull : ((1ull << G) - 1));
-            if (bits) { res = r0 + __ffsll((unsigned long long)bits) - 1; break; }
-        }
-        if (res >= rows) res = rows - 1;
+        const int res0 = leaf_pick<G, RMAX>(q2, F, frac, l, grp);
+        const int res = res0 < rows ? res0 : rows - 1;
         const int64_t row = leaf * F + res;
         for (int b = l; b < R; b += G) hs[b] *= rows_k[row * R + b];
         wave_lds_sync();
@@ -546,6 +630,12 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
     a.seed = seed; a.sweep = sweep;
     a.draws_row = draws_row; a.draws_eig = draws_eig;
     a.samples = samples; a.weights = weights;
+    // vector reads of leaf rows: R a multiple of the vector width and every
+    // drawn factor aligned to it
+    const int vw = node_bytes == 8 ? 2 : 4;
+    a.vec_leaf = R % vw == 0;
+    for (int k = 0; k < N; k++)
+        if (k != j && ((uintptr_t)modes[k].rows % (size_t)(vw * node_bytes)) != 0) a.vec_leaf = 0;
 #define DRAW_LAUNCH(TN, G, NV)                                                                         \
     hipLaunchKernelGGL((sts_draw_kernel<TN, G, NV>), dim3(grid_for(J, (kDrawThreads / 64) * (64 / G), 1LL << 30)), \
                        dim3(kDrawThreads), 0, s, a)
