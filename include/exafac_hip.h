@@ -221,6 +221,10 @@ int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G);
 int ex_eigh(ex_ctx* ctx, const double* A, int64_t R, double* lam, double* V);
 /* compute_pinv_square (common.h:405-443) with absolute eigenvalue tolerance tol. */
 int ex_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol, double* out);
+/* parallel_dsymm (common.h:696-721): out = Y G for Y (m x R) and a symmetric
+ * G (R x R, full storage), on the kernel the factor update runs at
+ * `precision` (EX_MIXED: FP32 inputs and products, on MFMA for R <= 32). */
+int ex_dsymm(ex_ctx* ctx, int precision, const double* G, const double* Y, int64_t m, int64_t R, double* out);
 /* deduplicate_design_matrix (als_optimizer.hpp:130-206): returns *Ju unique
  * tuples (ignoring column `mode`) in ascending order with weight x count. */
 int ex_dedup(ex_ctx* ctx, const uint32_t* samples, const double* weights, int64_t J, int N,

@@ -70,6 +70,7 @@ SIGNATURES = {
     "ex_gram": [_vp, _d, C.c_int64, C.c_int64, _d],
     "ex_eigh": [_vp, _d, C.c_int64, _d, _d],
     "ex_pinv": [_vp, _d, C.c_int64, C.c_double, _d],
+    "ex_dsymm": [_vp, C.c_int, _d, _d, C.c_int64, C.c_int64, _d],
     "ex_dedup": [_vp, _u32, _d, C.c_int64, C.c_int, C.c_int, _i64, _u32, _d, _i64],
 }
 
@@ -201,6 +202,15 @@ class Context:
         R = M.shape[0]
         out = np.empty((R, R))
         check(lib().ex_pinv(self.h, _p(M), R, tol, _p(out)))
+        return out
+
+    def dsymm(self, G, Y, precision=EX_FP64):
+        """out = Y G (the factor update's product) at the given precision."""
+        G = f64(G)
+        Y = f64(Y)
+        m, R = Y.shape
+        out = np.empty((m, R))
+        check(lib().ex_dsymm(self.h, precision, _p(G), _p(Y), m, R, _p(out)))
         return out
 
     def dedup(self, samples, weights, mode, dims):

@@ -1371,6 +1371,37 @@ int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G) {
     });
 }
 
+int ex_dsymm(ex_ctx* ctx, int precision, const double* G, const double* Y, int64_t m, int64_t R, double* out) {
+    return guarded([&] {
+        EX_CHECK(ctx && G && out && (m == 0 || Y), EX_ERR_ARG, "null argument");
+        EX_CHECK(R >= 1 && R <= kMaxRank, EX_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+        EX_CHECK(m >= 0, EX_ERR_ARG, "negative row count");
+        EX_CHECK(precision == EX_FP64 || precision == EX_MIXED, EX_ERR_ARG, "precision must be EX_FP64 or EX_MIXED");
+        hipStream_t st = ctx->c.stream;
+        const size_t n = (size_t)m * R, r2 = (size_t)R * R;
+        DevBuf dG, dY, dO;
+        if (precision == EX_FP64) {
+            dG.ensure(r2 * 8);
+            dY.ensure((n > 0 ? n : 1) * 8);
+            dO.ensure((n > 0 ? n : 1) * 8);
+            h2d(st, dG.p, G, r2 * 8);
+            h2d(st, dY.p, Y, n * 8);
+            k::dsymm(st, dG.as<double>(), dY.as<double>(), m, (int)R, dO.as<double>());
+            d2h(st, out, dO.p, n * 8);
+        } else {
+            std::vector<float> g32(G, G + r2), y32(Y, Y + n), o32(n);
+            dG.ensure(r2 * 4);
+            dY.ensure((n > 0 ? n : 1) * 4);
+            dO.ensure((n > 0 ? n : 1) * 4);
+            h2d(st, dG.p, g32.data(), r2 * 4);
+            h2d(st, dY.p, y32.data(), n * 4);
+            k::dsymm32(st, dG.as<float>(), dY.as<float>(), m, (int)R, dO.as<float>());
+            d2h(st, o32.data(), dO.p, n * 4);
+            for (size_t e = 0; e < n; e++) out[e] = o32[e];
+        }
+    });
+}
+
 int ex_eigh(ex_ctx* ctx, const double* A, int64_t R, double* lam, double* V) {
     return guarded([&] {
         EX_CHECK(ctx && A && lam && V, EX_ERR_ARG, "null argument");

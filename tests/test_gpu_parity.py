@@ -55,6 +55,29 @@ def test_pinv(ctx, orc, R):
         assert rel_err(ctx.pinv(M), orc.pinv(M)) < 1e-11
 
 
+@pytest.mark.parametrize("R", [1, 4, 8, 16, 25, 32, 48, 64])
+def test_dsymm(ctx, orc, R):
+    """The factor update's product out = Y G: FP64 against the oracle's
+    parallel_dsymm; mixed (FP32, MFMA for R <= 32) against Y G formed in FP64
+    from the FP32-rounded inputs, elementwise within the FP32 rounding bound
+    of an R-term sum, with a non-symmetric G (the mixed update multiplies by
+    G+ diag(1/sigma)) so a transposed operand fails."""
+    from rdist_tensor_amd import abi
+    rng = np.random.default_rng(300 + R)
+    B = rng.standard_normal((R + 3, R))
+    G = B.T @ B
+    Gn = G * rng.uniform(0.5, 2.0, R)[None, :]
+    eps32 = np.finfo(np.float32).eps
+    for m in [1, 31, 33, 1000, 70001]:
+        Y = rng.standard_normal((m, R))
+        assert rel_err(ctx.dsymm(G, Y), orc.dsymm(G, Y)) < REL
+        Y32 = Y.astype(np.float32).astype(np.float64)
+        G32 = Gn.astype(np.float32).astype(np.float64)
+        got = ctx.dsymm(Gn, Y, precision=abi.EX_MIXED)
+        bound = (R + 2) * eps32 * (np.abs(Y32) @ np.abs(G32)) + 1e-30
+        assert np.all(np.abs(got - Y32 @ G32) <= bound), (m, R)
+
+
 def test_dedup(ctx, orc):
     rng = np.random.default_rng(5)
     dims = [50, 7, 300, 2]
