@@ -12,7 +12,7 @@ so both final fits are "after --steps sweeps").  `value` is the mixed path's
 sweeps/s; the FP64 path's figures, the ratio and both final fits sit beside
 it.
 
-Prints one JSON line (the driver's contract) with `roofline` (the dominant
+Prints one JSON line (metric, value, unit, timing, config) with `roofline` (the dominant
 kernel of the reported path, timed live with HIP events on the library's
 stream) and `cpu_baseline` (the oracle, a bounded sample of the same
 workload on the host cores).

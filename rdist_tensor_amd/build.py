@@ -26,7 +26,8 @@ HEADERS = ["common.hpp", "device.hpp", "dsymm.hpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-    # packed FP32 instructions stay off
+    # packed FP32 instructions (v_pk_*_f32) stay off: some of their forms have
+    # given wrong results on this hardware without an error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
     "-I", os.path.join(ROOT, "include"),
