@@ -213,6 +213,18 @@ int ex_als_fit(ex_als* a, double* fit);
  * length needed (including the NUL). */
 int ex_als_stats_json(ex_als* a, char* buf, size_t* len);
 
+/* ---- tensor files (host only; no GPU call) -------------------------------- */
+/* FROSTT .tns text: one nonzero per line, N 1-based coordinates then the
+ * value (blank and '#' lines skipped).  ex_tns_scan counts the nonzeros and
+ * modes (N = fields of the first data line - 1, as
+ * tensor_io/process_frostt_tensor.cpp:29-40 does); ex_tns_read fills
+ * coords (nnz x N, raw as written), vals and the per-mode minimum / maximum,
+ * the MIN_MODE_SET / MAX_MODE_SET the converter stores
+ * (process_frostt_tensor.cpp:115-165).  Errors name the offending line. */
+int ex_tns_scan(const char* path, int64_t* nnz, int* nmodes);
+int ex_tns_read(const char* path, int64_t nnz, int nmodes, uint64_t* coords, double* vals, uint64_t* min_idx,
+                uint64_t* max_idx);
+
 /* ---- dense R x R kernels (parity hooks; host in/out) ----------------------- */
 /* compute_gram (common.h:363-403): G = A^T A, A is m x R. */
 int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G);

@@ -71,6 +71,9 @@ SIGNATURES = {
     "ex_eigh": [_vp, _d, C.c_int64, _d, _d],
     "ex_pinv": [_vp, _d, C.c_int64, C.c_double, _d],
     "ex_dsymm": [_vp, C.c_int, _d, _d, C.c_int64, C.c_int64, _d],
+    "ex_tns_scan": [C.c_char_p, _i64, C.POINTER(C.c_int)],
+    "ex_tns_read": [C.c_char_p, C.c_int64, C.c_int, C.POINTER(C.c_uint64), _d, C.POINTER(C.c_uint64),
+                    C.POINTER(C.c_uint64)],
     "ex_dedup": [_vp, _u32, _d, C.c_int64, C.c_int, C.c_int, _i64, _u32, _d, _i64],
 }
 

@@ -21,7 +21,8 @@ LIB = os.path.join(PKG, "libexafac_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["api.hip", "kernels_dense.hip", "kernels_sample.hip", "kernels_sparse.hip", "kernels_mixed.hip", "sort.hip"]
+HIP_SOURCES = ["api.hip", "kernels_dense.hip", "kernels_sample.hip", "kernels_sparse.hip", "kernels_mixed.hip", "sort.hip",
+               "tensor_io.hip"]
 HEADERS = ["common.hpp", "device.hpp", "dsymm.hpp"]
 
 HIP_FLAGS = [

@@ -366,7 +366,8 @@ PYBIND11_MODULE(py_module, m) {
         .def("set_factor", &LowRankTensor::set_factor)
         .def("get_sigma", &LowRankTensor::get_sigma)
         .def("set_sigma", &LowRankTensor::set_sigma)
-        .def_readonly("rank", &LowRankTensor::rank);
+        .def_readonly("rank", &LowRankTensor::rank)
+        .def_property_readonly("dim", &LowRankTensor::dim);
     py::class_<SparseTensor>(m, "SparseTensor")
         .def(py::init<TensorGrid&, py::array_t<uint32_t, py::array::c_style | py::array::forcecast>,
                       py::array_t<double, py::array::c_style | py::array::forcecast>, std::string>(),

@@ -1371,6 +1371,26 @@ int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G) {
     });
 }
 
+int ex_tns_scan(const char* path, int64_t* nnz, int* nmodes) {
+    return guarded([&] {
+        EX_CHECK(path && nnz && nmodes, EX_ERR_ARG, "null argument");
+        int64_t n = 0;
+        int N = 0;
+        tns_scan(path, n, N);
+        *nnz = n;
+        *nmodes = N;
+    });
+}
+
+int ex_tns_read(const char* path, int64_t nnz, int nmodes, uint64_t* coords, double* vals, uint64_t* min_idx,
+                uint64_t* max_idx) {
+    return guarded([&] {
+        EX_CHECK(path && min_idx && max_idx && (nnz == 0 || (coords && vals)), EX_ERR_ARG, "null argument");
+        EX_CHECK(nnz >= 0 && nmodes >= 1 && nmodes <= kMaxModes, EX_ERR_ARG, "bad nonzero or mode count");
+        tns_read(path, nnz, nmodes, coords, vals, min_idx, max_idx);
+    });
+}
+
 int ex_dsymm(ex_ctx* ctx, int precision, const double* G, const double* Y, int64_t m, int64_t R, double* out) {
     return guarded([&] {
         EX_CHECK(ctx && G && out && (m == 0 || Y), EX_ERR_ARG, "null argument");

@@ -242,6 +242,12 @@ void to_f64(hipStream_t s, const float* a, int64_t n, double* b);
 void design32(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
               const float* const* U, float* dg, float* dm, const uint64_t* Ju_dev = nullptr);
 void dsymm32(hipStream_t s, const float* G, const float* Y, int64_t m, int R, float* out);
+}  // namespace k
+// FROSTT .tns ingest (host): line count and mode count, then the raw
+// coordinates (nnz x N, as written), values and per-mode minimum / maximum.
+void tns_scan(const char* path, int64_t& nnz, int& N);
+void tns_read(const char* path, int64_t nnz, int N, uint64_t* coords, double* vals, uint64_t* mins, uint64_t* maxs);
+namespace k {
 void renormalize32(hipStream_t s, float* U, int64_t m, int R, double* sigma, DevBuf& part);
 // sigma_c = ||(Y Gi)[:, c]|| from GY = Y^T Y; Ms = Gi diag(1 / sigma) (FP32)
 void sigma_from_gram(hipStream_t s, const double* GY, const double* Gi, int R, double* sigma, float* Ms);

@@ -59,3 +59,47 @@ def test_decompose_sequence_matches_oracle(orc, algorithm, distribution):
     assert stats["sum_nonzeros_iterated"] == ost["nonzeros_iterated"]
     for k in range(3):
         np.testing.assert_allclose(lrt.get_factor(k), oa.U[k], rtol=0, atol=1e-9)
+
+
+def test_tensor_file_reader_and_factor_checkpoint(tmp_path, orc):
+    """A FROSTT .tns file (1-based, minimum above 1) through the converter and
+    DistSparseTensorE (coordinates shifted by the minimum, log_count
+    preprocessing, sparse_tensor_e.py:63-117) into the drop-in SparseTensor;
+    its ALS fits equal the oracle's on the shifted arrays.  Then a factor
+    checkpoint written and read back restores factors and column norms
+    bitwise."""
+    from rdist_tensor_amd import tensor_io
+    m = _module()
+    rng = np.random.default_rng(90)
+    dims = [120, 90, 60]
+    idx, vals = random_tensor(rng, dims, 6000)
+    raw = idx.astype(np.int64) + np.array([3, 1, 7])      # on disk: 1-based, shifted minima
+    vals = rng.integers(1, 20, len(vals)).astype(np.float64)
+    p = tmp_path / "x.tns"
+    with open(p, "w") as f:
+        for c, v in zip(raw, vals):
+            f.write(f"{c[0]} {c[1]} {c[2]} {v:.1f}\n")
+    conv = tensor_io.tns_to_tensor_file(p)
+    t = tensor_io.DistSparseTensorE(conv, preprocessing="log_count")
+    shift = raw - raw.min(0)
+    assert np.array_equal(t.tensor_idxs, shift.astype(np.uint32))
+    assert list(t.tensor_dims) == list(raw.max(0) - raw.min(0) + 1)
+    lv = np.log(vals + 1)
+    assert abs(t.sparse_tensor.tensor_norm - np.sqrt(np.sum(lv * lv))) < 1e-12 * np.sqrt(np.sum(lv * lv))
+    lrt = m.LowRankTensor(6, t.tensor_grid)
+    lrt.initialize_factors_gaussian_random(3)
+    U0 = [lrt.get_factor(k).copy() for k in range(3)]
+    opt = m.AccumulatorStationary(t.sparse_tensor, lrt, m.STS_CP(lrt, seed=2))
+    opt.initialize_ground_truth_for_als()
+    opt.execute_ALS_rounds(2, 1024, 1)
+    fits = json.loads(opt.get_statistics_json())["fits"][1:]
+    oa = orc.ALS(shift.astype(np.uint32), lv, [int(d) for d in t.tensor_dims], U0, "sts", seed=2)
+    ofits, _ = oa.run(2, 1024, 1)
+    np.testing.assert_allclose(fits, ofits, atol=1e-9)
+    ck = tensor_io.write_factors(lrt, tmp_path / "factors.npz")
+    other = m.LowRankTensor(6, t.tensor_grid)
+    other.initialize_factors_gaussian_random(9)
+    tensor_io.read_factors(other, ck)
+    for k in range(3):
+        assert np.array_equal(other.get_factor(k), lrt.get_factor(k))
+    assert np.array_equal(other.get_sigma(), lrt.get_sigma())
