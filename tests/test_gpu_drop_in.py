@@ -103,3 +103,29 @@ def test_tensor_file_reader_and_factor_checkpoint(tmp_path, orc):
     for k in range(3):
         assert np.array_equal(other.get_factor(k), lrt.get_factor(k))
     assert np.array_equal(other.get_sigma(), lrt.get_sigma())
+
+
+def test_synthetic_block_tensor():
+    """SparseTensor(tensor_grid, I, N, Q, seed) (sparse_tensor.hpp:102-215):
+    Q disjoint all-ones blocks, so the norm is sqrt(nnz), nnz is near its
+    expectation I^3 / Q^2, a seed reproduces the tensor, and exact ALS at
+    rank Q fits it closely."""
+    m = _module()
+    I, Q = 60, 4
+    grid = m.Grid(np.ones(3, dtype=np.int32))
+    tg = m.TensorGrid(np.array([I, I, I], dtype=np.int32), grid)
+    st = m.SparseTensor(tg, I, 3, Q, seed=5)
+    again = m.SparseTensor(tg, I, 3, Q, seed=5)
+    assert st.nnz == again.nnz > 0
+    assert abs(st.tensor_norm - np.sqrt(st.nnz)) < 1e-12 * np.sqrt(st.nnz)
+    assert 0.3 * I ** 3 / Q ** 2 < st.nnz < 3.0 * I ** 3 / Q ** 2
+    with pytest.raises(ValueError):
+        m.SparseTensor(tg, I, 4, Q, 5)
+    lrt = m.LowRankTensor(Q, tg)
+    lrt.initialize_factors_gaussian_random(1)
+    opt = m.ExactALS(st, lrt)
+    opt.initialize_ground_truth_for_als()
+    opt.execute_ALS_rounds(30, 0, 10)
+    fits = json.loads(opt.get_statistics_json())["fits"]
+    print(f"block tensor nnz {st.nnz}, exact-ALS fits {fits}")
+    assert fits[-1] > 0.5
