@@ -320,7 +320,7 @@ __device__ __forceinline__ void leaf_q2(const TN* __restrict__ blk, int rows, in
 // F - 1.  Every lane of the group forms the same sequential FP64 prefix and
 // keeps the entries r = k G + l.
 template <int G, int RMAX>
-__device__ __forceinline__ int leaf_pick(const double* q2, int F, double frac, int l, int grp) {
+__device__ __forceinline__ int leaf_pick(const double* q2, int F, double frac, int l, int grp, double& total) {
     constexpr int KMAX = (RMAX + G - 1) / G;
     double mine[KMAX];
     double run = 0.0;
@@ -336,6 +336,7 @@ __device__ __forceinline__ int leaf_pick(const double* q2, int F, double frac, i
             }
         }
     }
+    total = run;
     const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
 #pragma unroll
     for (int k = 0; k < KMAX; k++) {
@@ -404,6 +405,7 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
     wave_lds_sync();
     TN xx[NV * W];
     uint32_t eig_idx = 0;
+    bool dropped = false;
     for (int k = 0; k < N; k++) {
         if (k == j) continue;
         const k::StsModeArgs& md = a.m[k];
@@ -433,7 +435,12 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
         if (a.vec_leaf) leaf_q2<TN, G, W>(blk, rows, F, R, xs, q2, l);
         else leaf_q2<TN, G, 1>(blk, rows, F, R, xs, q2, l);
         wave_lds_sync();
-        const int res0 = leaf_pick<G, RMAX>(q2, F, frac, l, grp);
+        double leaf_mass;
+        const int res0 = leaf_pick<G, RMAX>(q2, F, frac, l, grp, leaf_mass);
+        // FP32 trees: a rounding residue between a node and its left child
+        // can route a draw into a leaf of zero mass; such a draw is dropped
+        // (weight 0) instead of carrying an infinite weight
+        if constexpr (sizeof(TN) == 4) dropped |= !(leaf_mass > 0.0);
         const int res = res0 < rows ? res0 : rows - 1;
         const int64_t row = leaf * F + res;
         for (int b = l; b < R; b += G) hs[b] *= rows_k[row * R + b];
@@ -450,7 +457,7 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
     q = group_total<G>(q);
     if (l == 0) {
         a.samples[s * N + j] = eig_idx;
-        a.weights[s] = (double)R / (q * (double)a.J);
+        a.weights[s] = dropped ? 0.0 : (double)R / (q * (double)a.J);
     }
 }
 
