@@ -190,9 +190,9 @@ __global__ __launch_bounds__(kThreads) void leaf_gram_bf16(const float* __restri
 }
 
 // ---------------------------------------------------------------- FP64 SPD solves
-// One workgroup of 256 threads holds an R x R matrix in registers: thread t
-// owns column j = t % R and rows g, g + GR, g + 2 GR, ... (g = t / R,
-// GR = 256 / R row groups, at most RPT rows).  Each elimination step
+// One workgroup of T threads (256, or 1024 for R <= 32) holds an R x R
+// matrix in registers: thread t owns column j = t % R and rows g, g + GR,
+// g + 2 GR, ... (g = t / R, GR = T / R row groups, at most RPT rows).  Each elimination step
 // publishes one column (and row) through LDS; the buffers alternate between
 // steps, so one barrier per step suffices.
 constexpr int kSpdThreads = 256;
@@ -207,7 +207,7 @@ struct SpdLayout {
     int j, g, GR;
     bool on;
     __device__ SpdLayout(int R) {
-        GR = kSpdThreads / R;
+        GR = (int)blockDim.x / R;
         j = threadIdx.x % R;
         g = threadIdx.x / R;
         on = g < GR;
@@ -326,8 +326,8 @@ __device__ void spd_cholesky_regs(SpdShared& sh, const SpdLayout& L, double (&a)
 
 // Inverse of a symmetric positive semidefinite R x R matrix (upper triangle
 // of M read); pivots at or below tol_rel * (largest diagonal entry) dropped.
-template <int RPT>
-__global__ __launch_bounds__(kSpdThreads) void spd_inverse_kernel(const double* __restrict__ M, int R, double tol_rel,
+template <int RPT, int T = kSpdThreads>
+__global__ __launch_bounds__(T) void spd_inverse_kernel(const double* __restrict__ M, int R, double tol_rel,
                                                                   double* __restrict__ out, float* __restrict__ out32) {
     __shared__ SpdShared sh;
     const SpdLayout L(R);
@@ -355,8 +355,8 @@ __global__ __launch_bounds__(kSpdThreads) void spd_inverse_kernel(const double* 
 // of slot 0, slots k rows from the Cholesky factor of (later Grams (.) slot 0).
 // Any such factorisation gives the same row-sampling distribution.  Every
 // thread reads and writes only its own elements of each slot.
-template <int RPT>
-__global__ __launch_bounds__(kSpdThreads) void sts_compute_m_mixed(int N, int j, int R,
+template <int RPT, int T = kSpdThreads>
+__global__ __launch_bounds__(T) void sts_compute_m_mixed(int N, int j, int R,
                                                                    const double* __restrict__ grams,
                                                                    double* __restrict__ M) {
     __shared__ SpdShared sh;
@@ -565,13 +565,18 @@ void tree_build_f32(hipStream_t s, const float* U, const TreeGeom& g, int R, flo
         else hipLaunchKernelGGL((KERNEL<16>), dim3(1), dim3(kSpdThreads), 0, STREAM, __VA_ARGS__);          \
     } while (0)
 
+// R <= 32: one element per thread (1024 threads), else the 256-thread tiling.
 void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, float* out32) {
-    RM_DISPATCH(R, spd_inverse_kernel, s, M, R, tol_rel, out, out32);
+    if (R <= 32) hipLaunchKernelGGL((spd_inverse_kernel<1, 1024>), dim3(1), dim3(1024), 0, s, M, R, tol_rel, out, out32);
+    else RM_DISPATCH(R, spd_inverse_kernel, s, M, R, tol_rel, out, out32);
     EX_HIP(hipGetLastError());
 }
 
 void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M) {
-    RM_DISPATCH(R, sts_compute_m_mixed, s, N, j, R, grams, M);
+    // R <= 32: one element per thread (1024 threads; faster than 256 threads
+    // with five rows each, and than one wavefront with sixteen)
+    if (R <= 32) hipLaunchKernelGGL((sts_compute_m_mixed<1, 1024>), dim3(1), dim3(1024), 0, s, N, j, R, grams, M);
+    else RM_DISPATCH(R, sts_compute_m_mixed, s, N, j, R, grams, M);
     EX_HIP(hipGetLastError());
 }
 
