@@ -453,28 +453,33 @@ __global__ __launch_bounds__(kThreads) void design_f32(const uint32_t* __restric
 
 // Column norms of U = Y Gi from the Gram of Y: sigma_c^2 = (Gi (Y^T Y) Gi)_cc;
 // Ms = Gi diag(1 / sigma) in FP32, so that Y Ms is U with unit columns.
-__global__ __launch_bounds__(kThreads) void sigma_from_gram_kernel(const double* __restrict__ GY,
-                                                                   const double* __restrict__ Gi, int R,
-                                                                   double* __restrict__ sigma,
-                                                                   float* __restrict__ Ms) {
-    __shared__ double t[kMaxRank * kMaxRank];   // GY Gi
-    __shared__ double sg[kMaxRank];
-    for (int o = threadIdx.x; o < R * R; o += blockDim.x) {
+__global__ __launch_bounds__(1024) void sigma_from_gram_kernel(const double* __restrict__ GY,
+                                                              const double* __restrict__ Gi, int R,
+                                                              double* __restrict__ sigma, float* __restrict__ Ms) {
+    __shared__ double gi[kMaxRank * kMaxRank];
+    __shared__ double t[kMaxRank * kMaxRank];   // Gi[a][c] (GY Gi)[a][c], then sigma
+    const int R2 = R * R;
+    for (int o = threadIdx.x; o < R2; o += blockDim.x) gi[o] = Gi[o];
+    __syncthreads();
+    for (int o = threadIdx.x; o < R2; o += blockDim.x) {
         const int a = o / R, c = o % R;
         double acc = 0.0;
-        for (int b = 0; b < R; b++) acc += GY[a * R + b] * Gi[b * R + c];
-        t[o] = acc;
+        for (int b = 0; b < R; b++) acc += GY[a * R + b] * gi[b * R + c];
+        t[o] = gi[a * R + c] * acc;
     }
     __syncthreads();
+    double sg = 0.0;
     if (threadIdx.x < R) {
         const int c = threadIdx.x;
         double acc = 0.0;
-        for (int a = 0; a < R; a++) acc += Gi[a * R + c] * t[a * R + c];
-        sg[c] = sqrt(acc > 0.0 ? acc : 0.0);
-        sigma[c] = sg[c];
+        for (int a = 0; a < R; a++) acc += t[a * R + c];
+        sg = sqrt(acc > 0.0 ? acc : 0.0);
+        sigma[c] = sg;
     }
     __syncthreads();
-    for (int o = threadIdx.x; o < R * R; o += blockDim.x) Ms[o] = (float)(Gi[o] / sg[o % R]);
+    if (threadIdx.x < R) t[threadIdx.x] = sg;
+    __syncthreads();
+    for (int o = threadIdx.x; o < R2; o += blockDim.x) Ms[o] = (float)(gi[o] / t[o % R]);
 }
 
 // Column sums of squares of an FP32 matrix, accumulated in FP64.
@@ -627,7 +632,7 @@ void renormalize32(hipStream_t s, float* U, int64_t m, int R, double* sigma, Dev
 }
 
 void sigma_from_gram(hipStream_t s, const double* GY, const double* Gi, int R, double* sigma, float* Ms) {
-    hipLaunchKernelGGL(sigma_from_gram_kernel, dim3(1), dim3(kThreads), 0, s, GY, Gi, R, sigma, Ms);
+    hipLaunchKernelGGL(sigma_from_gram_kernel, dim3(1), dim3(1024), 0, s, GY, Gi, R, sigma, Ms);
     EX_HIP(hipGetLastError());
 }
 
