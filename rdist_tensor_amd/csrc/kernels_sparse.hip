@@ -461,8 +461,11 @@ struct GroupRows {
 template <typename V>
 constexpr int group_vw() { return sizeof(V) == 4 ? 4 : 2; }
 
-// Grouped steps whose loads are issued together.
-constexpr int kStepBatch = 4;
+// Grouped steps whose loads are issued together: 8 for the FP32 exact
+// MTTKRP (every nonzero a hit, factor rows behind each), 4 otherwise (more
+// cost the sampled row pass its registers).
+template <typename V, bool kExact>
+constexpr int step_batch() { return (kExact && sizeof(V) == 4) ? 8 : 4; }
 
 // Sums each wavefront's hit and head counts per workgroup into
 // part[2 * blockIdx.x + {0, 1}] (no atomics: many workgroups adding to one
@@ -666,7 +669,7 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
             nh += (uint32_t)__popcll(mask);
         }
         wave_lds_sync();
-        if (nh > 0) st.template run<kStepBatch>(0, nh, meta, pre_of, row_of);
+        if (nh > 0) st.template run<step_batch<V, kExact>()>(0, nh, meta, pre_of, row_of);
         st.finish(cflag, ckey);
         wave_lds_sync();
         whits += nh;
@@ -901,7 +904,7 @@ __global__ __launch_bounds__(kBitThreads) void hit_scan_reduce(const uint32_t* _
         st.last_row = rrow[hits[e - 1]];
         st.first_cont = b > 0 && rrow[hits[b - 1]] == st.first_row;
         st.continues = e < H && rrow[hits[e]] == st.last_row;
-        st.template run<kStepBatch>(b, e, meta, pre_of, row_of);
+        st.template run<step_batch<V, false>()>(b, e, meta, pre_of, row_of);
         st.finish(cflag, ckey);
         whits += (unsigned long long)(e - b);
         wheads += st.nheads;
