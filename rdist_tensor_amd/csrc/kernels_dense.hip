@@ -40,11 +40,6 @@ __global__ __launch_bounds__(kThreads) void gram_final(const double* __restrict_
 // are added in wave order in LDS and the block writes its upper triangle.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// Chunks whose loads are in flight together (fewer for wide tiles, whose
-// operands take more registers).
-template <int NT>
-constexpr int gram_batch() { return NT == 1 ? 4 : (NT == 2 ? 2 : 1); }
-
 template <typename TA, int NT>
 __global__ __launch_bounds__(kThreads) void gram_mfma64_partial(const TA* __restrict__ A, int64_t mb,
                                                                 const uint64_t* __restrict__ m_dev, int R,
@@ -59,32 +54,24 @@ __global__ __launch_bounds__(kThreads) void gram_mfma64_partial(const TA* __rest
 #pragma unroll
         for (int tb = 0; tb < NT; tb++) acc[ta][tb] = f64x4{0.0, 0.0, 0.0, 0.0};
     const int64_t nchunks = (m + 15) / 16;
-    const int64_t stride = (int64_t)gridDim.x * 4;
-    constexpr int kGramBatch = gram_batch<NT>();
-    // kGramBatch chunks' loads in flight together, then their products
-    for (int64_t ch0 = (int64_t)blockIdx.x * 4 + wave; ch0 < nchunks; ch0 += kGramBatch * stride) {
-        double x[kGramBatch][4][NT];
+    for (int64_t ch = (int64_t)blockIdx.x * 4 + wave; ch < nchunks; ch += (int64_t)gridDim.x * 4) {
+        double x[4][NT];
 #pragma unroll
-        for (int u = 0; u < kGramBatch; u++)
+        for (int st = 0; st < 4; st++) {
+            const int64_t r = ch * 16 + 4 * st + kr;
 #pragma unroll
-            for (int st = 0; st < 4; st++) {
-                const int64_t r = (ch0 + u * stride) * 16 + 4 * st + kr;
-#pragma unroll
-                for (int t = 0; t < NT; t++) {
-                    const int c = 16 * t + kc;
-                    x[u][st][t] = (r < m && c < R) ? (double)A[r * R + c] : 0.0;
-                }
+            for (int t = 0; t < NT; t++) {
+                const int c = 16 * t + kc;
+                x[st][t] = (r < m && c < R) ? (double)A[r * R + c] : 0.0;
             }
+        }
 #pragma unroll
-        for (int u = 0; u < kGramBatch; u++)
+        for (int st = 0; st < 4; st++)
 #pragma unroll
-            for (int st = 0; st < 4; st++)
+            for (int ta = 0; ta < NT; ta++)
 #pragma unroll
-                for (int ta = 0; ta < NT; ta++)
-#pragma unroll
-                    for (int tb = ta; tb < NT; tb++)
-                        acc[ta][tb] =
-                            __builtin_amdgcn_mfma_f64_16x16x4f64(x[u][st][ta], x[u][st][tb], acc[ta][tb], 0, 0, 0);
+                for (int tb = ta; tb < NT; tb++)
+                    acc[ta][tb] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[st][ta], x[st][tb], acc[ta][tb], 0, 0, 0);
     }
     for (int w = 0; w < 4; w++) {
         if (wave == w) {
