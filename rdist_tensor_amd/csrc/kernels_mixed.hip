@@ -8,7 +8,6 @@
 //   C[4 (l / 16) + i][l % 16], i < 4.
 #include "common.hpp"
 #include "device.hpp"
-#include "dsymm.hpp"
 
 namespace exafac {
 namespace {
@@ -52,56 +51,72 @@ __device__ __forceinline__ void gram_chunk(const float* __restrict__ A, int64_t 
 }
 
 // ---------------------------------------------------------------- Y G on MFMA (FP32)
-// out = Y G for Y (m x R, FP32) and G (R x R, full storage), R <= 32, on
-// v_mfma_f32_32x32x2f32: one wavefront per 32-row tile (grid-stride), 16
-// steps of K = 2.  Lane l = i + 32 h supplies A[i][k] = Y[row0 + i][16 h + t]
-// and B[k][j] = G[16 h + t][i] in step t (the two K slots of a step take
-// columns t and 16 + t, so each lane reads 16 consecutive elements of its
-// row); the 32 x 32 result sits in lane j + 32 h, register r, as row
-// 8 (r / 4) + 4 h + r % 4, column j.  Entries past R or m are zero and not
-// stored.  VEC: rows read as float4 (R % 4 == 0, Y 16-byte aligned).
+// out = Y G for Y (m x R, FP32) and G (R x R, full storage), R <= 32 * KT, on
+// v_mfma_f32_32x32x2f32: one wavefront per 32-row tile (grid-stride), KT x 16
+// steps of K = 2 for each of the KT 32-column output blocks.  In step t of
+// K block kt, lane l = i + 32 h supplies A[i][.] = Y[row0 + i][32 kt + 16 h + t]
+// and B[.][j] = G[32 kt + 16 h + t][32 n + i] (the two K slots of a step take
+// columns 32 kt + t and 32 kt + 16 + t, so each lane reads 16 consecutive
+// elements of its row per K block); output block n sits in lane j + 32 h,
+// register r, as row 8 (r / 4) + 4 h + r % 4, column 32 n + j.  Entries past
+// R or m are zero and not stored.  VEC: rows read as float4 (R % 4 == 0, Y
+// 16-byte aligned).
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-template <bool VEC>
+template <bool VEC, int KT>
 __global__ __launch_bounds__(kThreads) void dsymm_mfma32(const float* __restrict__ G, const float* __restrict__ Y,
                                                          int64_t m, int R, float* __restrict__ out) {
     const int lane = threadIdx.x % 64, i = lane % 32, h = lane / 32;
-    float b[16];
+    float b[KT][KT][16];   // [output block n][K block kt][step]
 #pragma unroll
-    for (int t = 0; t < 16; t++) {
-        const int k = 16 * h + t;
-        b[t] = (k < R && i < R) ? G[k * R + i] : 0.0f;
-    }
+    for (int n = 0; n < KT; n++)
+#pragma unroll
+        for (int kt = 0; kt < KT; kt++)
+#pragma unroll
+            for (int t = 0; t < 16; t++) {
+                const int k = 32 * kt + 16 * h + t, c = 32 * n + i;
+                b[n][kt][t] = (k < R && c < R) ? G[k * R + c] : 0.0f;
+            }
     const int64_t tiles = (m + 31) / 32;
     for (int64_t tile = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64; tile < tiles;
          tile += (int64_t)gridDim.x * (kThreads / 64)) {
         const int64_t row0 = tile * 32, row = row0 + i;
-        float a[16];
-        if constexpr (VEC) {
+        float a[KT][16];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int c = 16 * h + 4 * q;
-                f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
-                if (row < m && c < R) v = *reinterpret_cast<const f32x4*>(Y + row * R + c);
+        for (int kt = 0; kt < KT; kt++) {
+            if constexpr (VEC) {
 #pragma unroll
-                for (int e = 0; e < 4; e++) a[4 * q + e] = v[e];
-            }
-        } else {
+                for (int q = 0; q < 4; q++) {
+                    const int c = 32 * kt + 16 * h + 4 * q;
+                    f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+                    if (row < m && c < R) v = *reinterpret_cast<const f32x4*>(Y + row * R + c);
 #pragma unroll
-            for (int t = 0; t < 16; t++) {
-                const int c = 16 * h + t;
-                a[t] = (row < m && c < R) ? Y[row * R + c] : 0.0f;
+                    for (int e = 0; e < 4; e++) a[kt][4 * q + e] = v[e];
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < 16; t++) {
+                    const int c = 32 * kt + 16 * h + t;
+                    a[kt][t] = (row < m && c < R) ? Y[row * R + c] : 0.0f;
+                }
             }
         }
-        f32x16 acc;
 #pragma unroll
-        for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+        for (int n = 0; n < KT; n++) {
+            f32x16 acc;
 #pragma unroll
-        for (int t = 0; t < 16; t++) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t], b[t], acc, 0, 0, 0);
+            for (int r = 0; r < 16; r++) acc[r] = 0.0f;
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int64_t orow = row0 + 8 * (r / 4) + 4 * h + r % 4;
-            if (orow < m && i < R) out[orow * R + i] = acc[r];
+            for (int kt = 0; kt < KT; kt++)
+#pragma unroll
+                for (int t = 0; t < 16; t++)
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[kt][t], b[n][kt][t], acc, 0, 0, 0);
+            const int c = 32 * n + i;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int64_t orow = row0 + 8 * (r / 4) + 4 * h + r % 4;
+                if (orow < m && c < R) out[orow * R + c] = acc[r];
+            }
         }
     }
 }
@@ -610,15 +625,16 @@ void design32(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, i
 
 void dsymm32(hipStream_t s, const float* G, const float* Y, int64_t m, int R, float* out) {
     if (m == 0) return;
-    if (R > 32) {
-        dsymm_tile<float>(s, G, Y, m, R, out);
-        return;
-    }
     const int64_t tiles = (m + 31) / 32;
     const unsigned nb = grid_for(tiles, kThreads / 64, 2048);
     const bool vec = R % 4 == 0 && ((uintptr_t)Y % 16) == 0;
-    if (vec) hipLaunchKernelGGL(dsymm_mfma32<true>, dim3(nb), dim3(kThreads), 0, s, G, Y, m, R, out);
-    else hipLaunchKernelGGL(dsymm_mfma32<false>, dim3(nb), dim3(kThreads), 0, s, G, Y, m, R, out);
+    if (R <= 32) {
+        if (vec) hipLaunchKernelGGL((dsymm_mfma32<true, 1>), dim3(nb), dim3(kThreads), 0, s, G, Y, m, R, out);
+        else hipLaunchKernelGGL((dsymm_mfma32<false, 1>), dim3(nb), dim3(kThreads), 0, s, G, Y, m, R, out);
+    } else {
+        if (vec) hipLaunchKernelGGL((dsymm_mfma32<true, 2>), dim3(nb), dim3(kThreads), 0, s, G, Y, m, R, out);
+        else hipLaunchKernelGGL((dsymm_mfma32<false, 2>), dim3(nb), dim3(kThreads), 0, s, G, Y, m, R, out);
+    }
     EX_HIP(hipGetLastError());
 }
 
