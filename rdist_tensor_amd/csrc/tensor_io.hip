@@ -8,8 +8,8 @@
 // and parsed in place; the caller receives the raw coordinates with the
 // per-mode minimum and maximum, as the HDF5 file stores them.
 #include <cerrno>
+#include <charconv>
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
 #include <memory>
 
@@ -97,15 +97,12 @@ uint64_t parse_index(const char* s, const char* e, int64_t lineno) {
     return v;
 }
 
+// Locale-independent (std::from_chars); a leading '+' is accepted as well.
 double parse_value(const char* s, const char* e, int64_t lineno) {
-    char tmp[64];
-    const size_t n = (size_t)(e - s);
-    EX_CHECK(n > 0 && n < sizeof(tmp), 1, "malformed value on line " + std::to_string(lineno));
-    std::memcpy(tmp, s, n);
-    tmp[n] = 0;
-    char* end = nullptr;
-    const double v = std::strtod(tmp, &end);
-    EX_CHECK(end == tmp + n, 1, "malformed value on line " + std::to_string(lineno));
+    if (s < e && *s == '+') s++;
+    double v = 0.0;
+    const auto r = std::from_chars(s, e, v);
+    EX_CHECK(s < e && r.ec == std::errc() && r.ptr == e, 1, "malformed value on line " + std::to_string(lineno));
     return v;
 }
 

@@ -26,7 +26,7 @@ def test_read_tns_matches_numpy(tmp_path):
     ref = np.loadtxt(p)
     assert c.dtype == np.uint64 and c.shape == (nnz, 4)
     assert np.array_equal(c, ref[:, :4].astype(np.uint64))
-    assert np.array_equal(v, ref[:, 4])             # strtod and NumPy round alike
+    assert np.array_equal(v, ref[:, 4])             # both round correctly
     assert np.array_equal(mn, coords.min(0)) and np.array_equal(mx, coords.max(0))
     idx, vals2, d = tensor_io.load_tensor_file(p)
     assert np.array_equal(idx, (coords - coords.min(0)).astype(np.uint32))
@@ -36,9 +36,9 @@ def test_read_tns_matches_numpy(tmp_path):
 
 def test_tns_layout_variants(tmp_path):
     """Tabs, carriage returns, comments, blank lines, a missing final newline,
-    exponent notation and 64-bit coordinates."""
+    exponent notation, a leading '+' and 64-bit coordinates."""
     p = tmp_path / "v.tns"
-    p.write_text("# a comment\n\n1\t2\t3\t1.5e-3\r\n  4 5 6   -2\n\n# tail\n7 8 18446744073709551615 3.25")
+    p.write_text("# a comment\n\n1\t2\t3\t1.5e-3\r\n  4 5 6   -2\n\n# tail\n7 8 18446744073709551615 +3.25")
     c, v, mn, mx = tensor_io.read_tns(p)
     assert c.tolist() == [[1, 2, 3], [4, 5, 6], [7, 8, 18446744073709551615]]
     assert v.tolist() == [1.5e-3, -2.0, 3.25]
