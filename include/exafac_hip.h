@@ -205,6 +205,15 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch);
  * optimizer alone (no fit evaluations, no statistics).  *seconds (optional)
  * receives the wall time from the first launch to the last completion. */
 int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, double* seconds);
+
+/* (no counterpart: sweeps replayed from a captured hipGraph) With on != 0,
+ * ex_als_run and ex_als_run_batch run the first sweep at a given J eagerly,
+ * capture one sweep (every mode step; the draws read the sweep number from
+ * device memory) and replay it for the later sweeps: the same kernels, the
+ * same results, one launch per sweep instead of one per kernel.  The
+ * statistics' region times then cover the eager sweep only.  Randomized
+ * optimizers only; on == 0 drops the graph. */
+int ex_als_use_graph(ex_als* a, int on);
 /* ALS_Optimizer::compute_exact_fit -- als_optimizer.hpp:208-211,
  * sparse_tensor.hpp:288-356 */
 int ex_als_fit(ex_als* a, double* fit);

@@ -59,6 +59,7 @@ SIGNATURES = {
     "ex_exact_als_create": [_vp, _vp, _vp, C.c_int, _pp],
     "ex_tensor_mttkrp_path": [_vp, C.c_int],
     "ex_als_run_batch": [_pp, C.c_int, C.c_int64, C.c_int64, _d],
+    "ex_als_use_graph": [_vp, C.c_int],
     "ex_smttkrp_forward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp],
     "ex_smttkrp_backward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp, _vp],
     "ex_als_destroy": [_vp],
@@ -371,6 +372,10 @@ class ALS(_Owned):
         sec = C.c_double()
         check(lib().ex_als_run_batch(arr, len(optimizers), rounds, J, C.byref(sec)))
         return sec.value
+
+    def use_graph(self, on=True):
+        """Replay captured sweeps in run / run_batch (ex_als_use_graph)."""
+        check(lib().ex_als_use_graph(self.h, 1 if on else 0))
 
     def step(self, mode, J, sweep):
         check(lib().ex_als_step(self.h, mode, J, sweep))

@@ -246,8 +246,10 @@ struct ex_sampler {
     ex_factors* f = nullptr;
     int kind = 0, precision = 0;
     uint64_t seed = 0;
+    // set while a sweep is captured: the draws read the sweep number here
+    const uint64_t* sweep_dev = nullptr;
     // CP-ARLS-LEV
-    DevBuf scores[kMaxModes], cdf[kMaxModes], sums, gtmp, gpinv;
+    DevBuf scores[kMaxModes], cdf[kMaxModes], sums, gtmp, gpinv, pivflag;
     // STS-CP (row-tree nodes FP64, or FP32 on the mixed path)
     TreeGeom g[kMaxModes];
     TreeGeom ge;
@@ -269,7 +271,7 @@ struct ex_als {
     // once grams_valid is set (each step refreshes its own mode's)
     bool exact = false, grams_valid = false;
     DevBuf grams;
-    DevBuf sd, wd, kd, dg, dm, G, Gi, Gi32, GY, Ms, Y, scal;
+    DevBuf sd, wd, kd, dg, dm, G, Gi, pivflag, GY, Ms, Y, scal;
     DevBuf tmp[4];
     DevBuf part;
     // u64[4]: hits (nonzeros iterated) and touched rows summed over steps, the
@@ -277,6 +279,15 @@ struct ex_als {
     DevBuf cnt;
     // region events of steps not yet read back: 6 per step
     std::vector<hipEvent_t> ev_pending, ev_pool;
+    // sweeps replayed from a captured graph (ex_als_use_graph): the sweep
+    // number the captured draws read, the graph of one sweep at sample
+    // count graph_J, and whether a capture is running (no region events then)
+    bool use_graph = false, capturing = false;
+    int64_t graph_J = -1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    DevBuf sweep_ctr;
+    std::vector<const void*> graph_fp;   // the buffers the graph was captured on
     double t_region[7] = {};
     std::string stats = "{}";
 };
@@ -296,7 +307,8 @@ void sampler_update_mixed(ex_sampler* s, int j) {
         s->gtmp.ensure((size_t)R * R * 8);
         s->gpinv.ensure((size_t)R * R * 8);
         k::gram_bf16(st, U, I, R, s->gtmp.as<double>(), s->part);
-        k::spd_inverse(st, s->gtmp.as<double>(), R, 1e-11, s->gpinv.as<double>(), nullptr);
+        s->pivflag.ensure(8);
+        k::spd_pinv(st, s->gtmp.as<double>(), R, 1e-11, s->gpinv.as<double>(), s->pivflag.as<int>());
         s->scores[j].ensure((size_t)I * 8);
         s->cdf[j].ensure((size_t)I * 8);
         k::dagat32(st, U, I, R, s->gpinv.as<double>(), s->scores[j].as<double>());
@@ -351,7 +363,7 @@ void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const do
         }
         Span sp(s->ctx, "arls_draw");
         k::arls_draw(st, N, j, J, cdfs, sc, s->sums.as<double>(), f->dims, s->seed, sweep, draws_row_dev, samples,
-                     weights);
+                     weights, s->sweep_dev);
         sp.end();
     } else {
         const bool mixed = s->precision == EX_MIXED;
@@ -377,7 +389,7 @@ void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const do
         }
         Span sd(s->ctx, "sts_draw");
         k::sts_draw(st, N, j, R, J, m, s->pinvp.as<double>(), s->seed, sweep, draws_row_dev, draws_eig_dev, samples,
-                    weights, mixed ? 4 : 8);
+                    weights, mixed ? 4 : 8, s->sweep_dev);
         sd.end();
     }
 }
@@ -426,9 +438,13 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     const int N = f->N, R = f->R;
     const int64_t Ij = f->dims[j];
     EX_CHECK(J > 0 && J < (int64_t)1 << 31, EX_ERR_ARG, "sample count J must be in [1, 2^31)");
-    hipEvent_t ev[6];
-    for (int q = 0; q < 6; q++) ev[q] = als_event(a);
-    auto mark = [&](int q) { EX_HIP(hipEventRecord(ev[q], st)); };
+    hipEvent_t ev[6] = {};
+    const bool regions = !a->capturing;   // region events are not captured
+    if (regions)
+        for (int q = 0; q < 6; q++) ev[q] = als_event(a);
+    auto mark = [&](int q) {
+        if (regions) EX_HIP(hipEventRecord(ev[q], st));
+    };
     mark(0);
     // Region 1: draw samples (accumulator_stationary.hpp:148-153)
     s->samples.ensure((size_t)J * N * 4);
@@ -453,7 +469,7 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     a->dm.ensure((size_t)J * R * vb);
     a->G.ensure((size_t)R * R * 8);
     a->Gi.ensure((size_t)R * R * 8);
-    a->Gi32.ensure((size_t)R * R * 4);
+    a->pivflag.ensure(8);
     s_dd.end();
     Span s_de(a->ctx, "design_gather");
     if (mixed) {
@@ -475,7 +491,7 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     Span s_dp(a->ctx, "design_gram_pinv", aux);
     if (mixed) {
         k::gram_bf16(aux, a->dg.as<float>(), Ju, R, a->G.as<double>(), a->part, ju);
-        k::spd_inverse(aux, a->G.as<double>(), R, 1e-11, a->Gi.as<double>(), a->Gi32.as<float>());
+        k::spd_pinv(aux, a->G.as<double>(), R, 1e-11, a->Gi.as<double>(), a->pivflag.as<int>());
     } else {
         k::gram(aux, a->dg.as<double>(), Ju, R, a->G.as<double>(), a->part, ju);
         k::pinv(aux, a->G.as<double>(), R, 1e-11, a->Gi.as<double>());
@@ -540,8 +556,129 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     sampler_update_impl(s, j);
     mark(5);
     // region times are read back later (als_resolve_regions): no wait here
+    if (!regions) return;
     for (int q = 0; q < 6; q++) a->ev_pending.push_back(ev[q]);
     if (a->ev_pending.size() >= 6 * 64) als_resolve_regions(a);
+}
+
+__global__ void set_u64(uint64_t* p, uint64_t v) { *p = v; }
+
+void exact_step_impl(ex_als* a, int j);
+
+// Addresses of every device buffer a sweep can touch.  A captured graph
+// holds raw addresses, so a buffer that grew (and moved) since the capture,
+// e.g. one the fit enlarges, forces a new capture.
+std::vector<const void*> graph_fingerprint(const ex_als* a) {
+    std::vector<const void*> v;
+    auto add = [&](const DevBuf& b) { v.push_back(b.p); };
+    const ex_sampler* s = a->s;
+    const ex_factors* f = a->f;
+    const ex_tensor* t = a->t;
+    for (const DevBuf* b : {&a->sd, &a->wd, &a->kd, &a->dg, &a->dm, &a->G, &a->Gi, &a->pivflag, &a->GY, &a->Ms, &a->Y,
+                            &a->scal, &a->part, &a->cnt, &a->grams, &a->sweep_ctr})
+        add(*b);
+    for (const DevBuf& b : a->tmp) add(b);
+    for (const DevBuf* b : {&s->sums, &s->gtmp, &s->gpinv, &s->grams, &s->M, &s->lam, &s->eig, &s->eig32, &s->pinvp,
+                            &s->part, &s->samples, &s->weights, &s->draws_row, &s->draws_eig, &s->pivflag})
+        add(*b);
+    for (int k2 = 0; k2 < kMaxModes; k2++) {
+        add(s->scores[k2]);
+        add(s->cdf[k2]);
+        add(s->nodes[k2]);
+        add(f->U[k2]);
+        add(f->U32[k2]);
+        const ModeLookup& L = t->lk[k2];
+        for (const DevBuf* b : {&L.fkey, &L.rrow, &L.rfib, &L.rval, &L.rval32, &L.cmap, &L.filt, &L.fptr, &L.kpos,
+                                &L.cpos, &L.marks, &L.cpart})
+            add(*b);
+    }
+    for (const DevBuf* b : {&f->sigma, &f->part, &f->gram, &t->coords, &t->vals}) add(*b);
+    for (const DevBuf& b : t->tmp) add(b);
+    for (const DevBuf& b : a->ctx->c.scratch) add(b);
+    return v;
+}
+
+void als_drop_graph(ex_als* a) {
+    if (a->gexec) (void)hipGraphExecDestroy(a->gexec);
+    if (a->graph) (void)hipGraphDestroy(a->graph);
+    a->gexec = nullptr;
+    a->graph = nullptr;
+    a->graph_J = -1;
+}
+
+// Captures one sweep (every mode step at sample count J) into a graph whose
+// draws read the sweep number from a->sweep_ctr.  The step's buffers must be
+// sized already (an eager sweep at the same J).  A captured memset is
+// refused: memset nodes have given wrong results on replays here.
+void als_capture_sweep(ex_als* a, int64_t J) {
+    hipStream_t st = a->ctx->c.stream;
+    als_drop_graph(a);
+    a->sweep_ctr.ensure(8);
+    EX_HIP(hipStreamSynchronize(st));
+    a->s->sweep_dev = a->sweep_ctr.as<uint64_t>();
+    a->capturing = true;
+    hipGraph_t g = nullptr;
+    EX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        for (int j = 0; j < a->f->N; j++) als_step_impl(a, j, J, 0);
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        a->capturing = false;
+        a->s->sweep_dev = nullptr;
+        throw;
+    }
+    const hipError_t e = hipStreamEndCapture(st, &g);
+    a->capturing = false;
+    a->s->sweep_dev = nullptr;
+    EX_HIP(e);
+    a->graph = g;
+    size_t n = 0;
+    EX_HIP(hipGraphGetNodes(g, nullptr, &n));
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n) EX_HIP(hipGraphGetNodes(g, nodes.data(), &n));
+    for (hipGraphNode_t node : nodes) {
+        hipGraphNodeType t;
+        EX_HIP(hipGraphNodeGetType(node, &t));
+        if (t == hipGraphNodeTypeMemset) {
+            als_drop_graph(a);
+            EX_CHECK(false, EX_ERR_UNSUPPORTED, "a captured sweep holds a memset node; run without graphs");
+        }
+    }
+    EX_HIP(hipGraphInstantiate(&a->gexec, g, nullptr, nullptr, 0));
+    a->graph_J = J;
+    a->graph_fp = graph_fingerprint(a);
+}
+
+// One sweep at sample count J and sweep number `round`: eager, or with
+// graphs on a replay of the captured sweep.  The first sweep at a given J
+// runs eagerly (it sizes every buffer) and then captures one.
+void als_sweep(ex_als* a, int64_t J, uint64_t round) {
+    if (a->exact) {
+        for (int j = 0; j < a->f->N; j++) exact_step_impl(a, j);
+        return;
+    }
+    if (!a->use_graph || a->graph_J != J) {
+        for (int j = 0; j < a->f->N; j++) als_step_impl(a, j, J, round);
+        if (a->use_graph) als_capture_sweep(a, J);
+        return;
+    }
+    if (graph_fingerprint(a) != a->graph_fp) als_capture_sweep(a, J);
+    hipStream_t st = a->ctx->c.stream;
+    ex_factors* f = a->f;
+    const bool mixed = a->precision == EX_MIXED;
+    // the host-side bookkeeping of the factor copies that the captured steps
+    // did once: bring the copy the steps read up to date (a host write since
+    // the capture), and mark every mode's other copy stale afterwards
+    if (mixed) f->sync32_all();
+    else f->sync64_all();
+    hipLaunchKernelGGL(set_u64, dim3(1), dim3(1), 0, st, a->sweep_ctr.as<uint64_t>(), round);
+    EX_HIP(hipGetLastError());
+    EX_HIP(hipGraphLaunch(a->gexec, st));
+    for (int j = 0; j < f->N; j++) {
+        if (mixed) f->wrote32(j);
+        else f->wrote64(j);
+    }
 }
 
 // Grams of every factor into a->grams (FP64 products and sums in both
@@ -583,10 +720,10 @@ void exact_step_impl(ex_als* a, int j) {
     mark(1);
     a->G.ensure((size_t)R * R * 8);
     a->Gi.ensure((size_t)R * R * 8);
-    a->Gi32.ensure((size_t)R * R * 4);
+    a->pivflag.ensure(8);
     Span s_gp(a->ctx, "exact_gram_pinv");
     k::chain_had(st, a->grams.as<double>(), N, j, R, a->G.as<double>());
-    if (mixed) k::spd_inverse(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>(), a->Gi32.as<float>());
+    if (mixed) k::spd_pinv(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>(), a->pivflag.as<int>());
     else k::pinv(st, a->G.as<double>(), R, 1e-11, a->Gi.as<double>());
     s_gp.end();
     mark(2);
@@ -1191,6 +1328,7 @@ int ex_als_destroy(ex_als* a) {
             else i++;
         for (hipEvent_t e : a->ev_pending) (void)hipEventDestroy(e);
         for (hipEvent_t e : a->ev_pool) (void)hipEventDestroy(e);
+        als_drop_graph(a);
         delete a;
     });
 }
@@ -1229,6 +1367,17 @@ int ex_als_fit(ex_als* a, double* fit) {
 // Host threads that enqueue the work of ex_als_run_batch.
 constexpr int kBatchThreads = 8;
 
+int ex_als_use_graph(ex_als* a, int on) {
+    return guarded([&] {
+        EX_CHECK(a, EX_ERR_ARG, "null optimizer");
+        EX_CHECK(!a->exact || !on, EX_ERR_UNSUPPORTED, "graph replay is for the randomized optimizers");
+        EX_HIP(hipSetDevice(a->ctx->c.device));
+        EX_HIP(hipStreamSynchronize(a->ctx->c.stream));
+        a->use_graph = on != 0;
+        if (!a->use_graph) als_drop_graph(a);
+    });
+}
+
 int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, double* seconds) {
     return guarded([&] {
         EX_CHECK(als && n >= 1, EX_ERR_ARG, "no optimizers");
@@ -1241,9 +1390,11 @@ int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, doubl
                 EX_CHECK(als[u]->ctx != als[t]->ctx, EX_ERR_ARG, "each optimizer needs its own context");
             maxN = std::max(maxN, als[t]->f->N);
         }
+        bool graphs = true;
         for (int t = 0; t < n; t++) {
             als_resolve_regions(als[t]);
             EX_HIP(hipStreamSynchronize(als[t]->ctx->c.stream));
+            graphs = graphs && als[t]->use_graph && !als[t]->exact;
         }
         auto t0 = std::chrono::steady_clock::now();
         // host threads enqueue the optimizers' mode steps (optimizer t on
@@ -1256,7 +1407,15 @@ int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, doubl
         for (int th = 0; th < T; th++)
             pool.emplace_back([&, th] {
                 try {
-                    for (int64_t round = 1; round <= rounds; round++)
+                    for (int64_t round = 1; round <= rounds; round++) {
+                        if (graphs) {
+                            // one launch of each optimizer's captured sweep
+                            for (int t = th; t < n; t += T) {
+                                EX_HIP(hipSetDevice(als[t]->ctx->c.device));
+                                als_sweep(als[t], J, (uint64_t)round);
+                            }
+                            continue;
+                        }
                         for (int j = 0; j < maxN; j++)
                             for (int t = th; t < n; t += T) {
                                 ex_als* a = als[t];
@@ -1265,6 +1424,7 @@ int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, doubl
                                 if (a->exact) exact_step_impl(a, j);
                                 else als_step_impl(a, j, J, (uint64_t)round);
                             }
+                    }
                     for (int t = th; t < n; t += T) EX_HIP(hipStreamSynchronize(als[t]->ctx->c.stream));
                 } catch (...) {
                     err[(size_t)th] = std::current_exception();
@@ -1296,10 +1456,7 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
         for (int64_t round = 1; round <= rounds; round++) {
             EX_HIP(hipStreamSynchronize(st));
             auto t0 = std::chrono::steady_clock::now();
-            for (int j = 0; j < a->f->N; j++) {
-                if (a->exact) exact_step_impl(a, j);
-                else als_step_impl(a, j, J, (uint64_t)round);
-            }
+            als_sweep(a, J, (uint64_t)round);
             EX_HIP(hipStreamSynchronize(st));
             als_total += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             if (round % epoch == 0) {

@@ -122,9 +122,8 @@ struct ModeLookup {
     DevBuf rfib;    // u32[nnz] fiber number
     DevBuf rval;    // f64[nnz]
     DevBuf rval32;  // f32[nnz], built on first use by the mixed-precision path
-    DevBuf cmap;    // u64[nfib]: (stamp << 32) | design row of the fibers sampled at step `stamp`
+    DevBuf cmap;    // u64[nfib]: 2^32 | design row of the fibers sampled by the running call, else 0
     DevBuf filt;    // u32[2^14]: bit (f mod 2^19) set for this step's sampled fibers f (zero between steps)
-    uint32_t stamp = 0;
     // hit-list path
     DevBuf fptr;    // u32[nfib + 1]: first key-order position of each fiber
     DevBuf kpos;    // u32[nnz]: row-order position of each key-ordered nonzero
@@ -151,7 +150,9 @@ void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& p
 // G = A^T A for FP32 A with FP64 products and sums
 void gram_f32in(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part);
 void eigh(hipStream_t s, const double* A, int R, double* lam, double* V, int ncopies = 1);
-void pinv(hipStream_t s, const double* M, int R, double tol, double* out, int ncopies = 1);
+// only_if: when given, the launch does nothing unless *only_if != 0
+void pinv(hipStream_t s, const double* M, int R, double tol, double* out, int ncopies = 1,
+          const int* only_if = nullptr);
 void dagat(hipStream_t s, const double* A, int64_t m, int R, const double* G, double* res);
 void dsymm(hipStream_t s, const double* G, const double* Y, int64_t m, int R, double* out);
 void col_sumsq(hipStream_t s, const double* U, int64_t m, int R, double* sigma, DevBuf& part);
@@ -168,7 +169,7 @@ void plain_sum(hipStream_t s, const double* v, int64_t n, double* out, DevBuf& p
 void arls_cdf(hipStream_t s, const double* scores, int64_t I, double* cdf, double* sum, DevBuf& part);
 void arls_draw(hipStream_t s, int N, int j, int64_t J, const double* const* cdfs, const double* const* scores,
                const double* sums, const int64_t* dims, uint64_t seed, uint64_t sweep,
-               const double* draws, uint32_t* samples, double* weights);
+               const double* draws, uint32_t* samples, double* weights, const uint64_t* sweep_dev = nullptr);
 // STS
 void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double* nodes);
 // internal nodes of a tree whose leaves are built (node = left + right)
@@ -192,7 +193,7 @@ struct StsModeArgs {
 void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* modes,
               const double* pinv_packed, uint64_t seed, uint64_t sweep,
               const double* draws_row, const double* draws_eig, uint32_t* samples, double* weights,
-              int node_bytes = 8);
+              int node_bytes = 8, const uint64_t* sweep_dev = nullptr);
 void pack_upper(hipStream_t s, const double* M, int R, double* packed);
 // sparse
 void keys_from_coords(hipStream_t s, const uint32_t* coords, int64_t n, const KeyLayout& kl, uint64_t* keys,
@@ -235,7 +236,13 @@ void col_sumsq_final_launch(hipStream_t s, const double* part, int nb, int R, do
 void gram_bf16(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part,
                const uint64_t* m_dev = nullptr);
 void tree_build_f32(hipStream_t s, const float* U, const TreeGeom& g, int R, float* nodes);
-void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, float* out32);
+// Gauss-Jordan inverse of an SPD matrix; pivots <= tol_rel * max diagonal are
+// dropped (their row and column zeroed) and counted into *dropped if given.
+void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* dropped = nullptr);
+// The pseudo-inverse: spd_inverse, replaced by the Jacobi pinv (same
+// tolerance) when a pivot was dropped, i.e. when M is numerically singular.
+// flag: one device int, written by the first launch and read by the second.
+void spd_pinv(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* flag);
 void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M);
 void to_f32(hipStream_t s, const double* a, int64_t n, float* b);
 void to_f64(hipStream_t s, const float* a, int64_t n, double* b);

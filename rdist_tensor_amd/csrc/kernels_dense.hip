@@ -345,7 +345,8 @@ __device__ void pinv_from_jacobi(EighShared& sh, int R, double tol, double* __re
 
 template <int NB, int NV>
 __global__ __launch_bounds__(kJacobiThreads) void pinv_kernel(const double* __restrict__ Min, int R, double tol,
-                                                        double* __restrict__ out) {
+                                                        double* __restrict__ out, const int* __restrict__ only_if) {
+    if (only_if && *only_if == 0) return;   // the same for the whole block
     __shared__ EighShared sh;
     const int64_t R2 = (int64_t)R * R;
     load_upper(Min + blockIdx.x * R2, R, sh.A);
@@ -706,8 +707,8 @@ void eigh(hipStream_t s, const double* A, int R, double* lam, double* V, int nco
     EX_HIP(hipGetLastError());
 }
 
-void pinv(hipStream_t s, const double* M, int R, double tol, double* out, int ncopies) {
-    JACOBI_DISPATCH(R, pinv_kernel, dim3(ncopies), s, M, R, tol, out);
+void pinv(hipStream_t s, const double* M, int R, double tol, double* out, int ncopies, const int* only_if) {
+    JACOBI_DISPATCH(R, pinv_kernel, dim3(ncopies), s, M, R, tol, out, only_if);
     EX_HIP(hipGetLastError());
 }
 

@@ -248,8 +248,10 @@ __device__ double spd_max_diag(SpdShared& sh, const SpdLayout& L, const double (
 // which gives the pseudo-inverse on the complementary subspace when the null
 // space is spanned by coordinate directions and a regularised inverse
 // otherwise.  The upper triangle of the result is mirrored to the lower.
+// Returns the number of dropped pivots (the same in every thread).
 template <int RPT>
-__device__ void spd_inverse_regs(SpdShared& sh, const SpdLayout& L, double (&a)[RPT], int R, double tol) {
+__device__ int spd_inverse_regs(SpdShared& sh, const SpdLayout& L, double (&a)[RPT], int R, double tol) {
+    int dropped = 0;
     for (int k = 0; k < R; k++) {
         const int b = k & 1;
         if (L.on) {
@@ -263,6 +265,7 @@ __device__ void spd_inverse_regs(SpdShared& sh, const SpdLayout& L, double (&a)[
         __syncthreads();
         const double piv = sh.col[b][k];
         const bool skip = !(piv > tol);
+        dropped += skip ? 1 : 0;
         const double inv = skip ? 0.0 : 1.0 / piv;
         if (L.on) {
             const double rk = sh.row[b][L.j];
@@ -302,6 +305,7 @@ __device__ void spd_inverse_regs(SpdShared& sh, const SpdLayout& L, double (&a)[
             if (i < R && i > L.j) a[m] = sh.full[L.j * R + i];
         }
     __syncthreads();
+    return dropped;
 }
 
 // Semidefinite Cholesky A = L L^T in registers (lower; entries above the
@@ -343,7 +347,7 @@ __device__ void spd_cholesky_regs(SpdShared& sh, const SpdLayout& L, double (&a)
 // of M read); pivots at or below tol_rel * (largest diagonal entry) dropped.
 template <int RPT, int T = kSpdThreads>
 __global__ __launch_bounds__(T) void spd_inverse_kernel(const double* __restrict__ M, int R, double tol_rel,
-                                                                  double* __restrict__ out, float* __restrict__ out32) {
+                                                                  double* __restrict__ out, int* __restrict__ dropped) {
     __shared__ SpdShared sh;
     const SpdLayout L(R);
     double a[RPT];
@@ -353,15 +357,15 @@ __global__ __launch_bounds__(T) void spd_inverse_kernel(const double* __restrict
         a[m] = (L.on && i < R) ? (i <= j ? M[i * R + j] : M[j * R + i]) : 0.0;
     }
     const double dmax = spd_max_diag<RPT>(sh, L, a, R);
-    spd_inverse_regs<RPT>(sh, L, a, R, tol_rel * dmax);
+    const int nd = spd_inverse_regs<RPT>(sh, L, a, R, tol_rel * dmax);
     if (L.on)
 #pragma unroll
         for (int m = 0; m < RPT; m++) {
             const int i = L.row(m);
             if (i >= R) continue;
             out[i * R + L.j] = a[m];
-            if (out32) out32[i * R + L.j] = (float)a[m];
         }
+    if (dropped && threadIdx.x == 0) *dropped = nd;
 }
 
 // computeM (sts_cp.hpp:80-188) with the eigendecompositions replaced by
@@ -586,10 +590,15 @@ void tree_build_f32(hipStream_t s, const float* U, const TreeGeom& g, int R, flo
     } while (0)
 
 // R <= 32: one element per thread (1024 threads), else the 256-thread tiling.
-void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, float* out32) {
-    if (R <= 32) hipLaunchKernelGGL((spd_inverse_kernel<1, 1024>), dim3(1), dim3(1024), 0, s, M, R, tol_rel, out, out32);
-    else RM_DISPATCH(R, spd_inverse_kernel, s, M, R, tol_rel, out, out32);
+void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* dropped) {
+    if (R <= 32) hipLaunchKernelGGL((spd_inverse_kernel<1, 1024>), dim3(1), dim3(1024), 0, s, M, R, tol_rel, out, dropped);
+    else RM_DISPATCH(R, spd_inverse_kernel, s, M, R, tol_rel, out, dropped);
     EX_HIP(hipGetLastError());
+}
+
+void spd_pinv(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* flag) {
+    spd_inverse(s, M, R, tol_rel, out, flag);
+    pinv(s, M, R, tol_rel, out, 1, flag);
 }
 
 void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M) {

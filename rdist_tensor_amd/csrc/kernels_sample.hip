@@ -133,6 +133,7 @@ struct StsArgs {
     int64_t J;
     const double* pinvp;    // packed upper of M[0] = pinv of the Hadamard product (FP64)
     uint64_t seed, sweep;
+    const uint64_t* sweep_dev;   // when set, the sweep number is read here (graph replays)
     const double* draws_row;
     const double* draws_eig;
     uint32_t* samples;
@@ -398,6 +399,7 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
     const int slot = wave * SPW + grp;
     const int64_t s = ((int64_t)blockIdx.x * (kDrawThreads / 64) + wave) * SPW + grp;
     if (s >= a.J) return;   // whole groups leave together
+    const uint64_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
     TN* hs = sh.h[slot];
     TN* xs = sh.x[slot];
     double* q2 = sh.q2[slot];
@@ -413,7 +415,7 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
         // eigen tree: query h, F = 1
         outer_g<TN, G, NV>(sh.pairs, hs, P, l, xx);
         const double ue = a.draws_eig ? a.draws_eig[(int64_t)k * a.J + s]
-                                      : uniform01(a.seed, stream_id(a.sweep, j, k, 1), s);
+                                      : uniform01(a.seed, stream_id(sweep, j, k, 1), s);
         double low, high;
         int64_t c = descend_g<TN, G, NV>(static_cast<const TN*>(md.eig), md.ge, S, P, xx, l, ue, low, high);
         const int64_t v = md.ge.leaf_idx(c);
@@ -423,7 +425,7 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
         // row tree: query sh = h * w_v, F = R
         outer_g<TN, G, NV>(sh.pairs, xs, P, l, xx);
         const double ur = a.draws_row ? a.draws_row[(int64_t)k * a.J + s]
-                                      : uniform01(a.seed, stream_id(a.sweep, j, k, 2), s);
+                                      : uniform01(a.seed, stream_id(sweep, j, k, 2), s);
         c = descend_g<TN, G, NV>(static_cast<const TN*>(md.nodes), md.g, S, P, xx, l, ur, low, high);
         const int64_t leaf = md.g.leaf_idx(c);
         const int F = (int)md.g.F;
@@ -536,6 +538,7 @@ struct ArlsArgs {
     int N, j;
     int64_t J;
     uint64_t seed, sweep;
+    const uint64_t* sweep_dev;   // when set, the sweep number is read here (graph replays)
     const double* draws;
     uint32_t* samples;
     double* weights;
@@ -544,13 +547,14 @@ struct ArlsArgs {
 __global__ __launch_bounds__(kThreads) void arls_draw_kernel(ArlsArgs a) {
     const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (s >= a.J) return;
+    const uint64_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
     double w = 0.0 - log((double)a.J);
     for (int k = 0; k < a.N; k++) {
         if (k == a.j) { a.samples[s * a.N + k] = 0; continue; }
         const int64_t I = a.dims[k];
         int64_t idx = 0;
         if (I >= 2) {
-            double u = a.draws ? a.draws[(int64_t)k * a.J + s] : uniform01(a.seed, stream_id(a.sweep, a.j, k, 0), s);
+            double u = a.draws ? a.draws[(int64_t)k * a.J + s] : uniform01(a.seed, stream_id(sweep, a.j, k, 0), s);
             const double* cp = a.cdf[k];
             int64_t lo = 0, hi = I;
             while (lo < hi) {
@@ -628,13 +632,13 @@ void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double
 
 void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* modes, const double* pinv_packed,
               uint64_t seed, uint64_t sweep, const double* draws_row, const double* draws_eig, uint32_t* samples,
-              double* weights, int node_bytes) {
+              double* weights, int node_bytes, const uint64_t* sweep_dev) {
     EX_CHECK(node_bytes == 8 || node_bytes == 4, 1, "node precision must be FP64 or FP32");
     StsArgs a;
     for (int k = 0; k < N; k++) a.m[k] = modes[k];
     a.N = N; a.j = j; a.R = R; a.J = J;
     a.pinvp = pinv_packed;
-    a.seed = seed; a.sweep = sweep;
+    a.seed = seed; a.sweep = sweep; a.sweep_dev = sweep_dev;
     a.draws_row = draws_row; a.draws_eig = draws_eig;
     a.samples = samples; a.weights = weights;
     // vector reads of leaf rows: R a multiple of the vector width and every
@@ -677,7 +681,7 @@ void arls_cdf(hipStream_t s, const double* scores, int64_t I, double* cdf, doubl
 
 void arls_draw(hipStream_t s, int N, int j, int64_t J, const double* const* cdfs, const double* const* scores,
                const double* sums, const int64_t* dims, uint64_t seed, uint64_t sweep, const double* draws,
-               uint32_t* samples, double* weights) {
+               uint32_t* samples, double* weights, const uint64_t* sweep_dev) {
     ArlsArgs a;
     for (int k = 0; k < N; k++) {
         a.cdf[k] = cdfs[k];
@@ -686,7 +690,7 @@ void arls_draw(hipStream_t s, int N, int j, int64_t J, const double* const* cdfs
     }
     a.sums = sums;
     a.N = N; a.j = j; a.J = J;
-    a.seed = seed; a.sweep = sweep;
+    a.seed = seed; a.sweep = sweep; a.sweep_dev = sweep_dev;
     a.draws = draws; a.samples = samples; a.weights = weights;
     hipLaunchKernelGGL(arls_draw_kernel, dim3(grid_for(J, kThreads, 1LL << 30)), dim3(kThreads), 0, s, a);
     EX_HIP(hipGetLastError());

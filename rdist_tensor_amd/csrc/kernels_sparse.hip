@@ -209,13 +209,13 @@ __global__ __launch_bounds__(kThreads) void rows_of_keyorder(const uint32_t* __r
 }
 
 // ---------------------------------------------------------------- sampled MTTKRP
-// Sampled fibers of this step: cmap[f] = (stamp << 32) | design row, where
-// the stamp numbers the step, and two bits of a 64 KB filter that the row
-// pass keeps in LDS (a Bloom filter with two hashes: f mod 2^19 and the top
-// 19 bits of f times 2654435761, Knuth's multiplicative hash).  A nonzero is
-// a hit when both of its fiber's filter bits are set and its cmap entry
-// carries this step's stamp, so cmap never needs clearing.  fsel[c] keeps
-// the fiber number (or ~0u) to clear the filter.
+// Sampled fibers of this step: cmap[f] = 2^32 | design row, and two bits of a
+// 64 KB filter that the row pass keeps in LDS (a Bloom filter with two
+// hashes: f mod 2^19 and the top 19 bits of f times 2654435761, Knuth's
+// multiplicative hash).  A nonzero is a hit when both of its fiber's filter
+// bits are set and its cmap entry is marked.  fsel[c] keeps the fiber number
+// (or ~0u) so that clear_fibers resets both afterwards: every call leaves
+// cmap and the filter zero, with nothing carried between calls on the host.
 constexpr int kFilterBits = 1 << 19;
 constexpr int kFilterWords = kFilterBits / 32;
 
@@ -225,8 +225,8 @@ __device__ __forceinline__ uint32_t filter_h2(uint32_t f) { return (f * 26544357
 __global__ __launch_bounds__(kThreads) void mark_fibers(const uint64_t* __restrict__ kd, int64_t Jb,
                                                         const uint64_t* __restrict__ Ju_dev,
                                                         const uint64_t* __restrict__ fkey, int64_t nfib,
-                                                        uint32_t stamp, uint64_t* __restrict__ cmap,
-                                                        uint32_t* __restrict__ filt, uint32_t* __restrict__ fsel) {
+                                                        uint64_t* __restrict__ cmap, uint32_t* __restrict__ filt,
+                                                        uint32_t* __restrict__ fsel) {
     const int64_t Ju = rows_of(Jb, Ju_dev);
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < Ju; c += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t key = kd[c];
@@ -241,32 +241,42 @@ __global__ __launch_bounds__(kThreads) void mark_fibers(const uint64_t* __restri
             const uint32_t h1 = filter_h1(f), h2 = filter_h2(f);
             atomicOr(filt + (h1 >> 5), 1u << (h1 & 31));
             atomicOr(filt + (h2 >> 5), 1u << (h2 & 31));
-            cmap[f] = ((uint64_t)stamp << 32) | (uint32_t)c;
+            cmap[f] = ((uint64_t)1 << 32) | (uint32_t)c;
         }
         fsel[c] = f;
     }
 }
 
-// Every set bit belongs to this step's samples, so whole words are cleared.
+// Every set bit belongs to this step's samples, so whole words are cleared;
+// the sampled fibers' map entries are reset too.
 __global__ __launch_bounds__(kThreads) void clear_fibers(const uint32_t* __restrict__ fsel, int64_t Jb,
                                                          const uint64_t* __restrict__ Ju_dev,
-                                                         uint32_t* __restrict__ filt) {
+                                                         uint32_t* __restrict__ filt, uint64_t* __restrict__ cmap) {
     const int64_t Ju = rows_of(Jb, Ju_dev);
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < Ju; c += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t f = fsel[c];
         if (f != ~0u) {
             filt[filter_h1(f) >> 5] = 0u;
             filt[filter_h2(f) >> 5] = 0u;
+            cmap[f] = 0ull;
         }
     }
+}
+
+// y[0, n) = 0 (a kernel rather than a memset, so that the step can be
+// captured in a graph).
+template <typename V>
+__global__ __launch_bounds__(kThreads) void zero_fill(V* __restrict__ y, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (V)0;
 }
 
 // Row-owned pass over the nonzeros in row order (sort_lookup.hpp:238-375
 // computes the same sums from the sampled side).  One wavefront per chunk
 // of kScanChunk consecutive nonzeros, four per lane; a nonzero is a hit when
-// its fiber's filter bits are set and its map entry carries the step's
-// stamp.  The hits are taken in order: a row that begins and ends inside the
-// chunk is summed and written to Y.  A row that crosses a chunk boundary
+// its fiber's filter bits are set and its map entry is marked.  The hits
+// are taken in order: a row that begins and ends inside the chunk is summed
+// and written to Y.  A row that crosses a chunk boundary
 // leaves partial sums in carry entries: slot 0 of a chunk for the row
 // running in from the left, slot 1 for the row beginning in the chunk and
 // running past its end.  Both slots always carry their row as key (flag 0
@@ -533,7 +543,7 @@ template <typename V, bool kExact, int G>
 __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* __restrict__ rrow,
                                                             const uint32_t* __restrict__ rfib,
                                                             const V* __restrict__ rval, int64_t nnz, int64_t nch,
-                                                            uint32_t stamp, const uint64_t* __restrict__ cmap,
+                                                            const uint64_t* __restrict__ cmap,
                                                             const uint32_t* __restrict__ filt,
                                                             const V* __restrict__ design, ExactSrc<V> ex, int R,
                                                             V* __restrict__ Y, V* __restrict__ carry,
@@ -641,7 +651,7 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
             }
 #pragma unroll
             for (int m = 0; m < M; m++) {
-                hit[m] = pos[m] && (uint32_t)(cm[m] >> 32) == stamp;
+                hit[m] = pos[m] && (cm[m] >> 32) != 0;
                 hc[m] = (uint32_t)cm[m];
             }
         }
@@ -1173,8 +1183,7 @@ void build_lookup(hipStream_t s, const uint32_t* coords, const double* vals, int
     L.fkey.ensure((size_t)nf * 8);
     L.cmap.ensure((size_t)nf * 8);
     L.filt.ensure((size_t)kFilterWords * 4);
-    EX_HIP(hipMemsetAsync(L.cmap.p, 0, (size_t)nf * 8, s));   // stamp 0: never a step's stamp
-    L.stamp = 0;
+    EX_HIP(hipMemsetAsync(L.cmap.p, 0, (size_t)nf * 8, s));   // no fiber marked
     EX_HIP(hipMemsetAsync(L.filt.p, 0, (size_t)kFilterWords * 4, s));
     hipLaunchKernelGGL(fiber_scatter_keys, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s, kout, nnz,
                        flags, pos, L.fkey.as<uint64_t>());
@@ -1425,16 +1434,10 @@ template <typename V>
 void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju, const V* design,
                     int R, V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts, hipEvent_t* scan_events,
                     const uint64_t* Ju_dev) {
-    EX_HIP(hipMemsetAsync(Y, 0, (size_t)I_j * R * sizeof(V), s));
+    hipLaunchKernelGGL(zero_fill<V>, dim3(grid_for(I_j * R, kThreads, 8192)), dim3(kThreads), 0, s, Y, I_j * R);
     if (scan_events) EX_HIP(hipEventRecord(scan_events[0], s));
     if (Ju > 0 && L.nfib > 0 && L.nnz > 0) {
         const int64_t nch = (L.nnz + kScanChunk - 1) / kScanChunk;
-        // a fresh stamp per call; wrapping to 0 would match never-marked
-        // entries, so the map is cleared then
-        if (++L.stamp == 0) {
-            EX_HIP(hipMemsetAsync(L.cmap.p, 0, (size_t)L.nfib * 8, s));
-            L.stamp = 1;
-        }
         if (use_hit_list(L, Ju)) {
             const int64_t nchb = (L.nnz + kMarkChunk - 1) / kMarkChunk;
             if (!L.marks.p) {
@@ -1497,7 +1500,6 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
             EX_HIP(hipGetLastError());
             return;
         }
-        const uint32_t stamp = L.stamp;
         tmp[1].ensure((size_t)Ju * 4 + 256);
         uint32_t* fsel = tmp[1].as<uint32_t>();
         carry_reserve<V>(tmp, nch, R);
@@ -1505,20 +1507,19 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
         uint32_t *flag, *key;
         carry_views(tmp[2], 2 * nch, R, vec, flag, key);
         hipLaunchKernelGGL(mark_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, kd, Ju, Ju_dev,
-                           L.fkey.as<uint64_t>(), L.nfib, stamp, L.cmap.as<uint64_t>(),
-                           L.filt.as<uint32_t>(), fsel);
+                           L.fkey.as<uint64_t>(), L.nfib, L.cmap.as<uint64_t>(), L.filt.as<uint32_t>(), fsel);
         const dim3 scan_grid(grid_for(nch, kScanThreads / 64, num_cus(s)));   // one workgroup per CU (LDS)
         unsigned long long* part = count_parts(L, counts, scan_grid.x);
         with_group(group_lanes<V>(R), [&](auto gc) {
             hipLaunchKernelGGL((row_scan_reduce<V, false, decltype(gc)::value>), scan_grid, dim3(kScanThreads), 0, s,
-                               L.rrow.as<uint32_t>(), L.rfib.as<uint32_t>(), vals, L.nnz, nch, stamp,
+                               L.rrow.as<uint32_t>(), L.rfib.as<uint32_t>(), vals, L.nnz, nch,
                                L.cmap.as<uint64_t>(), L.filt.as<uint32_t>(), design, ExactSrc<V>{}, R, Y, vec, flag,
                                key, part);
         });
         if (counts) hipLaunchKernelGGL(sum_counts, dim3(1), dim3(kThreads), 0, s, part, (int)scan_grid.x, counts);
         carry_reduce<V>(s, tmp, 2 * nch, R, Y);
         hipLaunchKernelGGL(clear_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, fsel, Ju, Ju_dev,
-                           L.filt.as<uint32_t>());
+                           L.filt.as<uint32_t>(), L.cmap.as<uint64_t>());
     }
     if (scan_events) EX_HIP(hipEventRecord(scan_events[1], s));
     EX_HIP(hipGetLastError());
@@ -1527,7 +1528,7 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
 template <typename V>
 void exact_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const V* const* U, const int64_t* dims, int R,
                   V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts) {
-    EX_HIP(hipMemsetAsync(Y, 0, (size_t)I_j * R * sizeof(V), s));
+    hipLaunchKernelGGL(zero_fill<V>, dim3(grid_for(I_j * R, kThreads, 8192)), dim3(kThreads), 0, s, Y, I_j * R);
     if (L.nnz > 0) {
         const int64_t nch = (L.nnz + kScanChunk - 1) / kScanChunk;
         ExactSrc<V> ex{};
@@ -1548,7 +1549,7 @@ void exact_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const V* const* U
         unsigned long long* part = count_parts(L, counts, scan_grid.x);
         with_group(group_lanes<V>(R), [&](auto gc) {
             hipLaunchKernelGGL((row_scan_reduce<V, true, decltype(gc)::value>), scan_grid, dim3(kScanThreads), 0, s,
-                               L.rrow.as<uint32_t>(), L.rfib.as<uint32_t>(), vals, L.nnz, nch, 0u, nullptr, nullptr,
+                               L.rrow.as<uint32_t>(), L.rfib.as<uint32_t>(), vals, L.nnz, nch, nullptr, nullptr,
                                nullptr, ex, R, Y, vec, flag, key, part);
         });
         if (counts) hipLaunchKernelGGL(sum_counts, dim3(1), dim3(kThreads), 0, s, part, (int)scan_grid.x, counts);

@@ -219,3 +219,93 @@ def test_batched_run_equals_single_runs():
     for a, b in zip(results["single"], results["batch"]):
         for x, y in zip(a, b):
             assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("kind", ["sts", "arls"])
+def test_graph_replay_equals_eager(ctx, precision, kind):
+    """Sweeps replayed from a captured graph (ex_als_use_graph) give the
+    eager run's factors and fits bitwise, for both samplers and precisions;
+    the batched runner replays too."""
+    import json
+    from rdist_tensor_amd import abi
+    rng = np.random.default_rng(77)
+    dims = [400, 300, 200]
+    R, J, rounds = 8, 4096, 5
+    idx, vals = random_tensor(rng, dims, 20000)
+    U = normalized_factors(rng, dims, R)
+    prec = abi.EX_MIXED if precision == "mixed" else abi.EX_FP64
+    sk = abi.EX_SAMPLER_STS_CP if kind == "sts" else abi.EX_SAMPLER_CP_ARLS_LEV
+    out = {}
+    for graph in (False, True):
+        T = abi.Tensor(ctx, idx, vals, dims)
+        F = set_factors(ctx, dims, R, U)
+        S = abi.Sampler(ctx, F, sk, precision=prec, seed=5)
+        A = abi.ALS(ctx, T, F, S, precision=prec)
+        A.init()
+        A.use_graph(graph)
+        A.run(rounds, J, 1)
+        st = json.loads(A.stats_json())
+        out[graph] = ([F.get(k) for k in range(3)], st["fits"], st["sum_nonzeros_iterated"])
+    for k in range(3):
+        assert np.array_equal(out[True][0][k], out[False][0][k]), k
+    assert out[True][1] == out[False][1]
+    assert out[True][2] == out[False][2]
+    print(f"{kind}/{precision}: {rounds} sweeps, fits {out[True][1][-1]:.6f}, graph replay bitwise equal")
+
+
+def test_batched_graph_replay_equals_eager():
+    """ex_als_run_batch with every optimizer on graph replay reaches the
+    eager batch's factors bitwise."""
+    from rdist_tensor_amd import abi, synthetic
+    batch = synthetic.make_batch(count=4, seed=4, size_range=(300, 3000))
+    results = {}
+    for graph in (False, True):
+        objs, opts = [], []
+        for dims, idx, vals, R, st in batch:
+            c = abi.Context(0)
+            T = abi.Tensor(c, idx, vals, dims)
+            F = abi.Factors(c, dims, R)
+            for k, u in enumerate(synthetic.factors(dims, R, st)):
+                F.set(k, u)
+            S = abi.Sampler(c, F, abi.EX_SAMPLER_CP_ARLS_LEV, precision=abi.EX_MIXED, seed=st)
+            A = abi.ALS(c, T, F, S, precision=abi.EX_MIXED)
+            A.init()
+            A.use_graph(graph)
+            objs.append((c, T, F, S, A))
+            opts.append(A)
+        abi.ALS.run_batch(opts, 4, 2048)
+        results[graph] = [[o[2].get(k) for k in range(3)] for o in objs]
+        for c, T, F, S, A in objs:
+            c.close()
+    for a, b in zip(results[True], results[False]):
+        for k in range(3):
+            assert np.all(np.isfinite(b[k]))
+            assert np.array_equal(a[k], b[k])
+
+
+def test_mixed_arls_singular_design_gram():
+    """A 1524 x 1940 x 1064 tensor with 38,800 uniform nonzeros at R = 16 and
+    J = 2048: few samples hit a nonzero, the factors become sparse and the
+    design Gram of the fourth sweep is singular.  Gauss-Jordan drops a pivot
+    there (a zero column of its inverse, and a 0/0 column norm); the mixed
+    path then takes the Jacobi pseudo-inverse, as the FP64 path does, and the
+    factors stay finite."""
+    from rdist_tensor_amd import abi, synthetic
+    dims, idx, vals, R, st = synthetic.make_batch(count=4, seed=4, size_range=(300, 3000))[2]
+    assert dims == [1524, 1940, 1064] and R == 16
+    c = abi.Context(0)
+    try:
+        T = abi.Tensor(c, idx, vals, dims)
+        F = abi.Factors(c, dims, R)
+        for k, u in enumerate(synthetic.factors(dims, R, st)):
+            F.set(k, u)
+        S = abi.Sampler(c, F, abi.EX_SAMPLER_CP_ARLS_LEV, precision=abi.EX_MIXED, seed=st)
+        A = abi.ALS(c, T, F, S, precision=abi.EX_MIXED)
+        A.init()
+        A.run(6, 2048, 1)
+        for k in range(3):
+            assert np.all(np.isfinite(F.get(k)))
+        assert np.isfinite(A.fit())
+    finally:
+        c.close()
