@@ -242,6 +242,13 @@ int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G);
 int ex_eigh(ex_ctx* ctx, const double* A, int64_t R, double* lam, double* V);
 /* compute_pinv_square (common.h:405-443) with absolute eigenvalue tolerance tol. */
 int ex_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol, double* out);
+/* (no counterpart: the mixed path's R x R solve, for callers that check it)
+ * Pseudo-inverse of a symmetric positive semidefinite M: Gauss-Jordan with
+ * pivots <= tol_rel * max diagonal dropped; when one is dropped (M singular)
+ * the result is replaced by the Moore-Penrose pseudo-inverse (pivoted
+ * Cholesky for R <= 32, the Jacobi pinv of ex_pinv with tolerance tol_rel
+ * above). */
+int ex_spd_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol_rel, double* out);
 /* parallel_dsymm (common.h:696-721): out = Y G for Y (m x R) and a symmetric
  * G (R x R, full storage), on the kernel the factor update runs at
  * `precision` (EX_MIXED: FP32 inputs and products, on MFMA for R <= 32). */

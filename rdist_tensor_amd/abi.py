@@ -71,6 +71,7 @@ SIGNATURES = {
     "ex_gram": [_vp, _d, C.c_int64, C.c_int64, _d],
     "ex_eigh": [_vp, _d, C.c_int64, _d, _d],
     "ex_pinv": [_vp, _d, C.c_int64, C.c_double, _d],
+    "ex_spd_pinv": [_vp, _d, C.c_int64, C.c_double, _d],
     "ex_dsymm": [_vp, C.c_int, _d, _d, C.c_int64, C.c_int64, _d],
     "ex_tns_scan": [C.c_char_p, _i64, C.POINTER(C.c_int)],
     "ex_tns_read": [C.c_char_p, C.c_int64, C.c_int, C.POINTER(C.c_uint64), _d, C.POINTER(C.c_uint64),
@@ -206,6 +207,15 @@ class Context:
         R = M.shape[0]
         out = np.empty((R, R))
         check(lib().ex_pinv(self.h, _p(M), R, tol, _p(out)))
+        return out
+
+    def spd_pinv(self, M, tol_rel=1e-11):
+        """The mixed path's pseudo-inverse of a PSD matrix (Gauss-Jordan, or
+        the Moore-Penrose pseudo-inverse when a pivot is dropped)."""
+        M = f64(M)
+        R = M.shape[0]
+        out = np.empty((R, R))
+        check(lib().ex_spd_pinv(self.h, _p(M), R, tol_rel, _p(out)))
         return out
 
     def dsymm(self, G, Y, precision=EX_FP64):

@@ -1609,6 +1609,21 @@ int ex_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol, double* out) {
     });
 }
 
+int ex_spd_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol_rel, double* out) {
+    return guarded([&] {
+        EX_CHECK(ctx && M && out, EX_ERR_ARG, "null argument");
+        EX_CHECK(R >= 1 && R <= kMaxRank, EX_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+        hipStream_t st = ctx->c.stream;
+        DevBuf dM, dO, flag;
+        dM.ensure((size_t)R * R * 8);
+        dO.ensure((size_t)R * R * 8);
+        flag.ensure(8);
+        h2d(st, dM.p, M, (size_t)R * R * 8);
+        k::spd_pinv(st, dM.as<double>(), (int)R, tol_rel, dO.as<double>(), flag.as<int>());
+        d2h(st, out, dO.p, (size_t)R * R * 8);
+    });
+}
+
 int ex_dedup(ex_ctx* ctx, const uint32_t* samples, const double* weights, int64_t J, int N, int mode,
              const int64_t* dims, uint32_t* samples_dedup, double* weights_dedup, int64_t* Ju) {
     return guarded([&] {

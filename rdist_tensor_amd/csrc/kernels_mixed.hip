@@ -551,6 +551,101 @@ __global__ __launch_bounds__(kThreads) void dagat_f32(const float* __restrict__ 
     }
 }
 
+// Moore-Penrose pseudo-inverse of a symmetric positive semidefinite R x R
+// matrix (R <= 32), one workgroup of 1024 threads, thread t on element
+// (t / 32, t % 32).  Diagonally pivoted Cholesky G = P L L^T P^T, stopped at
+// the first pivot <= tol_rel * max diagonal (rank r, L: R x r); then with
+// K = L (L^T L)^-1, G+ = P K K^T P^T (pinv(L L^T) = L (L^T L)^-2 L^T; the
+// r x r inverse by in-place Gauss-Jordan).  only_if: when given, the launch
+// does nothing unless *only_if != 0.
+__global__ __launch_bounds__(1024) void psd_pinv_chol_kernel(const double* __restrict__ M, int R, double tol_rel,
+                                                             double* __restrict__ out,
+                                                             const int* __restrict__ only_if) {
+    if (only_if && *only_if == 0) return;   // the same for the whole block
+    constexpr int P = 32;
+    __shared__ double A[P * P];   // Schur complement; L in columns < k (rows >= column)
+    __shared__ double C[P * P];   // L^T L, then its inverse
+    __shared__ double K[P * P];   // L (L^T L)^-1
+    __shared__ int perm[P];
+    const int t = threadIdx.x, i = t / P, j = t % P;
+    const bool in = i < R && j < R;
+    A[t] = in ? M[i * R + j] : 0.0;
+    if (t < P) perm[t] = t;
+    __syncthreads();
+    double dmax = 0.0;
+    for (int q = 0; q < R; q++) dmax = fmax(dmax, A[q * P + q]);
+    const double thr = tol_rel * dmax;
+    int r = R;
+    for (int k = 0; k < R; k++) {
+        // largest remaining diagonal (lowest index on ties); every thread finds the same
+        int p = k;
+        double best = A[k * P + k];
+        for (int q = k + 1; q < R; q++) {
+            const double d = A[q * P + q];
+            if (d > best) {
+                best = d;
+                p = q;
+            }
+        }
+        if (!(best > thr)) {
+            r = k;
+            break;
+        }
+        // swap rows and columns k and p (rows of the finished L columns move too)
+        const int si = i == k ? p : (i == p ? k : i), sj = j == k ? p : (j == p ? k : j);
+        const double v = A[si * P + sj];
+        const int pv = t < P ? perm[t == k ? p : (t == p ? k : t)] : 0;
+        __syncthreads();
+        A[t] = v;
+        if (t < P) perm[t] = pv;
+        __syncthreads();
+        // column k of L, then the Schur complement of the trailing block
+        const double lkk = sqrt(A[k * P + k]);
+        const double lik = (i >= k && i < R) ? (i == k ? lkk : A[i * P + k] / lkk) : 0.0;
+        const double ljk = (j > k && j < R) ? A[j * P + k] / lkk : 0.0;
+        const bool trail = in && i > k && j > k;
+        const double nv = trail ? A[t] - lik * ljk : 0.0;
+        __syncthreads();
+        if (trail) A[t] = nv;
+        if (j == k && i >= k && i < R) A[t] = lik;
+        __syncthreads();
+    }
+    // C = L^T L (r x r)
+    if (i < r && j < r) {
+        double acc = 0.0;
+        for (int q = i > j ? i : j; q < R; q++) acc += A[q * P + i] * A[q * P + j];
+        C[t] = acc;
+    } else {
+        C[t] = 0.0;
+    }
+    __syncthreads();
+    // in-place Gauss-Jordan inverse of C (SPD, pivots > 0)
+    for (int k = 0; k < r; k++) {
+        const double inv = 1.0 / C[k * P + k];
+        const double ci = C[i * P + k], rk = C[k * P + j];
+        double v = C[t];
+        if (i == k && j == k) v = inv;
+        else if (i == k) v = rk * inv;
+        else if (j == k) v = -ci * inv;
+        else v = v - ci * (rk * inv);
+        __syncthreads();
+        if (i < r && j < r) C[t] = v;
+        __syncthreads();
+    }
+    // K = L C^-1 (R x r)
+    if (i < R && j < r) {
+        double acc = 0.0;
+        for (int c = 0; c < r; c++) acc += (i >= c ? A[i * P + c] : 0.0) * C[c * P + j];
+        K[t] = acc;
+    }
+    __syncthreads();
+    if (in) {
+        double acc = 0.0;
+        for (int c = 0; c < r; c++) acc += K[i * P + c] * K[j * P + c];
+        out[perm[i] * R + perm[j]] = acc;
+    }
+}
+
 }  // namespace
 
 namespace k {
@@ -598,7 +693,12 @@ void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* 
 
 void spd_pinv(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* flag) {
     spd_inverse(s, M, R, tol_rel, out, flag);
-    pinv(s, M, R, tol_rel, out, 1, flag);
+    if (R <= 32) {
+        hipLaunchKernelGGL(psd_pinv_chol_kernel, dim3(1), dim3(1024), 0, s, M, R, tol_rel, out, flag);
+        EX_HIP(hipGetLastError());
+    } else {
+        pinv(s, M, R, tol_rel, out, 1, flag);
+    }
 }
 
 void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M) {

@@ -55,6 +55,33 @@ def test_pinv(ctx, orc, R):
         assert rel_err(ctx.pinv(M), orc.pinv(M)) < 1e-11
 
 
+@pytest.mark.parametrize("R", [1, 4, 8, 16, 25, 32, 48])
+def test_spd_pinv(ctx, R):
+    """The mixed path's solve against numpy's Moore-Penrose pseudo-inverse:
+    full rank (Gauss-Jordan), rank-deficient with a general null space and
+    with zero rows in the middle (pivoted Cholesky for R <= 32, Jacobi
+    above), and zero.  Eigenvalues are >= 0.25 or exactly zero, so every
+    tolerance choice agrees on the rank; an unpermuted result or a dropped
+    L^T L factor fails on the rank-deficient cases."""
+    rng = np.random.default_rng(500 + R)
+    cases = []
+    for rank in sorted({R, max(R - 1, 0), R // 2, 0}, reverse=True):
+        Q, _ = np.linalg.qr(rng.standard_normal((R, R)))
+        B = Q[:, :rank] * rng.uniform(0.5, 2.0, rank)[None, :]
+        cases.append(B @ B.T)
+    if R >= 4:
+        B = rng.standard_normal((R, R // 2))
+        B[1, :] = 0.0
+        B[R - 2, :] = 0.0
+        cases.append(B @ B.T)
+    for M in cases:
+        M = (M + M.T) / 2
+        ref = np.linalg.pinv(M, rcond=1e-10, hermitian=True)
+        got = ctx.spd_pinv(M)
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert np.abs(got - ref).max() <= 1e-9 * scale, (R, np.linalg.matrix_rank(M))
+
+
 @pytest.mark.parametrize("R", [1, 4, 8, 16, 25, 32, 48, 64])
 def test_dsymm(ctx, orc, R):
     """The factor update's product out = Y G: FP64 against the oracle's
