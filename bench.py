@@ -10,7 +10,10 @@ R x R solves), each for --steps timed sweeps after --warmup untimed sweeps
 (the factors and sampler are reset to the initial state after the warm-up,
 so both final fits are "after --steps sweeps").  `value` is the mixed path's
 sweeps/s; the FP64 path's figures, the ratio and both final fits sit beside
-it.
+it.  The timed sweeps are replayed from a HIP graph captured by one untimed
+sweep (--graphs 0: enqueued step by step); the span figures (per-kernel
+durations, rooflines) come from a second pass over the same sweeps, enqueued
+step by step with HIP events around each span, which reaches the same fit.
 
 Prints one JSON line (metric, value, unit, timing, config) with `roofline` (the dominant
 kernel of the reported path, timed live with HIP events on the library's
@@ -116,13 +119,31 @@ def run_gpu(args, cfg, ctx, T, U0, precision):
     for w in range(args.warmup):
         for j in range(N):
             A.step(j, J, 1000 + w)
-    # back to the initial state: the timed sweeps start where an untimed run would
-    for k in range(N):
-        F.set(k, U0[k])
-        S.update(k)
-    F.set_sigma(np.ones(R))
-    ctx.sync()
+    def reset():
+        # back to the initial state: timed sweeps start where an untimed run would
+        for k in range(N):
+            F.set(k, U0[k])
+            S.update(k)
+        F.set_sigma(np.ones(R))
+        ctx.sync()
+
+    reset()
     log(f"[bench] {precision}: warmup {args.warmup} sweeps done")
+    if args.graphs:
+        # the throughput pass: each sweep replayed from a captured HIP graph
+        # (the capture is made by one untimed sweep), no timing events
+        A.use_graph(True)
+        abi.ALS.run_batch([A], 1, J)
+        reset()
+        t0 = time.perf_counter()
+        abi.ALS.run_batch([A], args.steps, J)
+        ctx.sync()
+        elapsed = time.perf_counter() - t0
+        A.use_graph(False)
+        fit_tp = A.fit()
+        reset()
+    # the instrumented pass: the same sweeps (sweep numbers 1..K) enqueued
+    # step by step with the span events on, for the per-kernel figures
     ctx.timing(True)
     ctx.sync()
     t0 = time.perf_counter()
@@ -130,13 +151,18 @@ def run_gpu(args, cfg, ctx, T, U0, precision):
         for j in range(N):
             A.step(j, J, sweep)
     ctx.sync()
-    elapsed = time.perf_counter() - t0
+    elapsed_inst = time.perf_counter() - t0
     timing = json.loads(ctx.timing_json())
     ctx.timing(False)
     fit1 = A.fit()
-    log(f"[bench] {precision}: {args.steps} sweeps in {elapsed:.3f} s; fit {fit0:.6f} -> {fit1:.6f}")
+    if not args.graphs:
+        elapsed, fit_tp = elapsed_inst, fit1
+    log(f"[bench] {precision}: {args.steps} sweeps in {elapsed:.3f} s "
+        f"({'graph replay' if args.graphs else 'eager'}; instrumented pass {elapsed_inst:.3f} s); "
+        f"fit {fit0:.6f} -> {fit1:.6f}")
     del A, S, F
-    return dict(elapsed=elapsed, timing=timing, fit0=fit0, fit1=fit1)
+    return dict(elapsed=elapsed, elapsed_instrumented=elapsed_inst, timing=timing, fit0=fit0, fit1=fit1,
+                passes_agree=bool(fit_tp == fit1))
 
 
 def run_exact(args, cfg, ctx, T, U0, precision, nnz):
@@ -219,6 +245,8 @@ def summarize(args, cfg, g, vbytes, precision):
                 nnz_per_s=round(cnt.get("H", 0) / g["elapsed"], 1),
                 spans_ms_per_sweep={k: round(v["ms"] / K, 4) for k, v in spans.items()},
                 counters_per_mode_step={k: round(v / steps, 1) for k, v in cnt.items()},
+                instrumented_ms_per_step=round(g["elapsed_instrumented"] / K * 1e3, 3),
+                throughput_and_instrumented_fits_equal=g["passes_agree"],
                 fit={"initial": g["fit0"], "final": g["fit1"]})
 
 
@@ -330,6 +358,9 @@ def main():
                     help="how the sampled MTTKRP finds the sampled fibers' nonzeros")
     ap.add_argument("--exact-steps", type=int, default=3,
                     help="ExactALS sweeps per precision reported beside the randomized paths (0: skip)")
+    ap.add_argument("--graphs", type=int, default=1,
+                    help="time the sweeps replayed from a captured HIP graph (1) or enqueued step by step (0); "
+                         "per-kernel figures always come from a second, instrumented eager pass")
     ap.add_argument("--batch-count", type=int, default=256, help="tensors of the batched workload")
     ap.add_argument("--batch-graphs", type=int, default=0,
                     help="batched workload: replay each optimizer's sweep from a HIP graph (1) or enqueue it (0)")
@@ -389,7 +420,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{args.config}: {cfg['sampler']}, dims {cfg['dims']}, nnz {len(idx)}, "
                                f"Zipf {cfg['zipf']}, {cfg['values']} values, R={R}, J={J}, seed {cfg['seed']}, "
-                               f"{K} sweeps per precision",
+                               f"{K} sweeps per precision, "
+                               f"{'graph replay' if args.graphs else 'enqueued step by step'}",
                    "precision": head, "sampler": cfg["sampler"]},
         "sampled_mttkrp_gbs": h["sampled_mttkrp_gbs"],
         "nnz_per_s": h["nnz_per_s"],
