@@ -218,6 +218,11 @@ struct SpdShared {
     double full[kMaxRank * kMaxRank];
 };
 
+// Moore-Penrose pseudo-inverse of a PSD matrix by one 1024-thread workgroup
+// (R <= 32; defined with the other R x R kernels below).
+__device__ void psd_pinv_chol_block(const double* __restrict__ M, int R, double tol_rel, double* __restrict__ out,
+                                    double* A, double* C, double* K, int* perm);
+
 struct SpdLayout {
     int j, g, GR;
     bool on;
@@ -345,10 +350,14 @@ __device__ void spd_cholesky_regs(SpdShared& sh, const SpdLayout& L, double (&a)
 
 // Inverse of a symmetric positive semidefinite R x R matrix (upper triangle
 // of M read); pivots at or below tol_rel * (largest diagonal entry) dropped.
+// With pinv_on_drop (1024 threads, R <= 32 only), a dropped pivot makes the
+// same workgroup write the Moore-Penrose pseudo-inverse instead.
 template <int RPT, int T = kSpdThreads>
 __global__ __launch_bounds__(T) void spd_inverse_kernel(const double* __restrict__ M, int R, double tol_rel,
-                                                                  double* __restrict__ out, int* __restrict__ dropped) {
+                                                                  double* __restrict__ out, int* __restrict__ dropped,
+                                                                  int pinv_on_drop) {
     __shared__ SpdShared sh;
+    __shared__ int perm[32];
     const SpdLayout L(R);
     double a[RPT];
 #pragma unroll
@@ -358,6 +367,13 @@ __global__ __launch_bounds__(T) void spd_inverse_kernel(const double* __restrict
     }
     const double dmax = spd_max_diag<RPT>(sh, L, a, R);
     const int nd = spd_inverse_regs<RPT>(sh, L, a, R, tol_rel * dmax);
+    if (dropped && threadIdx.x == 0) *dropped = nd;
+    if constexpr (T == 1024) {
+        if (pinv_on_drop && nd > 0) {   // nd is the same in every thread
+            psd_pinv_chol_block(M, R, tol_rel, out, sh.full, sh.full + 1024, sh.full + 2048, perm);
+            return;
+        }
+    }
     if (L.on)
 #pragma unroll
         for (int m = 0; m < RPT; m++) {
@@ -365,7 +381,6 @@ __global__ __launch_bounds__(T) void spd_inverse_kernel(const double* __restrict
             if (i >= R) continue;
             out[i * R + L.j] = a[m];
         }
-    if (dropped && threadIdx.x == 0) *dropped = nd;
 }
 
 // computeM (sts_cp.hpp:80-188) with the eigendecompositions replaced by
@@ -552,21 +567,15 @@ __global__ __launch_bounds__(kThreads) void dagat_f32(const float* __restrict__ 
 }
 
 // Moore-Penrose pseudo-inverse of a symmetric positive semidefinite R x R
-// matrix (R <= 32), one workgroup of 1024 threads, thread t on element
+// matrix (R <= 32) by one workgroup of 1024 threads, thread t on element
 // (t / 32, t % 32).  Diagonally pivoted Cholesky G = P L L^T P^T, stopped at
 // the first pivot <= tol_rel * max diagonal (rank r, L: R x r); then with
 // K = L (L^T L)^-1, G+ = P K K^T P^T (pinv(L L^T) = L (L^T L)^-2 L^T; the
-// r x r inverse by in-place Gauss-Jordan).  only_if: when given, the launch
-// does nothing unless *only_if != 0.
-__global__ __launch_bounds__(1024) void psd_pinv_chol_kernel(const double* __restrict__ M, int R, double tol_rel,
-                                                             double* __restrict__ out,
-                                                             const int* __restrict__ only_if) {
-    if (only_if && *only_if == 0) return;   // the same for the whole block
+// r x r inverse by in-place Gauss-Jordan).  A, C, K: 32 x 32 doubles of LDS
+// each, perm: 32 ints of LDS.
+__device__ void psd_pinv_chol_block(const double* __restrict__ M, int R, double tol_rel, double* __restrict__ out,
+                                    double* A, double* C, double* K, int* perm) {
     constexpr int P = 32;
-    __shared__ double A[P * P];   // Schur complement; L in columns < k (rows >= column)
-    __shared__ double C[P * P];   // L^T L, then its inverse
-    __shared__ double K[P * P];   // L (L^T L)^-1
-    __shared__ int perm[P];
     const int t = threadIdx.x, i = t / P, j = t % P;
     const bool in = i < R && j < R;
     A[t] = in ? M[i * R + j] : 0.0;
@@ -685,20 +694,24 @@ void tree_build_f32(hipStream_t s, const float* U, const TreeGeom& g, int R, flo
     } while (0)
 
 // R <= 32: one element per thread (1024 threads), else the 256-thread tiling.
-void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* dropped) {
-    if (R <= 32) hipLaunchKernelGGL((spd_inverse_kernel<1, 1024>), dim3(1), dim3(1024), 0, s, M, R, tol_rel, out, dropped);
-    else RM_DISPATCH(R, spd_inverse_kernel, s, M, R, tol_rel, out, dropped);
+static void spd_inverse_launch(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* dropped,
+                               int pinv_on_drop) {
+    if (R <= 32)
+        hipLaunchKernelGGL((spd_inverse_kernel<1, 1024>), dim3(1), dim3(1024), 0, s, M, R, tol_rel, out, dropped,
+                           pinv_on_drop);
+    else RM_DISPATCH(R, spd_inverse_kernel, s, M, R, tol_rel, out, dropped, 0);
     EX_HIP(hipGetLastError());
 }
 
+void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* dropped) {
+    spd_inverse_launch(s, M, R, tol_rel, out, dropped, 0);
+}
+
 void spd_pinv(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* flag) {
-    spd_inverse(s, M, R, tol_rel, out, flag);
-    if (R <= 32) {
-        hipLaunchKernelGGL(psd_pinv_chol_kernel, dim3(1), dim3(1024), 0, s, M, R, tol_rel, out, flag);
-        EX_HIP(hipGetLastError());
-    } else {
-        pinv(s, M, R, tol_rel, out, 1, flag);
-    }
+    // R <= 32: one launch (the pivoted-Cholesky pseudo-inverse in the same
+    // workgroup); above, the Jacobi pinv as a second launch that reads flag
+    spd_inverse_launch(s, M, R, tol_rel, out, flag, 1);
+    if (R > 32) pinv(s, M, R, tol_rel, out, 1, flag);
 }
 
 void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M) {
