@@ -211,7 +211,9 @@ int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, doubl
  * capture one sweep (every mode step; the draws read the sweep number from
  * device memory) and replay it for the later sweeps: the same kernels, the
  * same results, one launch per sweep instead of one per kernel.  The
- * statistics' region times then cover the eager sweep only.  Randomized
+ * statistics' region times then cover the eager sweep only.  While the
+ * context's span timing (ex_ctx_timing) is on, sweeps run eagerly.  A capture
+ * that would hold a memset node fails with EX_ERR_UNSUPPORTED.  Randomized
  * optimizers only; on == 0 drops the graph. */
 int ex_als_use_graph(ex_als* a, int on);
 /* ALS_Optimizer::compute_exact_fit -- als_optimizer.hpp:208-211,

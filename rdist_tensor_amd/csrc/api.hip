@@ -652,15 +652,18 @@ void als_capture_sweep(ex_als* a, int64_t J) {
 
 // One sweep at sample count J and sweep number `round`: eager, or with
 // graphs on a replay of the captured sweep.  The first sweep at a given J
-// runs eagerly (it sizes every buffer) and then captures one.
+// runs eagerly (it sizes every buffer) and then captures one.  With the
+// context's span timing on, sweeps run eagerly (the spans are host-side
+// records of events enqueued step by step).
 void als_sweep(ex_als* a, int64_t J, uint64_t round) {
     if (a->exact) {
         for (int j = 0; j < a->f->N; j++) exact_step_impl(a, j);
         return;
     }
-    if (!a->use_graph || a->graph_J != J) {
+    const bool replay = a->use_graph && !a->ctx->timing;
+    if (!replay || a->graph_J != J) {
         for (int j = 0; j < a->f->N; j++) als_step_impl(a, j, J, round);
-        if (a->use_graph) als_capture_sweep(a, J);
+        if (replay) als_capture_sweep(a, J);
         return;
     }
     if (graph_fingerprint(a) != a->graph_fp) als_capture_sweep(a, J);

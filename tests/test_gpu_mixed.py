@@ -254,6 +254,41 @@ def test_graph_replay_equals_eager(ctx, precision, kind):
     print(f"{kind}/{precision}: {rounds} sweeps, fits {out[True][1][-1]:.6f}, graph replay bitwise equal")
 
 
+def test_graph_with_span_timing_runs_eagerly():
+    """With the context's span timing on, an optimizer with graphs enabled
+    enqueues every sweep step by step: each mode step's draw span is
+    recorded, and the factors equal the graph-replayed run's."""
+    import json
+    from rdist_tensor_amd import abi
+    rng = np.random.default_rng(78)
+    dims = [300, 250, 200]
+    R, J, rounds = 8, 2048, 4
+    idx, vals = random_tensor(rng, dims, 15000)
+    U = normalized_factors(rng, dims, R)
+    out = {}
+    for timing in (False, True):
+        c = abi.Context(0)
+        try:
+            T = abi.Tensor(c, idx, vals, dims)
+            F = set_factors(c, dims, R, U)
+            S = abi.Sampler(c, F, abi.EX_SAMPLER_STS_CP, precision=abi.EX_MIXED, seed=6)
+            A = abi.ALS(c, T, F, S, precision=abi.EX_MIXED)
+            A.init()
+            A.use_graph(True)
+            c.timing(timing)
+            A.run(rounds, J, 10 ** 9)
+            c.sync()
+            if timing:
+                spans = json.loads(c.timing_json())["spans"]
+                assert spans["sts_draw"]["count"] == rounds * len(dims)
+                c.timing(False)
+            out[timing] = [F.get(k) for k in range(3)]
+        finally:
+            c.close()
+    for k in range(3):
+        assert np.array_equal(out[True][k], out[False][k]), k
+
+
 def test_batched_graph_replay_equals_eager():
     """ex_als_run_batch with every optimizer on graph replay reaches the
     eager batch's factors bitwise."""
