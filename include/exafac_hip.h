@@ -247,9 +247,9 @@ int ex_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol, double* out);
 /* (no counterpart: the mixed path's R x R solve, for callers that check it)
  * Pseudo-inverse of a symmetric positive semidefinite M: Gauss-Jordan with
  * pivots <= tol_rel * max diagonal dropped; when one is dropped (M singular)
- * the result is replaced by the Moore-Penrose pseudo-inverse (pivoted
- * Cholesky for R <= 32, the Jacobi pinv of ex_pinv with tolerance tol_rel
- * above). */
+ * the result is replaced by the Moore-Penrose pseudo-inverse (R <= 32: a
+ * pivoted Cholesky stopped at the same relative tolerance; above: the Jacobi
+ * pinv of ex_pinv, with tol_rel as its absolute eigenvalue tolerance). */
 int ex_spd_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol_rel, double* out);
 /* parallel_dsymm (common.h:696-721): out = Y G for Y (m x R) and a symmetric
  * G (R x R, full storage), on the kernel the factor update runs at

@@ -239,9 +239,11 @@ void tree_build_f32(hipStream_t s, const float* U, const TreeGeom& g, int R, flo
 // Gauss-Jordan inverse of an SPD matrix; pivots <= tol_rel * max diagonal are
 // dropped (their row and column zeroed) and counted into *dropped if given.
 void spd_inverse(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* dropped = nullptr);
-// The pseudo-inverse: spd_inverse, replaced by the Jacobi pinv (same
-// tolerance) when a pivot was dropped, i.e. when M is numerically singular.
-// flag: one device int, written by the first launch and read by the second.
+// The pseudo-inverse: spd_inverse, replaced when a pivot was dropped (M
+// numerically singular) by the Moore-Penrose pseudo-inverse -- for R <= 32 a
+// pivoted Cholesky in the same workgroup (same relative tolerance), above 32
+// the Jacobi pinv as a second launch (tol_rel as its absolute tolerance).
+// flag: one device int, the dropped-pivot count (read by that second launch).
 void spd_pinv(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* flag);
 void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M);
 void to_f32(hipStream_t s, const double* a, int64_t n, float* b);
