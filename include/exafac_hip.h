@@ -170,6 +170,13 @@ int ex_sampler_update(ex_sampler* s, int mode);
 int ex_sampler_draw(ex_sampler* s, int64_t J, int mode, uint64_t sweep,
                     const double* draws_row, const double* draws_eig,
                     uint32_t* samples, double* weights);
+/* (no counterpart) STS-CP draws given weight 0 since the sampler was created:
+ * on the mixed path a rounding residue between a node and its left child can
+ * route a draw into a leaf of zero mass; such a draw keeps its row indices but
+ * carries weight 0 instead of an infinite one.  Always 0 on the FP64 path and
+ * for CP-ARLS-LEV.  ex_als_run reports the count of its sweeps as
+ * "zero_mass_draws" in the statistics. */
+int ex_sampler_dropped_draws(ex_sampler* s, int64_t* n);
 /* STS-CP internals for parity checks: the (N+2) x R x R matrix stack built
  * by STS_CP::computeM(mode) (sts_cp.hpp:80-188), slot k at [k*R*R]. */
 int ex_sts_compute_m(ex_sampler* s, int mode, double* M);
@@ -249,7 +256,7 @@ int ex_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol, double* out);
  * pivots <= tol_rel * max diagonal dropped; when one is dropped (M singular)
  * the result is replaced by the Moore-Penrose pseudo-inverse (R <= 32: a
  * pivoted Cholesky stopped at the same relative tolerance; above: the Jacobi
- * pinv of ex_pinv, with tol_rel as its absolute eigenvalue tolerance). */
+ * pinv of ex_pinv dropping eigenvalues <= tol_rel * max diagonal). */
 int ex_spd_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol_rel, double* out);
 /* parallel_dsymm (common.h:696-721): out = Y G for Y (m x R) and a symmetric
  * G (R x R, full storage), on the kernel the factor update runs at

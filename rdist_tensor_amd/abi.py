@@ -54,6 +54,7 @@ SIGNATURES = {
     "ex_sampler_update": [_vp, C.c_int],
     "ex_sampler_draw": [_vp, C.c_int64, C.c_int, C.c_uint64, _d, _d, _u32, _d],
     "ex_sts_compute_m": [_vp, C.c_int, _d],
+    "ex_sampler_dropped_draws": [_vp, _i64],
     "ex_arls_scores": [_vp, C.c_int, _d],
     "ex_als_create": [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _pp],
     "ex_exact_als_create": [_vp, _vp, _vp, C.c_int, _pp],
@@ -338,6 +339,12 @@ class Sampler(_Owned):
         de = _p(de_arr) if de_arr is not None else None
         check(lib().ex_sampler_draw(self.h, J, mode, sweep, dr, de, _p(samples, _u32), _p(w)))
         return samples, w
+
+    def dropped_draws(self):
+        """STS-CP draws given weight 0 since creation (ex_sampler_dropped_draws)."""
+        n = C.c_int64()
+        check(lib().ex_sampler_dropped_draws(self.h, C.byref(n)))
+        return n.value
 
     def sts_m(self, mode):
         N, R = self.f.N, self.f.R

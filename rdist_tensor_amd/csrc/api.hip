@@ -258,6 +258,9 @@ struct ex_sampler {
     DevBuf part;
     // draw outputs and injected draws
     DevBuf samples, weights, draws_row, draws_eig;
+    // u64: STS-CP draws given weight 0 since creation (mixed path: a
+    // rounding residue routed the draw into a leaf of zero mass)
+    DevBuf dropped;
 };
 
 struct ex_als {
@@ -389,7 +392,7 @@ void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const do
         }
         Span sd(s->ctx, "sts_draw");
         k::sts_draw(st, N, j, R, J, m, s->pinvp.as<double>(), s->seed, sweep, draws_row_dev, draws_eig_dev, samples,
-                    weights, mixed ? 4 : 8, s->sweep_dev);
+                    weights, mixed ? 4 : 8, s->sweep_dev, s->dropped.as<unsigned long long>());
         sd.end();
     }
 }
@@ -490,7 +493,9 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     EX_HIP(hipStreamWaitEvent(aux, a->ctx->ev_fork, 0));
     Span s_dp(a->ctx, "design_gram_pinv", aux);
     if (mixed) {
-        k::gram_bf16(aux, a->dg.as<float>(), Ju, R, a->G.as<double>(), a->part, ju);
+        // FP32 rows, FP64 products and sums: the rank decision of the solve
+        // below then sees the design's own rounding (2^-24), not BF16's
+        k::gram_f32in(aux, a->dg.as<float>(), Ju, R, a->G.as<double>(), a->part, ju);
         k::spd_pinv(aux, a->G.as<double>(), R, 1e-11, a->Gi.as<double>(), a->pivflag.as<int>());
     } else {
         k::gram(aux, a->dg.as<double>(), Ju, R, a->G.as<double>(), a->part, ju);
@@ -614,6 +619,10 @@ void als_capture_sweep(ex_als* a, int64_t J) {
     hipStream_t st = a->ctx->c.stream;
     als_drop_graph(a);
     a->sweep_ctr.ensure(8);
+    // bring the factor copy the steps read up to date now: a conversion
+    // enqueued inside the capture would be replayed by every later sweep
+    if (a->precision == EX_MIXED) a->f->sync32_all();
+    else a->f->sync64_all();
     EX_HIP(hipStreamSynchronize(st));
     a->s->sweep_dev = a->sweep_ctr.as<uint64_t>();
     a->capturing = true;
@@ -666,15 +675,16 @@ void als_sweep(ex_als* a, int64_t J, uint64_t round) {
         if (replay) als_capture_sweep(a, J);
         return;
     }
-    if (graph_fingerprint(a) != a->graph_fp) als_capture_sweep(a, J);
     hipStream_t st = a->ctx->c.stream;
     ex_factors* f = a->f;
     const bool mixed = a->precision == EX_MIXED;
     // the host-side bookkeeping of the factor copies that the captured steps
     // did once: bring the copy the steps read up to date (a host write since
-    // the capture), and mark every mode's other copy stale afterwards
+    // the capture) before any new capture, and mark every mode's other copy
+    // stale afterwards
     if (mixed) f->sync32_all();
     else f->sync64_all();
+    if (graph_fingerprint(a) != a->graph_fp) als_capture_sweep(a, J);
     hipLaunchKernelGGL(set_u64, dim3(1), dim3(1), 0, st, a->sweep_ctr.as<uint64_t>(), round);
     EX_HIP(hipGetLastError());
     EX_HIP(hipGraphLaunch(a->gexec, st));
@@ -1186,6 +1196,8 @@ int ex_sampler_create(ex_ctx* ctx, ex_factors* f, int kind, int precision, uint6
         const int N = f->N, R = f->R;
         s->sums.ensure((size_t)N * 8);
         s->part.ensure(4096);
+        s->dropped.ensure(8);
+        EX_HIP(hipMemsetAsync(s->dropped.p, 0, 8, ctx->c.stream));
         if (precision == EX_MIXED) f->enable32();
         if (kind == EX_SAMPLER_STS_CP) {
             const int S = tri_stride(R);
@@ -1248,6 +1260,15 @@ int ex_sampler_draw(ex_sampler* s, int64_t J, int mode, uint64_t sweep, const do
         sampler_draw_impl(s, J, mode, sweep, dr, de, s->samples.as<uint32_t>(), s->weights.as<double>());
         EX_HIP(hipMemcpyAsync(samples, s->samples.p, (size_t)J * N * 4, hipMemcpyDeviceToHost, st));
         d2h(st, weights, s->weights.p, (size_t)J * 8);
+    });
+}
+
+int ex_sampler_dropped_draws(ex_sampler* s, int64_t* n) {
+    return guarded([&] {
+        EX_CHECK(s && n, EX_ERR_ARG, "null argument");
+        uint64_t v = 0;
+        d2h(s->ctx->c.stream, &v, s->dropped.p, 8);
+        *n = (int64_t)v;
     });
 }
 
@@ -1451,8 +1472,9 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
         als_resolve_regions(a);
         for (double& x : a->t_region) x = 0.0;
         a->grams_valid = false;
-        uint64_t c0[2];
+        uint64_t c0[2], z0 = 0;
         d2h(st, c0, a->cnt.p, sizeof(c0));
+        if (a->s) d2h(st, &z0, a->s->dropped.p, 8);
         double als_total = 0.0, fit_total = 0.0;
         std::vector<int64_t> rl{0};
         std::vector<double> fits{0.0}, als_times{0.0}, fit_times{0.0};
@@ -1473,8 +1495,9 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
             }
         }
         als_resolve_regions(a);
-        uint64_t c1[2];
+        uint64_t c1[2], z1 = 0;
         d2h(st, c1, a->cnt.p, sizeof(c1));
+        if (a->s) d2h(st, &z1, a->s->dropped.p, 8);
         const double nonzeros = (double)(c1[0] - c0[0]);
         std::string js = "{";
         js += "\"als_total_time\": " + json_num(als_total);
@@ -1496,6 +1519,7 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
         js += ", \"sampler_update_time\": " + dstat(a->t_region[6]);
         js += ", \"spmm_time\": " + dstat(a->t_region[3]);
         js += ", \"sum_nonzeros_iterated\": " + json_num(nonzeros);
+        js += ", \"zero_mass_draws\": " + std::to_string(z1 - z0);
         js += ", \"device\": \"" + device_label(a->ctx) + "\"";
         js += "}";
         a->stats = js;

@@ -148,11 +148,14 @@ namespace k {
 void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part,
           const uint64_t* m_dev = nullptr);
 // G = A^T A for FP32 A with FP64 products and sums
-void gram_f32in(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part);
+void gram_f32in(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part,
+                const uint64_t* m_dev = nullptr);
 void eigh(hipStream_t s, const double* A, int R, double* lam, double* V, int ncopies = 1);
-// only_if: when given, the launch does nothing unless *only_if != 0
+// only_if: when given, the launch does nothing unless *only_if != 0.
+// relative: eigenvalues <= tol * (largest diagonal entry of M) are dropped
+// instead of eigenvalues <= tol.
 void pinv(hipStream_t s, const double* M, int R, double tol, double* out, int ncopies = 1,
-          const int* only_if = nullptr);
+          const int* only_if = nullptr, bool relative = false);
 void dagat(hipStream_t s, const double* A, int64_t m, int R, const double* G, double* res);
 void dsymm(hipStream_t s, const double* G, const double* Y, int64_t m, int R, double* out);
 void col_sumsq(hipStream_t s, const double* U, int64_t m, int R, double* sigma, DevBuf& part);
@@ -193,7 +196,8 @@ struct StsModeArgs {
 void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* modes,
               const double* pinv_packed, uint64_t seed, uint64_t sweep,
               const double* draws_row, const double* draws_eig, uint32_t* samples, double* weights,
-              int node_bytes = 8, const uint64_t* sweep_dev = nullptr);
+              int node_bytes = 8, const uint64_t* sweep_dev = nullptr,
+              unsigned long long* dropped = nullptr);
 void pack_upper(hipStream_t s, const double* M, int R, double* packed);
 // sparse
 void keys_from_coords(hipStream_t s, const uint32_t* coords, int64_t n, const KeyLayout& kl, uint64_t* keys,

@@ -345,11 +345,25 @@ __device__ void pinv_from_jacobi(EighShared& sh, int R, double tol, double* __re
 
 template <int NB, int NV>
 __global__ __launch_bounds__(kJacobiThreads) void pinv_kernel(const double* __restrict__ Min, int R, double tol,
-                                                        double* __restrict__ out, const int* __restrict__ only_if) {
+                                                        double* __restrict__ out, const int* __restrict__ only_if,
+                                                        int relative) {
     if (only_if && *only_if == 0) return;   // the same for the whole block
     __shared__ EighShared sh;
     const int64_t R2 = (int64_t)R * R;
     load_upper(Min + blockIdx.x * R2, R, sh.A);
+    if (relative) {
+        // tolerance relative to the largest diagonal entry (read before the
+        // rotations change the diagonal)
+        __shared__ double dmax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0.0;
+            for (int a = 0; a < R; a++) m = fmax(m, sh.A[tri_index(R, a, a)]);
+            dmax = m;
+        }
+        __syncthreads();
+        tol *= dmax;
+    }
     jacobi_block<NB, NV>(sh.A, sh.V, R, sh.j, sh.sched);
     pinv_from_jacobi(sh, R, tol, out + blockIdx.x * R2);
 }
@@ -675,8 +689,8 @@ void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& p
     gram_mfma64<double>(s, A, m, R, G, part, m_dev);
 }
 
-void gram_f32in(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part) {
-    gram_mfma64<float>(s, A, m, R, G, part);
+void gram_f32in(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part, const uint64_t* m_dev) {
+    gram_mfma64<float>(s, A, m, R, G, part, m_dev);
 }
 
 void gram_final_launch(hipStream_t s, const double* part, int nb, int R, double* G) {
@@ -707,8 +721,9 @@ void eigh(hipStream_t s, const double* A, int R, double* lam, double* V, int nco
     EX_HIP(hipGetLastError());
 }
 
-void pinv(hipStream_t s, const double* M, int R, double tol, double* out, int ncopies, const int* only_if) {
-    JACOBI_DISPATCH(R, pinv_kernel, dim3(ncopies), s, M, R, tol, out, only_if);
+void pinv(hipStream_t s, const double* M, int R, double tol, double* out, int ncopies, const int* only_if,
+          bool relative) {
+    JACOBI_DISPATCH(R, pinv_kernel, dim3(ncopies), s, M, R, tol, out, only_if, relative ? 1 : 0);
     EX_HIP(hipGetLastError());
 }
 

@@ -711,7 +711,7 @@ void spd_pinv(hipStream_t s, const double* M, int R, double tol_rel, double* out
     // R <= 32: one launch (the pivoted-Cholesky pseudo-inverse in the same
     // workgroup); above, the Jacobi pinv as a second launch that reads flag
     spd_inverse_launch(s, M, R, tol_rel, out, flag, 1);
-    if (R > 32) pinv(s, M, R, tol_rel, out, 1, flag);
+    if (R > 32) pinv(s, M, R, tol_rel, out, 1, flag, true);
 }
 
 void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M) {

@@ -138,6 +138,7 @@ struct StsArgs {
     const double* draws_eig;
     uint32_t* samples;
     double* weights;
+    unsigned long long* dropped;   // when set, draws given weight 0 are counted here
     int vec_leaf;           // leaf rows read as W-wide vectors (R % W == 0, aligned factors)
 };
 
@@ -460,6 +461,7 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
     if (l == 0) {
         a.samples[s * N + j] = eig_idx;
         a.weights[s] = dropped ? 0.0 : (double)R / (q * (double)a.J);
+        if (dropped && a.dropped) atomicAdd(a.dropped, 1ull);
     }
 }
 
@@ -632,7 +634,7 @@ void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double
 
 void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* modes, const double* pinv_packed,
               uint64_t seed, uint64_t sweep, const double* draws_row, const double* draws_eig, uint32_t* samples,
-              double* weights, int node_bytes, const uint64_t* sweep_dev) {
+              double* weights, int node_bytes, const uint64_t* sweep_dev, unsigned long long* dropped) {
     EX_CHECK(node_bytes == 8 || node_bytes == 4, 1, "node precision must be FP64 or FP32");
     StsArgs a;
     for (int k = 0; k < N; k++) a.m[k] = modes[k];
@@ -641,6 +643,7 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
     a.seed = seed; a.sweep = sweep; a.sweep_dev = sweep_dev;
     a.draws_row = draws_row; a.draws_eig = draws_eig;
     a.samples = samples; a.weights = weights;
+    a.dropped = dropped;
     // vector reads of leaf rows: R a multiple of the vector width and every
     // drawn factor aligned to it
     const int vw = node_bytes == 8 ? 2 : 4;

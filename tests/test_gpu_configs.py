@@ -1,0 +1,207 @@
+"""The benchmark workloads themselves, through the C-ABI, against the oracle.
+
+* CP-ARLS-LEV on the 2000 x 2000 x 2000 tensor with 200,000 uniform nonzeros,
+  R = 16, J = 32,768 (full size, FP64): factors and fits of three sweeps.
+* STS-CP on the 1e6 x 1e6 x 1e5 tensor with 5e7 Zipf(1.1) nonzeros, R = 32,
+  J = 65,536 (full size): one FP64 sweep against the oracle; mixed against
+  FP64 final fit after ten sweeps each.
+* The 4-mode R = 64, J = 131,072 STS-CP shape (Zipf(1.2) on modes 0-1,
+  1 + Poisson(2) values) at a size the oracle finishes in seconds (every mode
+  size and the nonzero count divided down; the full 5e8-nonzero tensor takes
+  minutes to generate on the host): two FP64 sweeps against the oracle.
+* The batched CP-ARLS-LEV workload (all 256 tensors): the whole batch in one
+  batched run, factors finite except whole columns the reference's
+  renormalisation divides by zero, a subset equal to single runs, and FP64
+  sweeps of a few of its tensors against the oracle.
+
+Bars: factors 1e-9 relative, fits 1e-9 absolute, nonzeros iterated exact
+(FP64); |fit difference| < 1e-3 for mixed against FP64.
+"""
+import json
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+def setup_als(ctx, idx, vals, dims, U, kind, precision, seed):
+    from rdist_tensor_amd import abi
+    T = abi.Tensor(ctx, idx, vals, dims)
+    F = abi.Factors(ctx, dims, U[0].shape[1])
+    for k in range(len(dims)):
+        F.set(k, U[k])
+    S = abi.Sampler(ctx, F, kind, precision=precision, seed=seed)
+    A = abi.ALS(ctx, T, F, S, precision=precision)
+    A.init()
+    return T, F, S, A
+
+
+def test_config_arls_2000cube_full_size(ctx, orc):
+    from rdist_tensor_amd import abi, synthetic
+    cfg, idx, vals = synthetic.make("arls_2000cube")
+    dims, R, J = cfg["dims"], cfg["R"], cfg["J"]
+    U0 = synthetic.factors(dims, R, cfg["seed"])
+    T, F, S, A = setup_als(ctx, idx, vals, dims, U0, abi.EX_SAMPLER_CP_ARLS_LEV, abi.EX_FP64, cfg["seed"])
+    A.run(3, J, 1)
+    st = json.loads(A.stats_json())
+    oa = orc.ALS(idx, vals, dims, U0, "arls", seed=cfg["seed"])
+    fits, ost = oa.run(3, J, 1)
+    errs = [rel_err(F.get(k), oa.U[k]) for k in range(3)]
+    print(f"2000^3 ARLS: fits {st['fits'][1:]} vs {list(fits)}; factor rel err {max(errs):.2e}; "
+          f"nonzeros {st['sum_nonzeros_iterated']} vs {ost['nonzeros_iterated']}")
+    np.testing.assert_allclose(st["fits"][1:], fits, atol=1e-9, rtol=0)
+    assert max(errs) < 1e-9
+    assert st["sum_nonzeros_iterated"] == ost["nonzeros_iterated"]
+
+
+@pytest.fixture(scope="module")
+def sts_1e6():
+    from rdist_tensor_amd import synthetic
+    cfg, idx, vals = synthetic.make("sts_1e6x1e6x1e5")
+    U0 = synthetic.factors(cfg["dims"], cfg["R"], cfg["seed"])
+    return cfg, idx, vals, U0
+
+
+@pytest.mark.timeout(400)
+def test_config_sts_1e6_fp64_sweep_vs_oracle(ctx, orc, sts_1e6):
+    from rdist_tensor_amd import abi
+    cfg, idx, vals, U0 = sts_1e6
+    dims, R, J = cfg["dims"], cfg["R"], cfg["J"]
+    T, F, S, A = setup_als(ctx, idx, vals, dims, U0, abi.EX_SAMPLER_STS_CP, abi.EX_FP64, cfg["seed"])
+    A.run(1, J, 1)
+    st = json.loads(A.stats_json())
+    oa = orc.ALS(idx, vals, dims, U0, "sts", seed=cfg["seed"])
+    fits, ost = oa.run(1, J, 1)
+    errs = [rel_err(F.get(k), oa.U[k]) for k in range(3)]
+    print(f"1e6 x 1e6 x 1e5 STS-CP FP64, one sweep: fit {st['fits'][1]:.12f} vs {fits[0]:.12f}; "
+          f"factor rel err {max(errs):.2e}; nonzeros {st['sum_nonzeros_iterated']} vs {ost['nonzeros_iterated']}",
+          flush=True)
+    assert abs(st["fits"][1] - fits[0]) < 1e-9
+    assert max(errs) < 1e-9
+    assert st["sum_nonzeros_iterated"] == ost["nonzeros_iterated"]
+    assert st["zero_mass_draws"] == 0
+
+
+def test_config_sts_1e6_mixed_vs_fp64_fit(ctx, sts_1e6):
+    from rdist_tensor_amd import abi
+    cfg, idx, vals, U0 = sts_1e6
+    dims, R, J = cfg["dims"], cfg["R"], cfg["J"]
+    fits = {}
+    for prec in (abi.EX_FP64, abi.EX_MIXED):
+        T, F, S, A = setup_als(ctx, idx, vals, dims, U0, abi.EX_SAMPLER_STS_CP, prec, cfg["seed"])
+        A.use_graph(True)
+        A.run(10, J, 5)
+        st = json.loads(A.stats_json())
+        fits[prec] = st["fits"][1:]
+        print(f"precision {prec}: fits {fits[prec]}, draws given weight 0: {st['zero_mass_draws']}", flush=True)
+        assert st["zero_mass_draws"] <= 10 * 3 * J * 1e-4
+        del A, S, F, T
+    assert abs(fits[abi.EX_FP64][-1] - fits[abi.EX_MIXED][-1]) < 1e-3
+
+
+@pytest.mark.timeout(300)
+def test_config_four_mode_rank64_vs_oracle(ctx, orc):
+    from rdist_tensor_amd import abi, synthetic
+    dims = [100_000, 10_000, 1_000, 100]
+    R, J = 64, 131072
+    idx = synthetic.coords(dims, 1_000_000, 2, [1.2, 1.2, None, None])
+    vals = synthetic.values(len(idx), 2, "poisson")
+    U0 = synthetic.factors(dims, R, 2)
+    T, F, S, A = setup_als(ctx, idx, vals, dims, U0, abi.EX_SAMPLER_STS_CP, abi.EX_FP64, 2)
+    A.run(2, J, 1)
+    st = json.loads(A.stats_json())
+    oa = orc.ALS(idx, vals, dims, U0, "sts", seed=2)
+    fits, ost = oa.run(2, J, 1)
+    errs = [rel_err(F.get(k), oa.U[k]) for k in range(4)]
+    print(f"4-mode R=64: fits {st['fits'][1:]} vs {list(fits)}; factor rel err {max(errs):.2e}; "
+          f"nonzeros {st['sum_nonzeros_iterated']} vs {ost['nonzeros_iterated']}", flush=True)
+    np.testing.assert_allclose(st["fits"][1:], fits, atol=1e-9, rtol=0)
+    assert max(errs) < 1e-9
+    assert st["sum_nonzeros_iterated"] == ost["nonzeros_iterated"]
+
+
+def _nan_columns(U):
+    """Columns of U that are NaN; every other column must be finite."""
+    bad = np.isnan(U)
+    cols = bad.any(axis=0)
+    assert np.array_equal(bad, np.broadcast_to(cols, U.shape)), "NaN entries outside whole columns"
+    assert np.all(np.isfinite(U[:, ~cols]))
+    return int(cols.sum())
+
+
+@pytest.mark.timeout(400)
+def test_config_batched_arls_full_batch():
+    """All 256 tensors of the batched workload, mixed precision, three
+    batched sweeps: NaN appears only as whole columns (an exactly zero column
+    of the update, which the reference's renormalisation divides by zero as
+    well) and is counted; eight of the optimizers run alone reach the batched
+    factors bitwise."""
+    from rdist_tensor_amd import abi, synthetic
+    batch = synthetic.make_batch(count=256, seed=4)
+    J, rounds = 8192, 3
+    objs = []
+    for dims, idx, vals, R, st in batch:
+        c = abi.Context(0)
+        T, F, S, A = setup_als(c, idx, vals, dims, synthetic.factors(dims, R, st),
+                               abi.EX_SAMPLER_CP_ARLS_LEV, abi.EX_MIXED, st)
+        objs.append((c, T, F, S, A))
+    try:
+        sec = abi.ALS.run_batch([o[4] for o in objs], rounds, J)
+        nan_cols = nan_factors = 0
+        for c, T, F, S, A in objs:
+            for k in range(3):
+                n = _nan_columns(F.get(k))
+                nan_cols += n
+                nan_factors += n > 0
+        print(f"batch of {len(objs)}: {rounds} sweeps in {sec:.2f} s; factors with NaN columns "
+              f"{nan_factors} of {3 * len(objs)} ({nan_cols} columns)", flush=True)
+        assert nan_factors <= 0.02 * 3 * len(objs)
+        batched = [[objs[t][2].get(k) for k in range(3)] for t in range(8)]
+    finally:
+        for c, *_ in objs:
+            c.close()
+    for t in range(8):
+        dims, idx, vals, R, st = batch[t]
+        c = abi.Context(0)
+        try:
+            T, F, S, A = setup_als(c, idx, vals, dims, synthetic.factors(dims, R, st),
+                                   abi.EX_SAMPLER_CP_ARLS_LEV, abi.EX_MIXED, st)
+            A.run(rounds, J, 10 ** 9)
+            for k in range(3):
+                np.testing.assert_array_equal(F.get(k), batched[t][k])
+        finally:
+            c.close()
+
+
+def test_config_batched_arls_fp64_vs_oracle(ctx, orc):
+    """FP64 sweeps of four tensors of the batched workload against the oracle
+    (NaN columns, where a sweep produces them, must coincide)."""
+    from rdist_tensor_amd import abi, synthetic
+    batch = synthetic.make_batch(count=4, seed=4)
+    J = 8192
+    for dims, idx, vals, R, st in batch:
+        U0 = synthetic.factors(dims, R, st)
+        T, F, S, A = setup_als(ctx, idx, vals, dims, U0, abi.EX_SAMPLER_CP_ARLS_LEV, abi.EX_FP64, st)
+        A.run(2, J, 1)
+        s = json.loads(A.stats_json())
+        oa = orc.ALS(idx, vals, dims, U0, "arls", seed=st)
+        fits, ost = oa.run(2, J, 1)
+        worst = 0.0
+        for k in range(3):
+            g, o = F.get(k), oa.U[k]
+            assert np.array_equal(np.isnan(g), np.isnan(o))
+            ok = ~np.isnan(o)
+            worst = max(worst, rel_err(g[ok], o[ok]))
+        print(f"dims {dims} R={R}: factor rel err {worst:.2e}; nonzeros {s['sum_nonzeros_iterated']} "
+              f"vs {ost['nonzeros_iterated']}", flush=True)
+        assert worst < 1e-9
+        assert s["sum_nonzeros_iterated"] == ost["nonzeros_iterated"]
+        del A, S, F, T

@@ -243,8 +243,13 @@ def test_sts_compute_m(ctx, orc):
                 assert np.abs(np.abs(Mg[slot]) - np.abs(Mo)).max() < 1e-9 * max(1.0, np.abs(Mo).max()), (j, slot)
 
 
-@pytest.mark.parametrize("R", [8, 16, 32])
+@pytest.mark.parametrize("R", [8, 16, 25, 32, 40, 64])
 def test_sts_draw_bitexact(ctx, orc, R):
+    """FP64 STS-CP draws with injected uniforms against the oracle, index by
+    index, at every FP64 instantiation of the draw kernel: 16 lanes per
+    sample (R <= 16), 32 (R <= 32; R = 25 reads leaf rows element by element,
+    the others as vectors), 64 lanes with nine (R <= 45) or seventeen vectors
+    per lane (R > 45)."""
     from rdist_tensor_amd import abi
     rng = np.random.default_rng(41 + R)
     dims = [3000, 1000, 517]

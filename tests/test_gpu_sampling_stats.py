@@ -131,3 +131,46 @@ def test_chi_square_has_power(ctx):
     pval, chi2, df = chi_square_p(counts, p_sts, J)
     print(f"ARLS draws against STS probabilities: chi2 {chi2:.1f} df {df} p {pval:.3g}")
     assert pval < 1e-6
+
+
+@pytest.mark.parametrize("precision,R", [("mixed", 16), ("mixed", 25), ("mixed", 32), ("mixed", 40),
+                                         ("mixed", 64), ("fp64", 40), ("fp64", 64)])
+def test_sts_draw_distribution_every_rank_class(ctx, precision, R):
+    """STS-CP draws at every lane layout of the draw kernel against exact
+    leverage (chi-square, total variation, weights): mixed precision with 16
+    lanes per sample (R <= 16), 8 lanes and seventeen float4 per lane (R <=
+    32; R = 25 reads leaf rows element by element), 32 lanes (R <= 45) and 64
+    lanes (R > 45); FP64 with 64 lanes and nine or seventeen double2 per lane.
+    Mode sizes above R keep every factor Gram full rank.  Draws given weight 0
+    (a zero-mass leaf reached through FP32 rounding) are counted by the
+    library and must stay below one in 10^4."""
+    from rdist_tensor_amd import abi
+    dims = [80, 70, 60]
+    U, F = make_factors(ctx, dims, R, seed=70 + R, skew=4.0)
+    prec = abi.EX_MIXED if precision == "mixed" else abi.EX_FP64
+    S = abi.Sampler(ctx, F, abi.EX_SAMPLER_STS_CP, precision=prec, seed=321 + R)
+    J = 200_000
+    wtol = 5e-2 if precision == "mixed" else 1e-9
+    total = 0
+    for j in range(3):
+        p, modes = exact_sts_probs(U, j)
+        samples, w = S.draw(J, j, sweep=5 + j)
+        total += J
+        b = bins_of(samples, modes, dims)
+        assert b.min() >= 0 and b.max() < len(p)
+        counts = np.bincount(b, minlength=len(p)).astype(np.float64)
+        pval, chi2, df = chi_square_p(counts, p, J)
+        tv = 0.5 * np.abs(counts / J - p).sum()
+        tv_exp = 0.5 * np.sqrt(2.0 * p * (1.0 - p) / (np.pi * J)).sum()
+        ok = w > 0
+        werr = np.max(np.abs(w[ok] * J * p[b[ok]] - 1.0))
+        print(f"sts/{precision} R={R} mode {j}: chi2 {chi2:.1f} df {df} p {pval:.3g} TV {tv:.4f} "
+              f"(exp {tv_exp:.4f}) max weight rel err {werr:.2e} zero-weight {int((~ok).sum())}")
+        assert pval > 1e-4
+        assert tv < 1.3 * tv_exp
+        assert werr < wtol
+    dropped = S.dropped_draws()
+    print(f"sts/{precision} R={R}: draws given weight 0: {dropped} of {total}")
+    assert dropped <= total * 1e-4
+    if precision == "fp64":
+        assert dropped == 0
