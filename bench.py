@@ -52,22 +52,52 @@ def tree_levels(n, F):
     return lvl
 
 
+def tree_nodes_per_level(n, F):
+    """Node counts per level of the PartitionTree heap over ceil(n/F) leaves
+    (root first; the last level may be partial)."""
+    leaves = -(-n // F)
+    total = 2 * leaves - 1
+    out, d = [], 0
+    while sum(out) < total:
+        out.append(min(1 << d, total - sum(out)))
+        d += 1
+    return out
+
+
 def sts_draw_bytes(dims, R, J, j, s=8):
-    """Algorithmic bytes of one STS-CP draw (all non-target modes): node reads
-    per sample per level (packed upper R(R+1)/2 values, before cache reuse;
-    eigen-tree nodes FP64, row-tree nodes s bytes), the leaf's F = R rows and
-    the chosen row (s bytes), the eigenvector row (FP64), samples and weights."""
+    """Algorithmic (compulsory) bytes of one STS-CP draw over every non-target
+    mode: each distinct tree node at most once (a level holds at most J
+    distinct visited nodes; packed upper R(R+1)/2 values at s bytes, the
+    eigen tree's too), each distinct leaf block of F = R factor rows at most
+    once, the R x R eigenvector rows and the packed pinv (FP64), samples and
+    weights written.  Re-reads of shared upper levels by many samples are
+    cache traffic, not counted."""
     P = tri(R)
     total = 0
     for k, n in enumerate(dims):
         if k == j:
             continue
-        le = tree_levels(R, 1) + 2          # root + levels + tail (eigen tree)
-        lr = tree_levels(n, R) + 2          # root + levels + tail (row tree)
-        total += J * (le * P * 8 + lr * P * s + R * R * s + R * s + R * 8)
-    total += J * P * 8                      # weight: h^T pinv h
+        for lv in tree_nodes_per_level(R, 1) + tree_nodes_per_level(n, R):
+            total += min(lv, J) * P * s
+        leaves = -(-n // R)
+        total += min(leaves, J) * R * R * s + R * R * 8
+    total += P * 8                          # packed pinv
     total += J * (len(dims) * 4 + 8)        # samples + weights written
     return total
+
+
+def tree_build_bytes(dims, R, j, s=8):
+    """One row-tree rebuild: the factor read once, every node written once."""
+    n = dims[j]
+    nodes = 2 * (-(-n // R)) - 1
+    return n * R * s + nodes * tri(R) * s
+
+
+def factor_update_bytes(dims, R, j, s=8):
+    """U_j = Y G+ and the column norms: FP64 reads Y, writes U, reads and
+    writes U again to scale (4 passes); mixed reads Y twice (its Gram, then
+    the product) and writes U once (3 passes)."""
+    return (4 if s == 8 else 3) * dims[j] * R * s
 
 
 def mttkrp_bytes(N, R, Ju, H, T, s=8):
@@ -131,18 +161,19 @@ def run_gpu(args, cfg, ctx, T, U0, precision):
     log(f"[bench] {precision}: warmup {args.warmup} sweeps done")
     graphs = bool(args.graphs)
     if graphs:
-        # the throughput pass: each sweep replayed from a captured HIP graph
-        # (the capture is made by one untimed sweep), no timing events
+        # the throughput pass: each sweep replayed from a captured HIP graph.
+        # One untimed sweep sizes the buffers and captures the graph; the
+        # factors and samplers are then reset from the host (the replays
+        # read the rewritten factors), so every timed sweep is a replay.
         A.use_graph(True)
         try:
             abi.ALS.run_batch([A], 1, J)
         except abi.ExafacError as e:   # a capture the library refuses: time the step-by-step pass
             log(f"[bench] {precision}: no graph capture ({e}); timing the step-by-step pass")
             graphs = False
-        A.use_graph(False)
+            A.use_graph(False)
         reset()
     if graphs:
-        A.use_graph(True)
         t0 = time.perf_counter()
         abi.ALS.run_batch([A], args.steps, J)
         ctx.sync()
@@ -151,18 +182,30 @@ def run_gpu(args, cfg, ctx, T, U0, precision):
         fit_tp = A.fit()
         reset()
     # the instrumented pass: the same sweeps (sweep numbers 1..K) enqueued
-    # step by step with the span events on, for the per-kernel figures
+    # step by step with the span events on, for the per-kernel figures; the
+    # fit after every sweep and the factors at the checkpoints are taken
+    # outside the spans
     ctx.timing(True)
     ctx.sync()
+    fits, ckpt = [], {}
+    dropped0 = S.dropped_draws()
     t0 = time.perf_counter()
     for sweep in range(1, args.steps + 1):
         for j in range(N):
             A.step(j, J, sweep)
+        ctx.sync()
+        if args.trajectory:
+            ctx.timing(False)
+            fits.append(A.fit())
+            if sweep in checkpoints(args.steps):
+                ckpt[sweep] = [F.get(k) * F.sigma()[None, :] for k in range(N)]
+            ctx.timing(True, clear=False)
     ctx.sync()
     elapsed_inst = time.perf_counter() - t0
     timing = json.loads(ctx.timing_json())
     ctx.timing(False)
     fit1 = A.fit()
+    dropped = S.dropped_draws() - dropped0
     if not graphs:
         elapsed, fit_tp = elapsed_inst, fit1
     log(f"[bench] {precision}: {args.steps} sweeps in {elapsed:.3f} s "
@@ -170,7 +213,24 @@ def run_gpu(args, cfg, ctx, T, U0, precision):
         f"fit {fit0:.6f} -> {fit1:.6f}")
     del A, S, F
     return dict(elapsed=elapsed, elapsed_instrumented=elapsed_inst, timing=timing, fit0=fit0, fit1=fit1,
-                passes_agree=bool(fit_tp == fit1), timed_by="graph replay" if graphs else "step by step")
+                passes_agree=bool(fit_tp == fit1), timed_by="graph replay" if graphs else "step by step",
+                fits=fits, checkpoints=ckpt, dropped=dropped)
+
+
+def checkpoints(K):
+    """Sweeps at which the factors of both precisions are compared."""
+    return sorted({1, 2, 5, 10, 20, K} & set(range(1, K + 1)))
+
+
+def factor_errors(a, b):
+    """Per checkpoint: the largest relative Frobenius difference over the modes
+    between the mixed path's factors (columns scaled by sigma) and the FP64
+    path's."""
+    out = {}
+    for sweep in sorted(set(a) & set(b)):
+        out[str(sweep)] = max(float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+                              for x, y in zip(a[sweep], b[sweep]))
+    return out
 
 
 def run_exact(args, cfg, ctx, T, U0, precision, nnz):
@@ -217,34 +277,57 @@ def run_exact(args, cfg, ctx, T, U0, precision, nnz):
 
 def summarize(args, cfg, g, vbytes, precision):
     """Throughput, dominant launch group and its roofline for one GPU run.
-    The candidates are the STS-CP draw (one launch per mode step) and the
-    sampled MTTKRP + KRP gather (one launch group per mode step); the one
-    with the larger total time is the dominant kernel."""
-    N, R, J = len(cfg["dims"]), cfg["R"], cfg["J"]
+    Candidates: the STS-CP draw, the sampled MTTKRP + KRP gather, the row-tree
+    rebuild and the factor update (HBM-bound: algorithmic bytes over the
+    average launch), and computeM (one-workgroup R x R eigen / Cholesky
+    solves, priced at its FP64 flops against the dense FP64 matrix peak).
+    The candidate with the largest total time is the dominant kernel."""
+    dims = cfg["dims"]
+    N, R, J = len(dims), cfg["R"], cfg["J"]
     K = args.steps
     spans = g["timing"]["spans"]
     cnt = g["timing"]["counters"]
     steps = max(cnt.get("mode_steps", 1), 1)
     traffic = load_traffic(os.path.join(ROOT, "profiles", "hbm_traffic.md"))
+    mean = lambda f: float(np.mean([f(j) for j in range(N)]))
     roofs = {}
+    def per_launch(name):
+        st = spans[name]
+        return st["ms"], st["ms"] / st["count"] * 1e-3
     if "sts_draw" in spans:
-        st = spans["sts_draw"]
-        avg = st["ms"] / st["count"] * 1e-3
-        per = float(np.mean([sts_draw_bytes(cfg["dims"], R, J, j, s=vbytes) for j in range(N)]))
-        roofs["sts_draw"] = (st["ms"], avg, per)
+        tot, avg = per_launch("sts_draw")
+        roofs["sts_draw"] = (tot, avg, mean(lambda j: sts_draw_bytes(dims, R, J, j, s=vbytes)), "hbm")
     if "sampled_mttkrp" in spans and cnt.get("mode_steps"):
         ms = spans["sampled_mttkrp"]["ms"] + spans.get("design_gather", {"ms": 0.0})["ms"]
-        avg = ms / steps * 1e-3
         per = mttkrp_bytes(N, R, cnt["J_u"] / steps, cnt["H"] / steps, cnt["T"] / steps, s=vbytes)
-        roofs["sampled_mttkrp"] = (ms, avg, per)
+        roofs["sampled_mttkrp"] = (ms, ms / steps * 1e-3, per, "hbm")
+    if "tree_build" in spans:
+        tot, avg = per_launch("tree_build")
+        roofs["tree_build"] = (tot, avg, mean(lambda j: tree_build_bytes(dims, R, j, s=vbytes)), "hbm")
+    if "factor_update" in spans:
+        tot, avg = per_launch("factor_update")
+        roofs["factor_update"] = (tot, avg, mean(lambda j: factor_update_bytes(dims, R, j, s=vbytes)), "hbm")
+    if "sts_compute_m" in spans:
+        tot, avg = per_launch("sts_compute_m")
+        # (N + 1) symmetric R x R factorisations (eigendecompositions or
+        # Cholesky) of about R^3 flops each, plus N Hadamard products
+        roofs["sts_compute_m"] = (tot, avg, float((N + 1) * R ** 3 + N * R * R), "mfma")
     out = {}
-    for name, (tot, avg, per) in roofs.items():
-        achieved = per / avg / 1e9
+    for name, (tot, avg, per, bound) in roofs.items():
         key = f"{name}/{precision}/{args.config}"
-        out[name] = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic.get(key) if args.scale == 1.0 else None,
-                     "avg_launch_ms": round(avg * 1e3, 4), "algorithmic_bytes_per_launch": int(per)}
+        if bound == "hbm":
+            achieved = per / avg / 1e9
+            out[name] = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": traffic.get(key) if args.scale == 1.0 else None,
+                         "avg_launch_ms": round(avg * 1e3, 4), "algorithmic_bytes_per_launch": int(per)}
+        else:
+            achieved = per / avg / 1e12
+            out[name] = {"kernel": name, "bound": "mfma", "achieved": round(achieved, 6),
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 8),
+                         "traffic": None, "avg_launch_ms": round(avg * 1e3, 4),
+                         "algorithmic_flops_per_launch": int(per)}
+        out[name]["ms_per_sweep"] = round(tot / K, 4)
     dom = max(roofs, key=lambda n: roofs[n][0]) if roofs else None
     mt = out.get("sampled_mttkrp")
     return dict(value=K / g["elapsed"], ms_per_step=g["elapsed"] / K * 1e3,
@@ -256,7 +339,8 @@ def summarize(args, cfg, g, vbytes, precision):
                 instrumented_ms_per_step=round(g["elapsed_instrumented"] / K * 1e3, 3),
                 timed_by=g["timed_by"],
                 throughput_and_instrumented_fits_equal=g["passes_agree"],
-                fit={"initial": g["fit0"], "final": g["fit1"]})
+                fit={"initial": g["fit0"], "final": g["fit1"]},
+                fits_per_sweep=g["fits"], zero_mass_draws=g.get("dropped"))
 
 
 def run_cpu_baseline(cfg, idx, vals, U0, budget_s=30.0):
@@ -371,6 +455,8 @@ def main():
                     help="time the sweeps replayed from a captured HIP graph (1) or enqueued step by step (0); "
                          "per-kernel figures always come from a second, instrumented eager pass")
     ap.add_argument("--batch-count", type=int, default=256, help="tensors of the batched workload")
+    ap.add_argument("--trajectory", type=int, default=1,
+                    help="instrumented pass: fit after every sweep and factor checkpoints (mixed vs FP64)")
     ap.add_argument("--batch-graphs", type=int, default=0,
                     help="batched workload: replay each optimizer's sweep from a HIP graph (1) or enqueue it (0)")
     args = ap.parse_args()
@@ -396,9 +482,10 @@ def main():
     T.build_lookups()
     ctx.sync()
     log(f"[bench] device {dev}; tensor upload + lookups {time.time() - t0:.1f} s")
-    res = {}
+    res, ckpts = {}, {}
     for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
         g = run_gpu(args, cfg, ctx, T, U0, prec)
+        ckpts[prec] = g.pop("checkpoints")
         res[prec] = summarize(args, cfg, g, 4 if prec == "mixed" else 8, prec)
     exact = {}
     if args.exact_steps > 0:
@@ -440,10 +527,13 @@ def main():
         "device": dev,
     }
     if "mixed" in res and "fp64" in res:
+        fm, fd = res["mixed"]["fits_per_sweep"], res["fp64"]["fits_per_sweep"]
         out["mixed_vs_fp64"] = {"speedup": round(res["mixed"]["value"] / res["fp64"]["value"], 3),
                                 "final_fit_mixed": res["mixed"]["fit"]["final"],
                                 "final_fit_fp64": res["fp64"]["fit"]["final"],
-                                "final_fit_abs_diff": abs(res["mixed"]["fit"]["final"] - res["fp64"]["fit"]["final"])}
+                                "final_fit_abs_diff": abs(res["mixed"]["fit"]["final"] - res["fp64"]["fit"]["final"]),
+                                "fit_abs_diff_per_sweep": [round(abs(a - b), 8) for a, b in zip(fm, fd)],
+                                "factor_rel_err_at_sweep": factor_errors(ckpts["mixed"], ckpts["fp64"])}
     out["paths"] = res
     if exact:
         out["exact_als"] = exact

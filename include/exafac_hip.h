@@ -69,7 +69,9 @@ int ex_ctx_sync(ex_ctx* ctx);
 int ex_ctx_device_name(ex_ctx* ctx, char* buf, size_t len);
 /* Per-launch-group timing on the context stream (no reference counterpart:
  * the reference's seven region timers, als_optimizer.hpp:21-27, are host
- * wall clocks).  Enabling clears the totals.  ex_ctx_timing_json returns
+ * wall clocks).  enable: 1 starts recording and clears the totals, 2 resumes
+ * recording and keeps them, 0 stops recording (the totals stay readable).
+ * ex_ctx_timing_json returns
  * {"spans": {name: {"ms": total, "count": n}}, "counters": {...}} with the
  * same buffer protocol as ex_als_stats_json. */
 int ex_ctx_timing(ex_ctx* ctx, int enable);
@@ -102,9 +104,12 @@ int ex_sampled_mttkrp(ex_tensor* t, int mode, int64_t Ju, const uint32_t* sample
  * counterpart; the results agree to rounding).  EX_MTTKRP_ROW_PASS reads
  * every nonzero's row and fiber number in row order; EX_MTTKRP_HIT_LIST
  * marks the sampled fibers' nonzeros, compacts the marks into a list of hit
- * positions in row order and visits only those; EX_MTTKRP_AUTO (default)
- * picks per step from nnz, the fiber count and J_u. */
-enum ex_mttkrp_path { EX_MTTKRP_AUTO = 0, EX_MTTKRP_ROW_PASS = 1, EX_MTTKRP_HIT_LIST = 2 };
+ * positions in row order and visits only those; EX_MTTKRP_BUCKETS reads the
+ * sampled fibers' nonzeros in key order, scatters them into buckets of
+ * row-order positions and sums each bucket in row order (traffic follows
+ * the hits; fewer than 2^31 nonzeros); EX_MTTKRP_AUTO (default) picks per
+ * step from nnz, the fiber count and J_u. */
+enum ex_mttkrp_path { EX_MTTKRP_AUTO = 0, EX_MTTKRP_ROW_PASS = 1, EX_MTTKRP_HIT_LIST = 2, EX_MTTKRP_BUCKETS = 3 };
 int ex_tensor_mttkrp_path(ex_tensor* t, int path);
 
 /* ---- differentiable sampled MTTKRP ------------------------------------- */

@@ -19,7 +19,7 @@ EX_FP64, EX_MIXED = 0, 1
 EX_SAMPLER_CP_ARLS_LEV, EX_SAMPLER_STS_CP = 0, 1
 EX_DIST_ACCUMULATOR_STATIONARY, EX_DIST_TENSOR_STATIONARY = 0, 1
 EX_PRE_NONE, EX_PRE_LOG_COUNT, EX_PRE_ONES = 0, 1, 2
-EX_MTTKRP_AUTO, EX_MTTKRP_ROW_PASS, EX_MTTKRP_HIT_LIST = 0, 1, 2
+EX_MTTKRP_AUTO, EX_MTTKRP_ROW_PASS, EX_MTTKRP_HIT_LIST, EX_MTTKRP_BUCKETS = 0, 1, 2, 3
 
 _vp = C.c_void_p
 _d = C.POINTER(C.c_double)
@@ -177,8 +177,10 @@ class Context:
     def sync(self):
         check(lib().ex_ctx_sync(self.h))
 
-    def timing(self, enable=True):
-        check(lib().ex_ctx_timing(self.h, 1 if enable else 0))
+    def timing(self, enable=True, clear=True):
+        """Span timing on (clearing the totals unless clear=False) or off
+        (the totals stay readable)."""
+        check(lib().ex_ctx_timing(self.h, (1 if clear else 2) if enable else 0))
 
     def timing_json(self):
         n = C.c_size_t(0)

@@ -594,7 +594,7 @@ std::vector<const void*> graph_fingerprint(const ex_als* a) {
         add(f->U32[k2]);
         const ModeLookup& L = t->lk[k2];
         for (const DevBuf* b : {&L.fkey, &L.rrow, &L.rfib, &L.rval, &L.rval32, &L.cmap, &L.filt, &L.fptr, &L.kpos,
-                                &L.cpos, &L.marks, &L.cpart})
+                                &L.cpos, &L.marks, &L.cpart, &L.krow, &L.kval, &L.kval32})
             add(*b);
     }
     for (const DevBuf* b : {&f->sigma, &f->part, &f->gram, &t->coords, &t->vals}) add(*b);
@@ -877,12 +877,15 @@ int ex_ctx_sync(ex_ctx* ctx) {
 int ex_ctx_timing(ex_ctx* ctx, int enable) {
     return guarded([&] {
         EX_CHECK(ctx, EX_ERR_ARG, "null context");
+        EX_CHECK(enable >= 0 && enable <= 2, EX_ERR_ARG, "enable must be 0, 1 or 2");
         EX_HIP(hipStreamSynchronize(ctx->c.stream));
         ctx->resolve();
         ctx->timing = enable != 0;
-        ctx->totals.clear();
-        ctx->counters.clear();
-        ctx->watches.clear();
+        if (enable == 1) {
+            ctx->totals.clear();
+            ctx->counters.clear();
+            ctx->watches.clear();
+        }
     });
 }
 
@@ -978,8 +981,9 @@ int ex_tensor_build_lookups(ex_tensor* t) {
 int ex_tensor_mttkrp_path(ex_tensor* t, int path) {
     return guarded([&] {
         EX_CHECK(t, EX_ERR_ARG, "null tensor");
-        EX_CHECK(path == EX_MTTKRP_AUTO || path == EX_MTTKRP_ROW_PASS || path == EX_MTTKRP_HIT_LIST, EX_ERR_ARG,
-                 "unknown sampled-MTTKRP path");
+        EX_CHECK(path == EX_MTTKRP_AUTO || path == EX_MTTKRP_ROW_PASS || path == EX_MTTKRP_HIT_LIST ||
+                     path == EX_MTTKRP_BUCKETS,
+                 EX_ERR_ARG, "unknown sampled-MTTKRP path");
         for (int k2 = 0; k2 < kMaxModes; k2++) t->lk[k2].mttkrp_path = path;
     });
 }

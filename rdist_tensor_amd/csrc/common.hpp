@@ -130,8 +130,12 @@ struct ModeLookup {
     DevBuf cpos;    // u32[nnz]: design row of the hit at a row-order position (valid where its bit is set)
     DevBuf marks;   // u8, whole 4096-position chunks: 1 at this step's hit positions (zero between steps)
     DevBuf cpart;   // u64 per-workgroup counter partials of the passes
+    // bucketed path (built on first use): key-order target row and value of each nonzero
+    DevBuf krow;    // u32[nnz]
+    DevBuf kval;    // f64[nnz]
+    DevBuf kval32;  // f32[nnz]
     int64_t maxfib = 0;    // nonzeros in the longest fiber
-    int mttkrp_path = 0;   // 0: chosen per step from the sizes, 1: full row pass, 2: hit bitmap
+    int mttkrp_path = 0;   // 0: chosen per step from the sizes, 1: full row pass, 2: hit bitmap, 3: bucketed hits
 };
 
 struct Ctx {

@@ -182,7 +182,6 @@ template <> struct Vec<double> {
 
 // LDS of one draw block: the packed-upper (a, b) table, and per sample the
 // running Hadamard product h, the current query and the leaf's q^2.
-constexpr int kDrawThreads = 256;
 template <typename TN, int SLOTS, int RMAX>
 struct DrawShared {
     uint16_t pairs[RMAX * (RMAX + 1) / 2];
@@ -350,24 +349,35 @@ __device__ __forceinline__ int leaf_pick(const double* q2, int F, double frac, i
     return F - 1;
 }
 
+// x^T G x of node c of a tree whose first nst nodes are staged in LDS
+// (snodes, same stride S): the staged copy when it holds c, else the node in
+// global memory.  The nodes of the top levels are staged, so every group of
+// a wavefront takes the same branch at a given level.
+template <typename TN, int G, int NV>
+__device__ __forceinline__ double quad_node(const TN* __restrict__ gnodes, const TN* snodes, int nst, int64_t c,
+                                            int S, const TN (&xx)[NV * Vec<TN>::W], int P, int l) {
+    if (c < nst) return quad_g<TN, G, NV>(snodes + c * S, xx, P, l);
+    return quad_g<TN, G, NV>(gnodes + c * S, xx, P, l);
+}
+
 // Root-to-leaf descent of one heap tree (partition_tree.hpp:226-307): masses
 // and cutoffs in FP64, left if u <= low + m_left / m_root.
 template <typename TN, int G, int NV>
-__device__ __forceinline__ int64_t descend_g(const TN* __restrict__ nodes, const TreeGeom& g, int S, int P,
-                                             const TN (&xx)[NV * Vec<TN>::W], int l, double u, double& low,
-                                             double& high) {
-    const double m = quad_g<TN, G, NV>(nodes, xx, P, l);
+__device__ __forceinline__ int64_t descend_g(const TN* __restrict__ nodes, const TN* snodes, int nst,
+                                             const TreeGeom& g, int S, int P, const TN (&xx)[NV * Vec<TN>::W], int l,
+                                             double u, double& low, double& high) {
+    const double m = quad_node<TN, G, NV>(nodes, snodes, nst, 0, S, xx, P, l);
     int64_t c = 0;
     low = 0.0;
     high = 1.0;
     for (uint32_t lvl = 0; lvl < g.lfill_level; lvl++) {
-        const double mL = quad_g<TN, G, NV>(nodes + (2 * c + 1) * S, xx, P, l);
+        const double mL = quad_node<TN, G, NV>(nodes, snodes, nst, 2 * c + 1, S, xx, P, l);
         const double cutoff = low + mL / m;
         if (u <= cutoff) { c = 2 * c + 1; high = cutoff; }
         else { c = 2 * c + 2; low = cutoff; }
     }
     if (!g.is_leaf(c)) {
-        const double mL = quad_g<TN, G, NV>(nodes + (2 * c + 1) * S, xx, P, l);
+        const double mL = quad_node<TN, G, NV>(nodes, snodes, nst, 2 * c + 1, S, xx, P, l);
         const double cutoff = low + mL / m;
         if (u <= cutoff) { c = 2 * c + 1; high = cutoff; }
         else { c = 2 * c + 2; low = cutoff; }
@@ -375,17 +385,51 @@ __device__ __forceinline__ int64_t descend_g(const TN* __restrict__ nodes, const
     return c;
 }
 
+// LDS of a draw workgroup: the per-sample state (DrawShared) and the nodes of
+// the top levels of the current mode's eigen tree and row tree.  A
+// 1024-thread workgroup has the CU to itself (150 KB); a 256-thread one
+// shares it with two others (48 KB).
+constexpr size_t draw_lds(int NT) { return NT >= 1024 ? 150 * 1024 : 48 * 1024; }
+constexpr int cx_stride(int R, int bytes) {
+    return bytes == 8 ? ((R * (R + 1) / 2 + 15) & ~15) : ((R * (R + 1) / 2 + 31) & ~31);
+}
+// Full levels of each tree that fit half of `nodes` staged nodes.
+constexpr int stage_levels(int nodes) {
+    int sl = 0;
+    while (((2 << sl) - 1) <= nodes / 2) sl++;
+    return sl;
+}
+template <typename TN, int G, int NV, int NT>
+struct DrawCfg {
+    static constexpr int W = Vec<TN>::W;
+    static constexpr int SPW = 64 / G;
+    static constexpr int RMAX = draw_rmax(G, NV, W);
+    static constexpr int SLOTS = (NT / 64) * SPW;
+    static constexpr int STRIDE = cx_stride(RMAX, (int)sizeof(TN));
+    static constexpr size_t SHARED = sizeof(DrawShared<TN, SLOTS, RMAX>);
+    static constexpr int BUDGET =
+        SHARED < draw_lds(NT) ? (int)((draw_lds(NT) - SHARED) / (STRIDE * sizeof(TN))) : 0;
+    static constexpr int NSTAGE = (1 << stage_levels(BUDGET)) - 1;   // staged nodes per tree
+};
+
 // STS-CP KRPDrawSamples (sts_cp.hpp:228-286): one sample per group of G
-// lanes, 64 / G samples per wavefront, every non-target mode in turn.  TN is
-// the precision of the factors and of the row- and eigen-tree nodes (double
-// on the plain path, float on the mixed path); masses, cutoffs, the leaf's
+// lanes, 64 / G samples per wavefront, NT threads per workgroup, every
+// non-target mode in turn.  Before each mode the workgroup stages the nodes
+// of the top levels of that mode's eigen tree and row tree in LDS (the levels
+// every sample reads); deeper levels come from global memory.  TN is the
+// precision of the factors and of the row- and eigen-tree nodes (double on
+// the plain path, float on the mixed path); masses, cutoffs, the leaf's
 // cumulative sums and the weight are FP64.
-template <typename TN, int G, int NV>
-__global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
-    constexpr int W = Vec<TN>::W;
-    constexpr int SPW = 64 / G;
-    constexpr int RMAX = draw_rmax(G, NV, W);
-    __shared__ DrawShared<TN, (kDrawThreads / 64) * SPW, RMAX> sh;
+template <typename TN, int G, int NV, int NT>
+__global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
+    using C = DrawCfg<TN, G, NV, NT>;
+    constexpr int W = C::W;
+    constexpr int SPW = C::SPW;
+    constexpr int RMAX = C::RMAX;
+    constexpr int NST = C::NSTAGE;
+    __shared__ DrawShared<TN, C::SLOTS, RMAX> sh;
+    __shared__ __attribute__((aligned(16))) TN stage[2][NST > 0 ? NST * C::STRIDE : 1];
+    typedef typename Vec<TN>::type VT;
     const int R = a.R, N = a.N, j = a.j;
     const int P = tri_size(R);
     const int S = sizeof(TN) == 8 ? tri_stride(R) : tri_stride32(R);
@@ -398,8 +442,8 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int grp = lane / G, l = lane % G;
     const int slot = wave * SPW + grp;
-    const int64_t s = ((int64_t)blockIdx.x * (kDrawThreads / 64) + wave) * SPW + grp;
-    if (s >= a.J) return;   // whole groups leave together
+    const int64_t s = ((int64_t)blockIdx.x * (NT / 64) + wave) * SPW + grp;
+    const bool active = s < a.J;   // whole groups; every thread still joins the barriers
     const uint64_t sweep = a.sweep_dev ? *a.sweep_dev : a.sweep;
     TN* hs = sh.h[slot];
     TN* xs = sh.x[slot];
@@ -413,12 +457,27 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
         if (k == j) continue;
         const k::StsModeArgs& md = a.m[k];
         const TN* rows_k = static_cast<const TN*>(md.rows);
+        const TN* eig_k = static_cast<const TN*>(md.eig);
+        const TN* nodes_k = static_cast<const TN*>(md.nodes);
+        // stage the top levels of both trees (S is a multiple of W)
+        const int ne = (int)md.ge.node_count < NST ? (int)md.ge.node_count : NST;
+        const int nr = (int)md.g.node_count < NST ? (int)md.g.node_count : NST;
+        __syncthreads();
+        if (NST > 0) {
+            const int ve = ne * S / W, vr = nr * S / W;
+            for (int i = threadIdx.x; i < ve + vr; i += NT) {
+                if (i < ve) reinterpret_cast<VT*>(stage[0])[i] = reinterpret_cast<const VT*>(eig_k)[i];
+                else reinterpret_cast<VT*>(stage[1])[i - ve] = reinterpret_cast<const VT*>(nodes_k)[i - ve];
+            }
+        }
+        __syncthreads();
+        if (!active) continue;
         // eigen tree: query h, F = 1
         outer_g<TN, G, NV>(sh.pairs, hs, P, l, xx);
         const double ue = a.draws_eig ? a.draws_eig[(int64_t)k * a.J + s]
                                       : uniform01(a.seed, stream_id(sweep, j, k, 1), s);
         double low, high;
-        int64_t c = descend_g<TN, G, NV>(static_cast<const TN*>(md.eig), md.ge, S, P, xx, l, ue, low, high);
+        int64_t c = descend_g<TN, G, NV>(eig_k, stage[0], ne, md.ge, S, P, xx, l, ue, low, high);
         const int64_t v = md.ge.leaf_idx(c);
         eig_idx = (uint32_t)v;
         for (int b = l; b < R; b += G) xs[b] = (TN)((double)hs[b] * md.w[v * R + b]);
@@ -427,7 +486,7 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
         outer_g<TN, G, NV>(sh.pairs, xs, P, l, xx);
         const double ur = a.draws_row ? a.draws_row[(int64_t)k * a.J + s]
                                       : uniform01(a.seed, stream_id(sweep, j, k, 2), s);
-        c = descend_g<TN, G, NV>(static_cast<const TN*>(md.nodes), md.g, S, P, xx, l, ur, low, high);
+        c = descend_g<TN, G, NV>(nodes_k, stage[1], nr, md.g, S, P, xx, l, ur, low, high);
         const int64_t leaf = md.g.leaf_idx(c);
         const int F = (int)md.g.F;
         const int rows = (int)((md.g.n - leaf * F) < F ? (md.g.n - leaf * F) : F);
@@ -450,6 +509,7 @@ __global__ __launch_bounds__(kDrawThreads) void sts_draw_kernel(StsArgs a) {
         wave_lds_sync();
         if (l == 0) a.samples[s * N + k] = (uint32_t)row;
     }
+    if (!active) return;
     // weight = R / (J * h^T pinv h), FP64
     double q = 0.0;
     for (int e = l; e < P; e += G) {
@@ -650,20 +710,22 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
     a.vec_leaf = R % vw == 0;
     for (int k = 0; k < N; k++)
         if (k != j && ((uintptr_t)modes[k].rows % (size_t)(vw * node_bytes)) != 0) a.vec_leaf = 0;
-#define DRAW_LAUNCH(TN, G, NV)                                                                         \
-    hipLaunchKernelGGL((sts_draw_kernel<TN, G, NV>), dim3(grid_for(J, (kDrawThreads / 64) * (64 / G), 1LL << 30)), \
-                       dim3(kDrawThreads), 0, s, a)
-    // lanes per sample G and W-wide vectors per lane NV: G * NV * W >= R (R + 1) / 2
+#define DRAW_LAUNCH(TN, G, NV, NT)                                                                     \
+    hipLaunchKernelGGL((sts_draw_kernel<TN, G, NV, NT>), dim3(grid_for(J, (NT / 64) * (64 / G), 1LL << 30)), \
+                       dim3(NT), 0, s, a)
+    // lanes per sample G and W-wide vectors per lane NV: G * NV * W >= R (R + 1) / 2;
+    // NT threads per workgroup (1024 where the registers of four waves per
+    // SIMD suffice: more samples share each staged copy of the top levels)
     if (node_bytes == 8) {
-        if (R <= 16) DRAW_LAUNCH(double, 16, 5);
-        else if (R <= 32) DRAW_LAUNCH(double, 32, 9);
-        else if (R <= 45) DRAW_LAUNCH(double, 64, 9);
-        else DRAW_LAUNCH(double, 64, 17);
+        if (R <= 16) DRAW_LAUNCH(double, 16, 5, 1024);
+        else if (R <= 32) DRAW_LAUNCH(double, 32, 9, 1024);
+        else if (R <= 45) DRAW_LAUNCH(double, 64, 9, 1024);
+        else DRAW_LAUNCH(double, 64, 17, 256);
     } else {
-        if (R <= 16) DRAW_LAUNCH(float, 16, 3);
-        else if (R <= 32) DRAW_LAUNCH(float, 8, 17);
-        else if (R <= 45) DRAW_LAUNCH(float, 32, 9);
-        else DRAW_LAUNCH(float, 64, 9);
+        if (R <= 16) DRAW_LAUNCH(float, 16, 3, 1024);
+        else if (R <= 32) DRAW_LAUNCH(float, 8, 17, 1024);
+        else if (R <= 45) DRAW_LAUNCH(float, 32, 9, 1024);
+        else DRAW_LAUNCH(float, 64, 9, 1024);
     }
     EX_HIP(hipGetLastError());
 }

@@ -922,6 +922,312 @@ __global__ __launch_bounds__(kBitThreads) void hit_scan_reduce(const uint32_t* _
     add_counts<NT>(whits, wheads, counts);
 }
 
+// ---------------------------------------------------------------- bucketed hits
+// Sampled MTTKRP whose traffic follows the hits, not the nonzeros.  The hits
+// (every nonzero of every sampled fiber) are read in key order, where each
+// fiber's nonzeros are contiguous, and scattered into buckets of 2^bsh
+// consecutive row-order positions.  Within a bucket every hit has its own
+// row-order position, so placing each hit at its position offset in LDS
+// restores the row order whatever order the scatter wrote them in; the row
+// sums then run as in the row pass (GroupRows) over chunks of kBkFine
+// positions, with the same carry lists.  No atomics touch Y and the sums do
+// not depend on the scatter order.
+constexpr int kBkFine = 2048;                  // positions per wavefront chunk of the reduce
+constexpr int kBkWin = 4 * kBkFine;            // positions placed in LDS per reduce round (4 waves)
+constexpr int kBkThreads = 256;                // reduce workgroup
+constexpr int kBkMaxBuckets = 8192;            // bucket counters held in LDS by the tile passes
+constexpr int kBkTile = 16384;                 // hits per workgroup of the tile passes
+constexpr int kBkTileThreads = 512;
+constexpr int kBkWaveHits = kBkTile / (kBkTileThreads / 64);
+
+// A hit as the scatter writes it: design row, target row, value (its
+// row-order position goes to a separate array read by the placement).
+template <typename V>
+struct BkRec {
+    uint32_t c, row;
+    V x;
+};
+
+// Per deduplicated sample c < Jb: elen[c] = nonzeros of its fiber (0 without
+// a fiber, and past the device count J_u), efs[c] = the fiber's first
+// key-order position.  One wavefront per sample.
+__global__ __launch_bounds__(kThreads) void bk_fibers(const uint64_t* __restrict__ kd, int64_t Jb,
+                                                      const uint64_t* __restrict__ Ju_dev,
+                                                      const uint64_t* __restrict__ fkey, int64_t nfib,
+                                                      const uint32_t* __restrict__ fptr, uint32_t* __restrict__ elen,
+                                                      uint32_t* __restrict__ efs) {
+    const int64_t Ju = rows_of(Jb, Ju_dev);
+    const int lane = threadIdx.x % 64;
+    for (int64_t c = blockIdx.x * (int64_t)(kThreads / 64) + threadIdx.x / 64; c < Jb;
+         c += (int64_t)gridDim.x * (kThreads / 64)) {
+        int64_t f = -1;
+        if (c < Ju) f = wave_find_key(fkey, nfib, kd[c], lane);
+        if (lane == 0) {
+            elen[c] = f < 0 ? 0u : fptr[f + 1] - fptr[f];
+            efs[c] = f < 0 ? 0u : fptr[f];
+        }
+    }
+}
+
+// Wavefront-cooperative upper bound - 1 over the ascending hoff[0, n): the
+// last c with hoff[c] <= h (hoff[0] = 0 <= h).
+__device__ __forceinline__ int64_t wave_last_le(const uint32_t* __restrict__ hoff, int64_t n, uint32_t h, int lane) {
+    int64_t lo = 0, hi = n;   // hoff[lo] <= h
+    while (hi - lo > 64) {
+        const int64_t stride = (hi - lo + 63) / 64;
+        const int64_t at = lo + (int64_t)lane * stride;
+        const int cnt = __popcll(__ballot(at < hi && hoff[at] <= h));
+        lo = lo + (int64_t)(cnt - 1) * stride;
+        const int64_t h2 = lo + stride;
+        hi = h2 < hi ? h2 : hi;
+    }
+    const int64_t at = lo + lane;
+    const int cnt = __popcll(__ballot(at < hi && hoff[at] <= h));
+    return lo + cnt - 1;
+}
+
+// Walks a wavefront's hits [hb, he) in batches of 64 in key order: for the
+// lane's hit h the owning sample c (hoff[c] <= h < hoff[c] + elen[c]) and
+// the key-order position p.  win: 64 words of LDS of this wavefront.
+struct BkCursor {
+    const uint32_t* hoff;
+    const uint32_t* efs;
+    int64_t Jb;
+    int64_t c0;          // sample of the batch's first hit
+    uint32_t h0off;      // hoff[c0]
+    __device__ void start(const uint32_t* hoff_, const uint32_t* efs_, int64_t Jb_, uint32_t hb, int lane) {
+        hoff = hoff_;
+        efs = efs_;
+        Jb = Jb_;
+        c0 = wave_last_le(hoff, Jb, hb, lane);
+        h0off = hoff[c0];
+    }
+    // lane's hit hb + lane (valid when < he): sample c and position p; moves
+    // c0 to the sample of hit hb + 63 for the next batch
+    __device__ void batch(uint32_t hb, int lane, uint32_t* win, uint32_t& c, uint32_t& p) {
+        const uint32_t h = hb + (uint32_t)lane;
+        int64_t cnt = 0;   // samples after c0 whose first hit is <= h
+        int64_t cw = c0;
+        while (true) {
+            const int64_t at = cw + 1 + lane;
+            const uint32_t wv = at < Jb ? hoff[at] : 0xFFFFFFFFu;
+            win[lane] = wv;
+            wave_lds_sync();
+            // entries of this window <= h: binary search in the sorted window
+            int lo = 0, hi = 64;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (win[mid] <= h) lo = mid + 1; else hi = mid;
+            }
+            cnt += lo;
+            const uint32_t last = win[63];
+            wave_lds_sync();
+            if (last > hb + 63u) break;   // later windows start past the batch
+            cw += 64;
+        }
+        const int64_t cc = c0 + cnt;
+        c = (uint32_t)cc;
+        const uint32_t ho = cc == c0 ? h0off : hoff[cc];
+        p = efs[cc] + (h - ho);
+        const uint32_t cl = readlane_u32(c, 63);
+        if ((int64_t)cl != c0) {
+            c0 = cl;
+            h0off = hoff[cl];
+        }
+    }
+};
+
+// Per tile of kBkTile hits (one workgroup): the hit count of every bucket
+// (bucket = row-order position >> bsh) into hist[tile][NB].
+__global__ __launch_bounds__(kBkTileThreads) void bk_hist(const uint32_t* __restrict__ hoff,
+                                                          const uint32_t* __restrict__ elen,
+                                                          const uint32_t* __restrict__ efs, int64_t Jb,
+                                                          const uint32_t* __restrict__ kpos, int bsh, int NB,
+                                                          uint32_t* __restrict__ hist) {
+    __shared__ uint32_t lh[kBkMaxBuckets];
+    __shared__ uint32_t win[kBkTileThreads / 64][64];
+    const uint32_t H = hoff[Jb - 1] + elen[Jb - 1];
+    const int64_t t0 = (int64_t)blockIdx.x * kBkTile;
+    if (t0 >= (int64_t)H) return;
+    for (int b = threadIdx.x; b < NB; b += kBkTileThreads) lh[b] = 0u;
+    __syncthreads();
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    const int64_t hb0 = t0 + (int64_t)wv * kBkWaveHits;
+    const int64_t he = hb0 + kBkWaveHits < (int64_t)H ? hb0 + kBkWaveHits : (int64_t)H;
+    if (hb0 < he) {
+        BkCursor cur;
+        cur.start(hoff, efs, Jb, (uint32_t)hb0, lane);
+        for (int64_t hb = hb0; hb < he; hb += 64) {
+            uint32_t c, p;
+            cur.batch((uint32_t)hb, lane, win[wv], c, p);
+            if (hb + lane < he) atomicAdd(&lh[kpos[p] >> bsh], 1u);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < NB; b += kBkTileThreads) hist[(int64_t)blockIdx.x * NB + b] = lh[b];
+}
+
+// Per bucket b: hist[t][b] <- hits of b in tiles before t (tile order), and
+// btot[b] = hits of b.
+__global__ __launch_bounds__(kThreads) void bk_colscan(const uint32_t* __restrict__ hoff,
+                                                       const uint32_t* __restrict__ elen, int64_t Jb, int NB,
+                                                       uint32_t* __restrict__ hist, uint32_t* __restrict__ btot) {
+    const int b = blockIdx.x * kThreads + threadIdx.x;
+    if (b >= NB) return;
+    const uint32_t H = hoff[Jb - 1] + elen[Jb - 1];
+    const int64_t nt = ((int64_t)H + kBkTile - 1) / kBkTile;
+    uint32_t acc = 0;
+    int64_t t = 0;
+    for (; t + 8 <= nt; t += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = hist[(t + q) * NB + b];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            hist[(t + q) * NB + b] = acc;
+            acc += v[q];
+        }
+    }
+    for (; t < nt; t++) {
+        const uint32_t v = hist[t * NB + b];
+        hist[t * NB + b] = acc;
+        acc += v;
+    }
+    btot[b] = acc;
+}
+
+// The tile's hits again: each written to its bucket's range (base[b] +
+// the hits of b in earlier tiles + its slot within this tile, taken with an
+// LDS counter: the order inside a tile's share of a bucket is not fixed and
+// does not matter).  rk = row-order position, rec = (design row, target row,
+// value).
+template <typename V>
+__global__ __launch_bounds__(kBkTileThreads) void bk_scatter(const uint32_t* __restrict__ hoff,
+                                                             const uint32_t* __restrict__ elen,
+                                                             const uint32_t* __restrict__ efs, int64_t Jb,
+                                                             const uint32_t* __restrict__ kpos,
+                                                             const uint32_t* __restrict__ krow,
+                                                             const V* __restrict__ kval, int bsh, int NB,
+                                                             const uint32_t* __restrict__ hist,
+                                                             const uint32_t* __restrict__ base,
+                                                             uint32_t* __restrict__ rk, BkRec<V>* __restrict__ rec) {
+    __shared__ uint32_t lc[kBkMaxBuckets];
+    __shared__ uint32_t win[kBkTileThreads / 64][64];
+    const uint32_t H = hoff[Jb - 1] + elen[Jb - 1];
+    const int64_t t0 = (int64_t)blockIdx.x * kBkTile;
+    if (t0 >= (int64_t)H) return;
+    for (int b = threadIdx.x; b < NB; b += kBkTileThreads) lc[b] = base[b] + hist[(int64_t)blockIdx.x * NB + b];
+    __syncthreads();
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    const int64_t hb0 = t0 + (int64_t)wv * kBkWaveHits;
+    const int64_t he = hb0 + kBkWaveHits < (int64_t)H ? hb0 + kBkWaveHits : (int64_t)H;
+    if (hb0 >= he) return;
+    BkCursor cur;
+    cur.start(hoff, efs, Jb, (uint32_t)hb0, lane);
+    for (int64_t hb = hb0; hb < he; hb += 64) {
+        uint32_t c, p;
+        cur.batch((uint32_t)hb, lane, win[wv], c, p);
+        if (hb + lane < he) {
+            const uint32_t q = kpos[p];
+            const uint32_t r = krow[p];
+            const V x = kval[p];
+            const uint32_t at = atomicAdd(&lc[q >> bsh], 1u);
+            rk[at] = q;
+            rec[at] = BkRec<V>{c, r, x};
+        }
+    }
+}
+
+// One workgroup per bucket: every hit placed at its position offset in LDS
+// (a window of kBkWin positions at a time), then wavefront q sums the hits of
+// the window's q-th chunk of kBkFine positions in row order (GroupRows, as
+// the row pass sums a chunk), rows crossing a chunk boundary into the carry
+// lists (chunk w = position / kBkFine).
+template <typename V, int G>
+__global__ __launch_bounds__(kBkThreads) void bk_reduce(const uint32_t* __restrict__ base,
+                                                        const uint32_t* __restrict__ btot,
+                                                        const uint32_t* __restrict__ rk,
+                                                        const BkRec<V>* __restrict__ rec,
+                                                        const uint32_t* __restrict__ rrow, int64_t nnz, int bsh,
+                                                        const V* __restrict__ design, int R, V* __restrict__ Y,
+                                                        V* __restrict__ carry, uint32_t* __restrict__ cflag,
+                                                        uint32_t* __restrict__ ckey,
+                                                        unsigned long long* __restrict__ counts) {
+    constexpr int VW = group_vw<V>();
+    __shared__ uint32_t slot[kBkWin];
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    const int64_t pb = (int64_t)blockIdx.x << bsh;
+    const int64_t pe0 = pb + ((int64_t)1 << bsh);
+    const int64_t pe = pe0 < nnz ? pe0 : nnz;
+    const uint32_t r0 = base[blockIdx.x], nb = btot[blockIdx.x];
+    unsigned long long whits = 0, wheads = 0;
+    auto pre_of = [](uint32_t c) -> uint64_t { return (uint64_t)c; };
+    auto row_of = [design, R](uint64_t h, int col0, V (&d)[VW]) {
+        const V* u = design + (int64_t)h * R;
+#pragma unroll
+        for (int v = 0; v < VW; v++) d[v] = col0 + v < R ? u[col0 + v] : (V)0;
+    };
+    for (int64_t ws = pb; ws < pe; ws += kBkWin) {
+        for (int i = threadIdx.x; i < kBkWin; i += kBkThreads) slot[i] = ~0u;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < nb; i += kBkThreads) {
+            const int64_t off = (int64_t)rk[r0 + i] - ws;
+            if (off >= 0 && off < kBkWin) slot[off] = i;
+        }
+        __syncthreads();
+        const int64_t cb = ws + (int64_t)wv * kBkFine;
+        const int64_t ce0 = cb + kBkFine;
+        const int64_t ce = ce0 < pe ? ce0 : pe;
+        if (cb < ce) {
+            // the chunk's hits in position order, compacted in place
+            uint32_t* sl = slot + wv * kBkFine;
+            uint32_t nh = 0;
+            for (int m = 0; m < kBkFine / 64; m++) {
+                const uint32_t v = sl[64 * m + lane];
+                const bool hit = v != ~0u;
+                const uint64_t mask = __ballot(hit);
+                wave_lds_sync();
+                if (hit) sl[nh + lanes_below(mask)] = v;
+                nh += (uint32_t)__popcll(mask);
+            }
+            wave_lds_sync();
+            const int64_t w = cb / kBkFine;
+            GroupRows<V, G, VW> st;
+            st.init(lane, R, w, Y, carry);
+            st.first_row = rrow[cb];
+            st.last_row = rrow[ce - 1];
+            st.first_cont = cb > 0 && rrow[cb - 1] == st.first_row;
+            st.continues = ce < nnz && rrow[ce] == st.last_row;
+            auto meta = [&](int64_t k, uint32_t& r, uint32_t& c, V& x) {
+                const BkRec<V> h = rec[r0 + sl[k]];
+                r = h.row;
+                c = h.c;
+                x = h.x;
+            };
+            if (nh > 0) st.template run<step_batch<V, false>()>(0, nh, meta, pre_of, row_of);
+            st.finish(cflag, ckey);
+            whits += nh;
+            wheads += st.nheads;
+        }
+        __syncthreads();
+    }
+    add_counts<kBkThreads>(whits, wheads, counts);
+}
+
+// Key-order copies of each nonzero's target row and value (from the row
+// order, through kpos) for the bucketed path.
+template <typename V>
+__global__ __launch_bounds__(kThreads) void bk_keyorder(const uint32_t* __restrict__ kpos,
+                                                        const uint32_t* __restrict__ rrow, const V* __restrict__ rval,
+                                                        int64_t nnz, uint32_t* __restrict__ krow,
+                                                        V* __restrict__ kval) {
+    for (int64_t p = blockIdx.x * (int64_t)kThreads + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * kThreads) {
+        const uint32_t q = kpos[p];
+        if (krow) krow[p] = rrow[q];
+        kval[p] = rval[q];
+    }
+}
+
 // One level of the carry reduction: entries (key, R-vector, flag) sorted by
 // key, kScanChunk entries per wavefront (four per lane), flagged entries
 // summed in order per key.  A key whose flagged entries all lie in the
@@ -1416,6 +1722,78 @@ unsigned long long* count_parts(ModeLookup& L, const uint64_t* counts, unsigned 
     return L.cpart.as<unsigned long long>();
 }
 
+// The bucketed path (see bk_fibers .. bk_reduce): Jb bounds the sample count
+// (the device count J_u is read by the kernels).
+template <typename V>
+void sampled_mttkrp_buckets(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Jb,
+                            const V* design, int R, V* Y, DevBuf* tmp, uint64_t* counts, const uint64_t* Ju_dev) {
+    const int64_t nnz = L.nnz;
+    EX_CHECK(nnz < ((int64_t)1 << 31), 4, "the bucketed sampled MTTKRP needs fewer than 2^31 nonzeros");
+    DevBuf& kv = sizeof(V) == 4 ? L.kval32 : L.kval;
+    if (!kv.p) {
+        // key-order target rows and values, on first use
+        const bool rows = !L.krow.p;
+        if (rows) L.krow.ensure((size_t)nnz * 4);
+        kv.ensure((size_t)nnz * sizeof(V));
+        hipLaunchKernelGGL(bk_keyorder<V>, dim3(grid_for(nnz, kThreads, 16384)), dim3(kThreads), 0, s,
+                           L.kpos.as<uint32_t>(), L.rrow.as<uint32_t>(), vals, nnz,
+                           rows ? L.krow.as<uint32_t>() : nullptr, kv.as<V>());
+    }
+    int bsh = 13;
+    while (((nnz - 1) >> bsh) + 1 > kBkMaxBuckets) bsh++;
+    const int NB = (int)(((nnz - 1) >> bsh) + 1);
+    // hits <= every nonzero of Jb fibers of at most maxfib nonzeros
+    const double hb = (double)Jb * (double)L.maxfib;
+    const int64_t Hb = hb < (double)nnz ? (int64_t)hb : nnz;
+    const int64_t ntb = (Hb + kBkTile - 1) / kBkTile;
+    const int64_t nfine = (nnz + kBkFine - 1) / kBkFine;
+    const size_t sb1 = exclusive_scan_u32(s, nullptr, 0, nullptr, nullptr, Jb);
+    const size_t sb2 = exclusive_scan_u32(s, nullptr, 0, nullptr, nullptr, NB);
+    const size_t scan_b = sb1 > sb2 ? sb1 : sb2;
+    size_t o = 0;
+    auto take = [&](size_t bytes) { size_t r = o; o = (o + bytes + 255) & ~(size_t)255; return r; };
+    const size_t o_len = take((size_t)Jb * 4), o_efs = take((size_t)Jb * 4), o_hoff = take((size_t)Jb * 4),
+                 o_bt = take((size_t)NB * 4), o_base = take((size_t)NB * 4), o_t = take(scan_b),
+                 o_hist = take((size_t)(ntb > 0 ? ntb : 1) * NB * 4), o_rk = take((size_t)Hb * 4 + 4),
+                 o_rec = take((size_t)Hb * sizeof(BkRec<V>) + 16);
+    tmp[1].ensure(o);
+    char* tb = tmp[1].as<char>();
+    uint32_t* elen = (uint32_t*)(tb + o_len);
+    uint32_t* efs = (uint32_t*)(tb + o_efs);
+    uint32_t* hoff = (uint32_t*)(tb + o_hoff);
+    uint32_t* btot = (uint32_t*)(tb + o_bt);
+    uint32_t* base = (uint32_t*)(tb + o_base);
+    uint32_t* hist = (uint32_t*)(tb + o_hist);
+    uint32_t* rk = (uint32_t*)(tb + o_rk);
+    BkRec<V>* rec = (BkRec<V>*)(tb + o_rec);
+    carry_reserve<V>(tmp, nfine, R);
+    V* vec;
+    uint32_t *flag, *key;
+    carry_views(tmp[2], 2 * nfine, R, vec, flag, key);
+    hipLaunchKernelGGL(bk_fibers, dim3(grid_for(Jb, kThreads / 64, 16384)), dim3(kThreads), 0, s, kd, Jb, Ju_dev,
+                       L.fkey.as<uint64_t>(), L.nfib, L.fptr.as<uint32_t>(), elen, efs);
+    exclusive_scan_u32(s, tb + o_t, scan_b, elen, hoff, Jb);
+    if (ntb > 0) {
+        hipLaunchKernelGGL(bk_hist, dim3((unsigned)ntb), dim3(kBkTileThreads), 0, s, hoff, elen, efs, Jb,
+                           L.kpos.as<uint32_t>(), bsh, NB, hist);
+    }
+    hipLaunchKernelGGL(bk_colscan, dim3((NB + kThreads - 1) / kThreads), dim3(kThreads), 0, s, hoff, elen, Jb, NB,
+                       hist, btot);
+    exclusive_scan_u32(s, tb + o_t, scan_b, btot, base, NB);
+    if (ntb > 0) {
+        hipLaunchKernelGGL(bk_scatter<V>, dim3((unsigned)ntb), dim3(kBkTileThreads), 0, s, hoff, elen, efs, Jb,
+                           L.kpos.as<uint32_t>(), L.krow.as<uint32_t>(), kv.as<V>(), bsh, NB, hist, base, rk, rec);
+    }
+    unsigned long long* part = count_parts(L, counts, (unsigned)NB);
+    with_group(group_lanes<V>(R), [&](auto gc) {
+        hipLaunchKernelGGL((bk_reduce<V, decltype(gc)::value>), dim3((unsigned)NB), dim3(kBkThreads), 0, s, base,
+                           btot, rk, rec, L.rrow.as<uint32_t>(), nnz, bsh, design, R, Y, vec, flag, key, part);
+    });
+    if (counts) hipLaunchKernelGGL(sum_counts, dim3(1), dim3(kThreads), 0, s, part, NB, counts);
+    carry_reduce<V>(s, tmp, 2 * nfine, R, Y);
+    EX_HIP(hipGetLastError());
+}
+
 // Hit list or full row pass.  The hit list costs one mark byte per
 // nonzero plus a few scattered accesses per hit; the full pass reads every
 // nonzero's row and fiber number and handles the hits inside that pass.
@@ -1438,6 +1816,11 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
     if (scan_events) EX_HIP(hipEventRecord(scan_events[0], s));
     if (Ju > 0 && L.nfib > 0 && L.nnz > 0) {
         const int64_t nch = (L.nnz + kScanChunk - 1) / kScanChunk;
+        if (L.mttkrp_path == 3) {
+            sampled_mttkrp_buckets<V>(s, L, vals, kd, Ju, design, R, Y, tmp, counts, Ju_dev);
+            if (scan_events) EX_HIP(hipEventRecord(scan_events[1], s));
+            return;
+        }
         if (use_hit_list(L, Ju)) {
             const int64_t nchb = (L.nnz + kMarkChunk - 1) / kMarkChunk;
             if (!L.marks.p) {

@@ -23,13 +23,14 @@ def rel_err(a, b):
     return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
 
 
-def run_pair(ctx, orc, dims, R, nnz, J, sampler, seed, sweeps=2):
+def run_pair(ctx, orc, dims, R, nnz, J, sampler, seed, sweeps=2, path=0):
     from rdist_tensor_amd import abi
     rng = np.random.default_rng(seed)
     idx, vals = random_tensor(rng, dims, nnz)
     U = normalized_factors(rng, dims, R)
     kind = abi.EX_SAMPLER_STS_CP if sampler == "sts" else abi.EX_SAMPLER_CP_ARLS_LEV
     T = abi.Tensor(ctx, idx, vals, dims)
+    T.mttkrp_path(path)
     F = abi.Factors(ctx, dims, R)
     for k in range(len(dims)):
         F.set(k, U[k])
@@ -47,9 +48,11 @@ def run_pair(ctx, orc, dims, R, nnz, J, sampler, seed, sweeps=2):
     return worst, A.fit(), oa.fit()
 
 
+@pytest.mark.parametrize("path", [0, 3])
 @pytest.mark.parametrize("sampler", ["sts", "arls"])
 @pytest.mark.parametrize("case", ["rank1", "mode_of_size_1", "one_sample", "modes_below_R"])
-def test_edge_cases_match_oracle(ctx, orc, sampler, case):
+def test_edge_cases_match_oracle(ctx, orc, sampler, case, path):
+    """path 0: the automatic sampled-MTTKRP choice; 3: bucketed hits."""
     if case == "rank1":
         dims, R, nnz, J = [200, 150, 100], 1, 5000, 512
     elif case == "mode_of_size_1":
@@ -60,7 +63,7 @@ def test_edge_cases_match_oracle(ctx, orc, sampler, case):
         dims, R, nnz, J = [5, 7, 3], 8, 90, 256   # every tree one (partial) leaf
     print(f"{sampler}/{case}: dims {dims} R {R} J {J}", flush=True)
     seed = {"rank1": 141, "mode_of_size_1": 142, "one_sample": 143, "modes_below_R": 144}[case]
-    worst, fg, fo = run_pair(ctx, orc, dims, R, nnz, J, sampler, seed=seed)
+    worst, fg, fo = run_pair(ctx, orc, dims, R, nnz, J, sampler, seed=seed, path=path)
     print(f"{sampler}/{case}: factor rel err {worst:.2e}, fit {fg:.6f} vs {fo:.6f}")
     assert worst < 1e-9
     if np.isfinite(fo):
@@ -90,14 +93,17 @@ def test_empty_tensor(ctx):
     A.step(0, 256, 1)   # Y = 0: the factor's columns have zero norm (as in the reference)
 
 
-def test_samples_without_nonzeros(ctx, orc):
+@pytest.mark.parametrize("path", [0, 1, 2, 3])
+def test_samples_without_nonzeros(ctx, orc, path):
     """A very sparse tensor where most sampled fibers are empty: rows of Y
-    without hits stay zero, the rest match the oracle."""
+    without hits stay zero, the rest match the oracle (every sampled-MTTKRP
+    path)."""
     from rdist_tensor_amd import abi
     rng = np.random.default_rng(6)
     dims = [2000, 3000, 4000]
     idx, vals = random_tensor(rng, dims, 2000)
     T = abi.Tensor(ctx, idx, vals, dims)
+    T.mttkrp_path(path)
     T.build_lookups()
     for mode in range(3):
         lk = orc.Lookup(idx, vals, mode, dims)

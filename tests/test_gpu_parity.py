@@ -125,7 +125,7 @@ def test_dedup(ctx, orc):
         assert np.array_equal(wd_g, wd_o)
 
 
-PATHS = {"auto": 0, "row_pass": 1, "hit_list": 2}
+PATHS = {"auto": 0, "row_pass": 1, "hit_list": 2, "buckets": 3}
 
 
 @pytest.mark.parametrize("path", list(PATHS))
@@ -153,7 +153,7 @@ def test_sampled_mttkrp_fixed_samples(ctx, orc, R, path):
         assert rel_err(Yg, Yo) < REL
 
 
-@pytest.mark.parametrize("path", ["row_pass", "hit_list"])
+@pytest.mark.parametrize("path", ["row_pass", "hit_list", "buckets"])
 def test_sampled_mttkrp_long_rows(ctx, orc, path):
     """Rows far longer than one chunk of either pass (multi-level carries),
     chunks holding more hits than one staging round, empty rows."""
@@ -274,7 +274,7 @@ def test_sts_draw_bitexact(ctx, orc, R):
         assert rel_err(wg[ok], wo[ok]) < 1e-11
 
 
-@pytest.mark.parametrize("path", ["auto", "hit_list"])
+@pytest.mark.parametrize("path", ["auto", "hit_list", "buckets"])
 @pytest.mark.parametrize("sampler", ["arls", "sts"])
 def test_als_step_and_run_match_oracle(ctx, orc, sampler, path):
     from rdist_tensor_amd import abi
@@ -305,7 +305,7 @@ def test_als_step_and_run_match_oracle(ctx, orc, sampler, path):
     assert abs(A.fit() - oa.fit()) < 1e-9
 
 
-@pytest.mark.parametrize("path", ["row_pass", "hit_list"])
+@pytest.mark.parametrize("path", ["row_pass", "hit_list", "buckets"])
 @pytest.mark.parametrize("sampler", ["arls", "sts"])
 def test_als_run_statistics_match_oracle(ctx, orc, sampler, path):
     """execute_ALS_rounds: fits per epoch and the nonzeros iterated summed
