@@ -268,7 +268,10 @@ int ex_spd_pinv(ex_ctx* ctx, const double* M, int64_t R, double tol_rel, double*
  * `precision` (EX_MIXED: FP32 inputs and products, on MFMA for R <= 32). */
 int ex_dsymm(ex_ctx* ctx, int precision, const double* G, const double* Y, int64_t m, int64_t R, double* out);
 /* deduplicate_design_matrix (als_optimizer.hpp:130-206): returns *Ju unique
- * tuples (ignoring column `mode`) in ascending order with weight x count. */
+ * tuples (ignoring column `mode`) with weight x count, in the order of their
+ * first occurrence among the samples when the non-target coordinates pack
+ * into 63 bits (hashing, as the ALS steps deduplicate), else in ascending
+ * tuple order (the reference's sorted order). */
 int ex_dedup(ex_ctx* ctx, const uint32_t* samples, const double* weights, int64_t J, int N,
              int mode, const int64_t* dims, uint32_t* samples_dedup, double* weights_dedup,
              int64_t* Ju);

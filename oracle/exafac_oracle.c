@@ -420,55 +420,95 @@ static double quad_form(const double* G, const double* x, int64_t R) {
     return res;
 }
 
+/* |x|^T |G| |x| over G's upper triangle, mirrored: the scale of the rounding
+ * error of any evaluation of x^T G x as a sum of the products G_ab x_a x_b. */
+static double quad_abs(const double* G, const double* x, int64_t R) {
+    double res = 0.0;
+    for (int64_t a = 0; a < R; a++) {
+        double acc = 0.0;
+        for (int64_t b = 0; b < R; b++) acc += fabs(sym_upper(G, R, a, b)) * fabs(x[b]);
+        res += fabs(x[a]) * acc;
+    }
+    return res;
+}
+
+/* Unit roundoff of IEEE double. */
+#define ORC_U 0x1p-53
+
 /* PTSample (partition_tree.hpp:204-380).  Query vectors x (J x R); the chosen
  * row of U is multiplied into h; the row index goes to samples[i*width+col].
  * margin[i] is lowered to the smallest distance, in draw units, between the
- * draw and any bin boundary compared against on the way down. */
+ * draw and any bin boundary compared against on the way down.
+ *
+ * slack[i] (when given) is lowered to the smallest (distance - bound) over
+ * the same comparisons, where bound is a rounding-error bound of that
+ * boundary for any evaluation that takes the same tree nodes and factor rows
+ * and sums the same products in another order: a node's mass x^T G x, here
+ * t = G x then x . t (2R + 1 roundings deep), elsewhere summed in another
+ * order of at most 2R + 64 roundings, so the two differ by at most
+ * (4R + 65) u |x|^T |G| |x|; the cutoff low + mL / m then differs by at most
+ * b_low + (bL + (mL / m) bm) / m + 4u.  In a leaf the row products u_r . x
+ * (R-term sums, at most R + 8 deep elsewhere) differ by (2R + 8) u sum_b
+ * |u_rb x_b|, which bounds the cumulative fractions the draw is compared
+ * with.  A draw whose slack is <= 0 may land in either bin under another
+ * summation order; one with positive slack has a single correct bin. */
 void orc_ptree_sample(const orc_ptree* t, const double* U, int64_t J,
                       const double* x, double* h, const double* draws,
-                      uint32_t* samples, int64_t width, int64_t col, double* margin) {
+                      uint32_t* samples, int64_t width, int64_t col, double* margin, double* slack) {
     int64_t R = t->R, R2 = R * R, F = t->F;
+    const double Kq = (double)(4 * R + 65) * ORC_U;
+    const double Kl = (double)(2 * R + 8) * ORC_U;
     #pragma omp parallel
     {
         double* q = (double*)malloc(sizeof(double) * (F > 0 ? F : 1));
+        double* bq = (double*)malloc(sizeof(double) * (F > 0 ? F : 1));
         #pragma omp for schedule(static)
         for (int64_t i = 0; i < J; i++) {
             const double* xi = x + i * R;
             double u = draws[i];
             double mg = margin ? margin[i] : INFINITY;
+            double sl = slack ? slack[i] : INFINITY;
             double m = quad_form(t->G, xi, R);
+            const double bm = slack ? Kq * quad_abs(t->G, xi, R) : 0.0;
             int64_t c = 0;
-            double low = 0.0, high = 1.0;
-            for (uint32_t lvl = 0; lvl < t->lfill_level; lvl++) {
-                double mL = quad_form(t->G + (2 * c + 1) * R2, xi, R);
-                double cutoff = low + mL / m;
-                double d = fabs(u - cutoff);
-                if (d < mg) mg = d;
-                if (u <= cutoff) { c = 2 * c + 1; high = cutoff; }
-                else { c = 2 * c + 2; low = cutoff; }
+            double low = 0.0, high = 1.0, blo = 0.0, bhi = 0.0;
+            /* the comparison at one level (partition_tree.hpp:248-277) */
+            #define ORC_LEVEL()                                                                   \
+            {                                                                                     \
+                const double* GL = t->G + (2 * c + 1) * R2;                                       \
+                double mL = quad_form(GL, xi, R);                                                 \
+                double cutoff = low + mL / m;                                                     \
+                double d = fabs(u - cutoff);                                                      \
+                if (d < mg) mg = d;                                                               \
+                double bc = 0.0;                                                                  \
+                if (slack) {                                                                      \
+                    bc = blo + (Kq * quad_abs(GL, xi, R) + (mL / m) * bm) / m + 4.0 * ORC_U;      \
+                    if (d - bc < sl) sl = d - bc;                                                 \
+                }                                                                                 \
+                if (u <= cutoff) { c = 2 * c + 1; high = cutoff; bhi = bc; }                      \
+                else { c = 2 * c + 2; low = cutoff; blo = bc; }                                   \
             }
+            for (uint32_t lvl = 0; lvl < t->lfill_level; lvl++) ORC_LEVEL()
             /* tail level (partition_tree.hpp:279-307); the condition
              * node_count > nodes_before_lfill always holds */
-            if (t->node_count > t->nodes_before_lfill && !ptree_is_leaf(t, c)) {
-                double mL = quad_form(t->G + (2 * c + 1) * R2, xi, R);
-                double cutoff = low + mL / m;
-                double d = fabs(u - cutoff);
-                if (d < mg) mg = d;
-                if (u <= cutoff) { c = 2 * c + 1; high = cutoff; }
-                else { c = 2 * c + 2; low = cutoff; }
-            }
+            if (t->node_count > t->nodes_before_lfill && !ptree_is_leaf(t, c)) ORC_LEVEL()
+            #undef ORC_LEVEL
             int64_t leaf = orc_ptree_leaf_idx(t, c);
             int64_t res = 0;
             if (F > 1) {
                 double frac = (u - low) / (high - low);
                 int64_t rows = t->n - leaf * F < F ? t->n - leaf * F : F;
                 for (int64_t r = 0; r < F; r++) {
-                    double acc = 0.0;
+                    double acc = 0.0, aabs = 0.0;
                     if (r < rows) {
                         const double* ur = U + (leaf * F + r) * R;
                         for (int64_t b = 0; b < R; b++) acc += ur[b] * xi[b];
+                        if (slack)
+                            for (int64_t b = 0; b < R; b++) aabs += fabs(ur[b] * xi[b]);
                     }
                     q[r] = acc;
+                    /* bound of q_r^2 across evaluation orders */
+                    bq[r] = 2.0 * fabs(acc) * Kl * aabs + (Kl * aabs) * (Kl * aabs) + 2.0 * ORC_U * acc * acc;
                 }
                 double run = 0.0;
                 for (int64_t r = 0; r < F; r++) { double tmp = q[r] * q[r]; q[r] = run; run += tmp; }
@@ -477,9 +517,21 @@ void orc_ptree_sample(const orc_ptree* t, const double* U, int64_t J,
                 for (int64_t r = 0; r < F - 1; r++) {
                     if (frac < q[r + 1]) { res = r; break; }
                 }
+                double bP = 0.0, bT = 0.0;
+                if (slack) {
+                    for (int64_t r = 0; r < F; r++) bT += bq[r];
+                    bT += (double)(F + 2) * ORC_U * run;
+                }
                 for (int64_t r = 1; r < rows; r++) {
                     double d = fabs(frac - q[r]) * (high - low);
                     if (d < mg) mg = d;
+                    if (slack) {
+                        /* boundary r: cumulative fraction q[r] = P_r / T */
+                        bP += bq[r - 1];
+                        double bcut = (bP + (double)(r + 2) * ORC_U * run + q[r] * bT) / run + 2.0 * ORC_U;
+                        double bd = blo + frac * (blo + bhi) + bcut * (high - low) + 4.0 * ORC_U;
+                        if (d - bd < sl) sl = d - bd;
+                    }
                 }
                 /* A draw at the very top of the last, partial leaf would make
                  * the reference read past the factor (res = F-1 >= rows);
@@ -491,8 +543,10 @@ void orc_ptree_sample(const orc_ptree* t, const double* U, int64_t J,
             const double* ur = U + row * R;
             for (int64_t b = 0; b < R; b++) h[i * R + b] *= ur[b];
             if (margin) margin[i] = mg;
+            if (slack) slack[i] = sl;
         }
         free(q);
+        free(bq);
     }
 }
 
@@ -604,7 +658,13 @@ void orc_sts_free(orc_sts* s) {
 /* update_sampler (sts_cp.hpp:63-65) -> ExactLeverageTree::build_tree */
 void orc_sts_update(orc_sts* s, int j, const double* Uj) {
     orc_ptree_build(&s->row[j], Uj);
-    orc_gram(Uj, s->dims[j], s->R, s->grams + (int64_t)j * s->R2);
+    /* ancestor_grams[0] = compute_gram(U_j) (exact_leverage_tree.hpp:163-168):
+     * taken here as the tree's root, the same sum UjT Uj grouped by leaves
+     * then levels (the root's upper triangle, mirrored) */
+    const double* root = s->row[j].G;
+    double* g = s->grams + (int64_t)j * s->R2;
+    for (int64_t a = 0; a < s->R; a++)
+        for (int64_t b = 0; b < s->R; b++) g[a * s->R + b] = a <= b ? root[a * s->R + b] : root[b * s->R + a];
 }
 
 const double* orc_sts_slot(const orc_sts* s, int k) { return s->M + (int64_t)k * s->R2; }
@@ -685,7 +745,7 @@ void orc_sts_compute_m(orc_sts* s, int j) {
  * [k*J + i]. */
 void orc_sts_draw(orc_sts* s, const double* const* U, int j, int64_t J,
                   const double* draws_eig, const double* draws_row,
-                  uint32_t* samples, double* weights, double* margin) {
+                  uint32_t* samples, double* weights, double* margin, double* slack) {
     int N = s->N;
     int64_t R = s->R, R2 = s->R2;
     orc_sts_compute_m(s, j);
@@ -694,14 +754,15 @@ void orc_sts_draw(orc_sts* s, const double* const* U, int j, int64_t J,
     for (int64_t i = 0; i < J * R; i++) h[i] = 1.0;
     for (int64_t i = 0; i < J * N; i++) samples[i] = 0;
     if (margin) for (int64_t i = 0; i < J; i++) margin[i] = INFINITY;
+    if (slack) for (int64_t i = 0; i < J; i++) slack[i] = INFINITY;
     for (int k = 0; k < N; k++) {
         if (k == j) continue;
         memcpy(sh, h, sizeof(double) * J * R);
         int off = (k + 1 == j) ? k + 2 : k + 1;
         orc_ptree_sample(&s->eig[k], s->M + (int64_t)off * R2, J, h, sh,
-                         draws_eig + (int64_t)k * J, samples, N, j, margin);
+                         draws_eig + (int64_t)k * J, samples, N, j, margin, slack);
         orc_ptree_sample(&s->row[k], U[k], J, sh, h,
-                         draws_row + (int64_t)k * J, samples, N, k, margin);
+                         draws_row + (int64_t)k * J, samples, N, k, margin, slack);
     }
     orc_dagat(h, J, R, s->M, weights);
     for (int64_t i = 0; i < J; i++) weights[i] = (double)R / (weights[i] * (double)J);
@@ -1093,7 +1154,7 @@ void orc_als_step(orc_als* a, int j, int64_t J, uint64_t sweep) {
     if (a->sampler == ORC_ARLS)
         orc_arls_draw(N, a->dims, (const double* const*)a->scores, j, J, dr, samples, w, NULL);
     else
-        orc_sts_draw(a->sts, (const double* const*)a->U, j, J, de, dr, samples, w, NULL);
+        orc_sts_draw(a->sts, (const double* const*)a->U, j, J, de, dr, samples, w, NULL, NULL);
     double t1 = now_s();
     a->st.t_sampling += t1 - t0;
     uint32_t* sd = (uint32_t*)malloc(sizeof(uint32_t) * J * N);

@@ -48,7 +48,7 @@ void orc_ptree_multiply(orc_ptree* t, const double* mat);
 int64_t orc_ptree_leaf_idx(const orc_ptree* t, int64_t c);
 void orc_ptree_sample(const orc_ptree* t, const double* U, int64_t J,
                       const double* x, double* h, const double* draws,
-                      uint32_t* samples, int64_t width, int64_t col, double* margin);
+                      uint32_t* samples, int64_t width, int64_t col, double* margin, double* slack);
 
 /* ---- CP-ARLS-LEV (cp_arls_lev.hpp) ---- */
 void orc_arls_scores(const double* U, int64_t I, int64_t R, double* scores);
@@ -66,7 +66,7 @@ const double* orc_sts_gram(const orc_sts* s, int k);
 const orc_ptree* orc_sts_tree(const orc_sts* s, int k, int eigen);
 void orc_sts_draw(orc_sts* s, const double* const* U, int j, int64_t J,
                   const double* draws_eig, const double* draws_row,
-                  uint32_t* samples, double* weights, double* margin);
+                  uint32_t* samples, double* weights, double* margin, double* slack);
 
 /* ---- ALS step pieces (als_optimizer.hpp, accumulator_stationary.hpp) ---- */
 int64_t orc_dedup(const uint32_t* samples, const double* weights, int64_t J, int N, int j,

@@ -65,7 +65,7 @@ def lib():
             "orc_ptree_multiply": (None, [C.POINTER(PTree), _d]),
             "orc_ptree_leaf_idx": (C.c_int64, [C.POINTER(PTree), C.c_int64]),
             "orc_ptree_sample": (None, [C.POINTER(PTree), _d, C.c_int64, _d, _d, _d, _u32,
-                                        C.c_int64, C.c_int64, _d]),
+                                        C.c_int64, C.c_int64, _d, _d]),
             "orc_arls_scores": (None, [_d, C.c_int64, C.c_int64, _d]),
             "orc_arls_draw": (None, [C.c_int, _i64, C.POINTER(_d), C.c_int, C.c_int64, _d, _u32, _d, _d]),
             "orc_sts_create": (C.c_void_p, [C.c_int, _i64, C.c_int64, C.POINTER(_d)]),
@@ -75,7 +75,7 @@ def lib():
             "orc_sts_slot": (_d, [C.c_void_p, C.c_int]),
             "orc_sts_gram": (_d, [C.c_void_p, C.c_int]),
             "orc_sts_tree": (C.POINTER(PTree), [C.c_void_p, C.c_int, C.c_int]),
-            "orc_sts_draw": (None, [C.c_void_p, C.POINTER(_d), C.c_int, C.c_int64, _d, _d, _u32, _d, _d]),
+            "orc_sts_draw": (None, [C.c_void_p, C.POINTER(_d), C.c_int, C.c_int64, _d, _d, _u32, _d, _d, _d]),
             "orc_dedup": (C.c_int64, [_u32, _d, C.c_int64, C.c_int, C.c_int, _i64, _u32, _d]),
             "orc_design": (None, [_u32, _d, C.c_int64, C.c_int, C.c_int, C.POINTER(_d), C.c_int64, _d, _d]),
             "orc_lookup_create": (C.c_void_p, [C.c_int, C.c_int, _i64, C.c_int64, _u32, _d]),
@@ -225,8 +225,10 @@ class Tree:
         draws = f64(draws)
         samples = np.zeros((J, 1), np.uint32)
         margin = np.full(J, np.inf)
+        slack = np.full(J, np.inf)
         lib().orc_ptree_sample(C.byref(self.t), _p(U), J, _p(x), _p(h), _p(draws),
-                               _p(samples, _u32), 1, 0, _p(margin))
+                               _p(samples, _u32), 1, 0, _p(margin), _p(slack))
+        self.slack = slack
         return samples[:, 0].copy(), h, margin
 
 
@@ -296,8 +298,13 @@ class STS:
         samples = np.empty((J, N), np.uint32)
         w = np.empty(J)
         margin = np.empty(J)
+        slack = np.empty(J)
         lib().orc_sts_draw(self.h, _ptr_array(self.U), j, J, _p(de), _p(dr), _p(samples, _u32),
-                           _p(w), _p(margin))
+                           _p(w), _p(margin), _p(slack))
+        # per sample: the smallest distance to a compared boundary minus that
+        # boundary's rounding bound (orc_ptree_sample); <= 0: the bin depends
+        # on the summation order
+        self.slack = slack
         return samples, w, margin
 
 

@@ -275,6 +275,7 @@ struct ex_als {
     bool exact = false, grams_valid = false;
     DevBuf grams;
     DevBuf sd, wd, kd, dg, dm, G, Gi, pivflag, GY, Ms, Y, scal;
+    DevBuf dht, dhw;   // hashed deduplication: table (zero between steps), per-sample slots
     DevBuf tmp[4];
     DevBuf part;
     // u64[4]: hits (nonzeros iterated) and touched rows summed over steps, the
@@ -463,8 +464,12 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     // J_u stays on the device: every later kernel reads it; J bounds the sizes
     uint64_t* cnt = a->cnt.as<uint64_t>();
     const uint64_t* ju = cnt + 3;
-    k::dedup(st, s->samples.as<uint32_t>(), s->weights.as<double>(), J, N, kl, a->sd.as<uint32_t>(),
-             a->wd.as<double>(), a->kd.as<uint64_t>(), a->tmp, cnt + 3, cnt + 2);
+    if (kl.bits <= 63)
+        k::dedup_hash(st, s->samples.as<uint32_t>(), s->weights.as<double>(), J, N, kl, a->sd.as<uint32_t>(),
+                      a->wd.as<double>(), a->kd.as<uint64_t>(), a->dht, a->dhw, cnt + 3, cnt + 2);
+    else
+        k::dedup(st, s->samples.as<uint32_t>(), s->weights.as<double>(), J, N, kl, a->sd.as<uint32_t>(),
+                 a->wd.as<double>(), a->kd.as<uint64_t>(), a->tmp, cnt + 3, cnt + 2);
     const int64_t Ju = J;
     const bool mixed = a->precision == EX_MIXED;
     const size_t vb = mixed ? 4 : 8;   // bytes per design / MTTKRP value
@@ -580,7 +585,7 @@ std::vector<const void*> graph_fingerprint(const ex_als* a) {
     const ex_factors* f = a->f;
     const ex_tensor* t = a->t;
     for (const DevBuf* b : {&a->sd, &a->wd, &a->kd, &a->dg, &a->dm, &a->G, &a->Gi, &a->pivflag, &a->GY, &a->Ms, &a->Y,
-                            &a->scal, &a->part, &a->cnt, &a->grams, &a->sweep_ctr})
+                            &a->scal, &a->part, &a->cnt, &a->grams, &a->sweep_ctr, &a->dht, &a->dhw})
         add(*b);
     for (const DevBuf& b : a->tmp) add(b);
     for (const DevBuf* b : {&s->sums, &s->gtmp, &s->gpinv, &s->grams, &s->M, &s->lam, &s->eig, &s->eig32, &s->pinvp,
@@ -1677,8 +1682,12 @@ int ex_dedup(ex_ctx* ctx, const uint32_t* samples, const double* weights, int64_
         kd.ensure(JJ * 8);
         h2d(st, ds.p, samples, (size_t)J * N * 4);
         h2d(st, dw.p, weights, (size_t)J * 8);
-        int64_t nu = k::dedup(st, ds.as<uint32_t>(), dw.as<double>(), J, N, kl, sd.as<uint32_t>(), wd.as<double>(),
-                              kd.as<uint64_t>(), tmp);
+        DevBuf table, work;
+        const int64_t nu = kl.bits <= 63
+                               ? k::dedup_hash(st, ds.as<uint32_t>(), dw.as<double>(), J, N, kl, sd.as<uint32_t>(),
+                                               wd.as<double>(), kd.as<uint64_t>(), table, work)
+                               : k::dedup(st, ds.as<uint32_t>(), dw.as<double>(), J, N, kl, sd.as<uint32_t>(),
+                                          wd.as<double>(), kd.as<uint64_t>(), tmp);
         EX_HIP(hipMemcpyAsync(samples_dedup, sd.p, (size_t)nu * N * 4, hipMemcpyDeviceToHost, st));
         d2h(st, weights_dedup, wd.p, (size_t)nu * 8);
         *Ju = nu;

@@ -215,6 +215,12 @@ void build_lookup(hipStream_t s, const uint32_t* coords, const double* vals, int
 int64_t dedup(hipStream_t s, const uint32_t* samples, const double* weights, int64_t J, int N, const KeyLayout& kl,
               uint32_t* sd, double* wd, uint64_t* kd, DevBuf* tmp, uint64_t* Ju_dev = nullptr,
               uint64_t* Ju_total = nullptr);
+// The same unique tuples, keys and weight x multiplicity by hashing (keys of
+// at most 63 bits), in the order of first occurrence among the samples.
+// table: the hash table (kept zeroed between calls); work: per-sample slots.
+int64_t dedup_hash(hipStream_t s, const uint32_t* samples, const double* weights, int64_t J, int N,
+                   const KeyLayout& kl, uint32_t* sd, double* wd, uint64_t* kd, DevBuf& table, DevBuf& work,
+                   uint64_t* Ju_dev = nullptr, uint64_t* Ju_total = nullptr);
 void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
             const double* const* U, double* d_gram, double* d_mttkrp, const uint64_t* Ju_dev = nullptr);
 // Y (I_j x R, zero-filled here) = sampled MTTKRP of the Ju deduplicated

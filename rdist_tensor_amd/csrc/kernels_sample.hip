@@ -23,10 +23,14 @@ __device__ __forceinline__ void tri_ab(int R, int e, int& a, int& b) {
 
 // ---------------------------------------------------------------- row trees
 // Leaf Grams: one wavefront per leaf; the leaf's F = R rows are staged in LDS
-// and each lane produces its packed-upper elements.
+// and each lane produces its packed-upper elements, summing the rows' products
+// in row order without fused multiply-adds (the oracle's sums exactly, so
+// that every node of the tree, and the factor Gram at its root, is bitwise
+// the oracle's).
 template <int WPB>
 __global__ __launch_bounds__(64 * WPB) void leaf_gram_kernel(const double* __restrict__ U, int64_t n, int R,
                                                              TreeGeom g, double* __restrict__ nodes) {
+#pragma clang fp contract(off)
     __shared__ double blk[WPB][kMaxRank * kMaxRank / WPB];
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
     const int P = tri_size(R), S = tri_stride(R);

@@ -144,6 +144,120 @@ __global__ __launch_bounds__(kThreads) void dedup_finish(const uint32_t* __restr
     }
 }
 
+// Deduplication by hashing (als_optimizer.hpp:130-206 computes the same
+// unique tuples and weight x multiplicity by sorting).  The unique tuples
+// come out in the order of their first occurrence among the samples, which
+// depends only on the samples: no sort, three launches.  The table (hkey:
+// key + 1, 0 = empty; hfirst: ~(first sample index), 0 = none; hcount)
+// starts zeroed and is left zeroed by dh_scatter, which clears every slot it
+// used.
+constexpr int kDhTile = 1024;   // samples per workgroup of dh_count / dh_scatter
+
+__device__ __forceinline__ uint32_t dh_slot(uint64_t key, int tbits) {
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - tbits));
+}
+
+__global__ __launch_bounds__(kThreads) void dh_insert(const uint32_t* __restrict__ samples, int64_t J, KeyArgs ka,
+                                                      int tbits, unsigned long long* __restrict__ hkey,
+                                                      uint32_t* __restrict__ hfirst, uint32_t* __restrict__ hcount,
+                                                      uint32_t* __restrict__ slotof) {
+    const uint32_t mask = (1u << tbits) - 1u;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < J; i += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long key = pack(samples + i * ka.N, ka) + 1ull;
+        uint32_t h = dh_slot(key, tbits);
+        while (true) {
+            const unsigned long long prev = atomicCAS(hkey + h, 0ull, key);
+            if (prev == 0ull || prev == key) break;
+            h = (h + 1u) & mask;
+        }
+        atomicMax(hfirst + h, ~(uint32_t)i);
+        atomicAdd(hcount + h, 1u);
+        slotof[i] = h;
+    }
+}
+
+// tcount[t] = first occurrences among samples [t kDhTile, (t + 1) kDhTile).
+__global__ __launch_bounds__(kThreads) void dh_count(const uint32_t* __restrict__ slotof, int64_t J,
+                                                     const uint32_t* __restrict__ hfirst,
+                                                     uint32_t* __restrict__ tcount) {
+    __shared__ uint32_t wsum[kThreads / 64];
+    const int64_t b = (int64_t)blockIdx.x * kDhTile;
+    uint32_t n = 0;
+    for (int q = 0; q < kDhTile / kThreads; q++) {
+        const int64_t i = b + q * kThreads + threadIdx.x;
+        if (i < J) n += hfirst[slotof[i]] == ~(uint32_t)i ? 1u : 0u;
+    }
+    for (int off = 32; off >= 1; off >>= 1) n += __shfl_xor(n, off, 64);
+    if (threadIdx.x % 64 == 0) wsum[threadIdx.x / 64] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kThreads / 64; w++) t += wsum[w];
+        tcount[blockIdx.x] = t;
+    }
+}
+
+// Writes each first occurrence u (in sample order): sd[u] = its tuple, kd[u]
+// = its key, wd[u] = weight x multiplicity, and clears its table slot.  The
+// last workgroup writes J_u (and adds it to *Ju_total).
+__global__ __launch_bounds__(kThreads) void dh_scatter(const uint32_t* __restrict__ samples,
+                                                       const double* __restrict__ weights, int64_t J, KeyArgs ka,
+                                                       const uint32_t* __restrict__ slotof,
+                                                       const uint32_t* __restrict__ tcount,
+                                                       unsigned long long* __restrict__ hkey,
+                                                       uint32_t* __restrict__ hfirst, uint32_t* __restrict__ hcount,
+                                                       uint32_t* __restrict__ sd, double* __restrict__ wd,
+                                                       uint64_t* __restrict__ kd, uint64_t* __restrict__ Ju_dev,
+                                                       uint64_t* __restrict__ Ju_total) {
+    constexpr int PER = kDhTile / kThreads;
+    __shared__ uint32_t wsum[kThreads / 64];
+    __shared__ uint32_t base;
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    if (threadIdx.x == 0) {
+        uint32_t o = 0;
+        for (unsigned t = 0; t < blockIdx.x; t++) o += tcount[t];
+        base = o;
+    }
+    // thread t takes samples b + PER t .. b + PER t + PER - 1 (in order)
+    const int64_t i0 = (int64_t)blockIdx.x * kDhTile + (int64_t)threadIdx.x * PER;
+    bool first[PER];
+    uint32_t mine = 0;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int64_t i = i0 + q;
+        first[q] = i < J && hfirst[slotof[i]] == ~(uint32_t)i;
+        mine += first[q] ? 1u : 0u;
+    }
+    // exclusive scan of the per-thread counts over the workgroup
+    uint32_t x = mine;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) wsum[wv] = x;
+    __syncthreads();
+    uint32_t pre = base + x - mine;
+    for (int w = 0; w < wv; w++) pre += wsum[w];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        if (!first[q]) continue;
+        const int64_t i = i0 + q;
+        const uint32_t h = slotof[i];
+        const uint32_t u = pre++;
+        for (int k = 0; k < ka.N; k++) sd[(int64_t)u * ka.N + k] = samples[i * ka.N + k];
+        kd[u] = (uint64_t)(hkey[h] - 1ull);
+        wd[u] = weights[i] * (double)hcount[h];
+        hkey[h] = 0ull;
+        hfirst[h] = 0u;
+        hcount[h] = 0u;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kThreads - 1) {
+        // the last thread of the last tile: pre is now the total count
+        if (Ju_dev) *Ju_dev = pre;
+        if (Ju_total) *Ju_total += pre;
+    }
+}
+
 // ---------------------------------------------------------------- design
 struct DesignArgs {
     const double* U[kMaxModes];
@@ -1551,6 +1665,40 @@ int64_t dedup(hipStream_t s, const uint32_t* samples, const double* weights, int
                        hpos, rep, kd);
     hipLaunchKernelGGL(dedup_finish, dim3(grid_for(J, kThreads, 16384)), dim3(kThreads), 0, s, samples, weights, N,
                        hpos, rep, cnt, sd, wd);
+    EX_HIP(hipGetLastError());
+    return Ju_dev ? -1 : (int64_t)read_u64(s, cnt);
+}
+
+int64_t dedup_hash(hipStream_t s, const uint32_t* samples, const double* weights, int64_t J, int N,
+                   const KeyLayout& kl, uint32_t* sd, double* wd, uint64_t* kd, DevBuf& table, DevBuf& work,
+                   uint64_t* Ju_dev, uint64_t* Ju_total) {
+    EX_CHECK(kl.bits <= 63, 4, "hashed deduplication needs tuple keys of at most 63 bits");
+    EX_CHECK(J < ((int64_t)1 << 31), 1, "sample count J must be below 2^31");
+    if (J == 0) return 0;
+    int tbits = 10;
+    while (((int64_t)1 << tbits) < 2 * J) tbits++;
+    const int64_t T = (int64_t)1 << tbits;
+    const void* before = table.p;
+    table.ensure((size_t)T * 16);
+    // a new table starts empty (a fill kernel, not a memset: this runs beside
+    // graph captures); every call leaves the slots it used empty
+    if (table.p != before)
+        hipLaunchKernelGGL(zero_fill<uint32_t>, dim3(grid_for((int64_t)(table.bytes / 4), kThreads, 8192)),
+                           dim3(kThreads), 0, s, table.as<uint32_t>(), (int64_t)(table.bytes / 4));
+    unsigned long long* hkey = table.as<unsigned long long>();
+    uint32_t* hfirst = (uint32_t*)(hkey + T);
+    uint32_t* hcount = hfirst + T;
+    const int64_t ntiles = (J + kDhTile - 1) / kDhTile;
+    work.ensure((size_t)J * 4 + (size_t)ntiles * 4 + 256);
+    uint64_t* cnt = work.as<uint64_t>();
+    uint32_t* slotof = (uint32_t*)(cnt + 2);
+    uint32_t* tcount = slotof + J;
+    const KeyArgs ka = key_args(kl);
+    hipLaunchKernelGGL(dh_insert, dim3(grid_for(J, kThreads, 16384)), dim3(kThreads), 0, s, samples, J, ka, tbits,
+                       hkey, hfirst, hcount, slotof);
+    hipLaunchKernelGGL(dh_count, dim3((unsigned)ntiles), dim3(kThreads), 0, s, slotof, J, hfirst, tcount);
+    hipLaunchKernelGGL(dh_scatter, dim3((unsigned)ntiles), dim3(kThreads), 0, s, samples, weights, J, ka, slotof, tcount,
+                       hkey, hfirst, hcount, sd, wd, kd, Ju_dev ? Ju_dev : cnt, Ju_total);
     EX_HIP(hipGetLastError());
     return Ju_dev ? -1 : (int64_t)read_u64(s, cnt);
 }

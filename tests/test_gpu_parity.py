@@ -121,8 +121,39 @@ def test_dedup(ctx, orc):
         sd_g, wd_g = ctx.dedup(s, ww, mode, dims)
         sd_o, wd_o = orc.dedup(s, ww, mode, dims)
         cols = [k for k in range(4) if k != mode]
-        assert np.array_equal(sd_g[:, cols], sd_o[:, cols])
-        assert np.array_equal(wd_g, wd_o)
+        # the device lists the unique tuples in the order of their first
+        # occurrence (hashing); the oracle in ascending order (sorting)
+        _, first = np.unique(s[:, cols], axis=0, return_index=True)
+        assert np.array_equal(sd_g, s[np.sort(first)])
+        order = np.lexsort(sd_g[:, cols].T[::-1])
+        assert np.array_equal(sd_g[order][:, cols], sd_o[:, cols])
+        assert np.array_equal(wd_g[order], wd_o)
+
+
+def test_dedup_heavy_duplicates(ctx, orc):
+    """70,000 samples over 69 hash tiles, a third of them copies of five
+    tuples and the rest spread over a key space of 2^40: unique tuples,
+    first-occurrence order and weight x multiplicity exact."""
+    rng = np.random.default_rng(15)
+    dims = [1 << 20, 1 << 20, 5]
+    J = 70000
+    s = np.stack([rng.integers(0, d, J) for d in dims], 1).astype(np.uint32)
+    hot = s[:5].copy()
+    pick = rng.random(J) < 0.33
+    s[pick] = hot[rng.integers(0, 5, pick.sum())]
+    mode = 2
+    cols = [0, 1]
+    lin = s[:, 0].astype(np.int64) * dims[1] + s[:, 1]
+    w = (1.0 + (lin % 997)) / 997.0
+    sd_g, wd_g = ctx.dedup(s, w, mode, dims)
+    u, first, cnt = np.unique(s[:, cols], axis=0, return_index=True, return_counts=True)
+    order_first = np.sort(first)
+    assert np.array_equal(sd_g, s[order_first])
+    lin_u = s[order_first, 0].astype(np.int64) * dims[1] + s[order_first, 1]
+    cnt_by_lin = dict(zip(u[:, 0].astype(np.int64) * dims[1] + u[:, 1], cnt))
+    want = w[order_first] * np.array([cnt_by_lin[x] for x in lin_u], np.float64)
+    assert np.array_equal(wd_g, want)
+    print(f"J={J}: {len(sd_g)} unique tuples, largest multiplicity {cnt.max()}")
 
 
 PATHS = {"auto": 0, "row_pass": 1, "hit_list": 2, "buckets": 3}
@@ -265,10 +296,13 @@ def test_sts_draw_bitexact(ctx, orc, R):
         dr = rng.random((3, J))
         sg, wg = S.draw(J, j, draws_row=dr, draws_eig=de)
         s_o, wo, margin = so.draw(j, J, de, dr)
-        near = margin < 1e-12
+        # exempt: within 1e-12 of a boundary, or within the boundary's
+        # rounding bound (the oracle's slack <= 0)
+        near = (margin < 1e-12) | (so.slack <= 0)
         cols = [k for k in range(3) if k != j]
         mismatch = np.any(sg[:, cols] != s_o[:, cols], axis=1)
-        print(f"R={R} mode {j}: near-boundary {near.sum()}, mismatches {mismatch.sum()}")
+        print(f"R={R} mode {j}: near-boundary {near.sum()} (1e-12: {(margin < 1e-12).sum()}), "
+              f"mismatches {mismatch.sum()}")
         assert not np.any(mismatch & ~near)
         ok = ~mismatch
         assert rel_err(wg[ok], wo[ok]) < 1e-11

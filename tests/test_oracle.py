@@ -441,3 +441,85 @@ def test_exact_als_fit_is_monotone_and_recovers_planted(orc):
     a = orc.ALS(idx, vals, [30, 20, 25], normalized_factors(rng, [30, 20, 25], 4), "exact")
     fits, _ = a.run(6, 0, 1)
     assert np.all(np.diff(fits) > -1e-10), fits
+
+
+def _descend_fsum(G, n, F, lfill, node_count, nodes_upto, cl_off, U, x, u, shift=0):
+    """A second evaluation of the PartitionTree descent: every node mass as
+    the exactly rounded sum (math.fsum) of its products G_ab x_a x_b, every
+    leaf row product u_r . x likewise.  shift moves the chosen leaf row (a
+    planted mistake)."""
+    R = len(x)
+    outer = np.outer(x, x)
+
+    def mass(c):
+        Gc = np.triu(G[c]) + np.triu(G[c], 1).T
+        return math.fsum((Gc * outer).ravel())
+
+    m = mass(0)
+    c, low, high = 0, 0.0, 1.0
+    for _ in range(lfill):
+        cut = low + mass(2 * c + 1) / m
+        c, low, high = (2 * c + 1, low, cut) if u <= cut else (2 * c + 2, cut, high)
+    if 2 * c + 1 < node_count:
+        cut = low + mass(2 * c + 1) / m
+        c, low, high = (2 * c + 1, low, cut) if u <= cut else (2 * c + 2, cut, high)
+    leaf = c - nodes_upto if c >= nodes_upto else c - cl_off
+    rows = min(F, n - leaf * F)
+    q2 = [math.fsum(U[leaf * F + r] * x) ** 2 if r < rows else 0.0 for r in range(F)]
+    total = math.fsum(q2)
+    frac = (u - low) / (high - low)
+    res, run = F - 1, 0.0
+    for r in range(F - 1):
+        run += q2[r]
+        if frac < run / total:
+            res = r
+            break
+    res = min(res, rows - 1)
+    return leaf * F + min(res + shift, rows - 1)
+
+
+def _bound_case(orc, scale, noise, J=3000):
+    """Rows (scale x a common column) + noise, queries orthogonal to the
+    all-ones direction: u_r . x is the noise part while the Gram entries
+    carry the common one, so the node masses cancel by about (scale/noise)^2."""
+    rng = np.random.default_rng(44)
+    n, R = 3000, 6
+    U = scale * rng.standard_normal((n, 1)) + noise * rng.standard_normal((n, R))
+    t = orc.Tree(n, R, R)
+    t.build(U)
+    x = rng.standard_normal((J, R))
+    x -= x.mean(axis=1, keepdims=True)
+    draws = rng.random(J)
+    samples, _, margin = t.sample(U, x, draws)
+    tt = t.t
+    geo = (n, R, tt.lfill_level, tt.node_count, tt.nodes_upto_lfill, tt.complete_level_offset)
+    return t.G, geo, U, x, draws, samples, margin, t.slack
+
+
+@pytest.mark.parametrize("scale, noise", [(100.0, 1e-3), (10.0, 1e-2)])
+def test_boundary_rounding_bound_covers_reordered_sums(orc, scale, noise):
+    """The oracle's slack (distance to each compared boundary minus that
+    boundary's rounding bound) against a second evaluation of the same
+    descent that sums every mass and row product exactly rounded (fsum).
+    With masses cancelling by about 1e10 the two evaluations put some draws
+    in different bins, and every such draw has slack <= 0 (the bound is
+    sound).  With cancellation of about 1e6 few draws are ambiguous, and a
+    planted mistake (the chosen row moved by one) disagrees on draws whose
+    slack is positive (the bound leaves the check its power)."""
+    G, geo, U, x, draws, samples, margin, slack = _bound_case(orc, scale, noise)
+    n, F, lfill, nc, nu, cl = geo
+    J = len(draws)
+    other = np.array([_descend_fsum(G, n, F, lfill, nc, nu, cl, U, x[i], draws[i]) for i in range(J)])
+    diff = other != samples
+    ambiguous = slack <= 0
+    print(f"scale {scale}, noise {noise}: draws in another bin under fsum {diff.sum()}, ambiguous by the "
+          f"bound {ambiguous.sum()}, below the 1e-12 margin {(margin < 1e-12).sum()}")
+    assert np.all(ambiguous[diff])
+    if scale == 100.0:
+        assert diff.sum() > 0
+    else:
+        assert ambiguous.mean() < 0.01
+        planted = np.array([_descend_fsum(G, n, F, lfill, nc, nu, cl, U, x[i], draws[i], shift=1)
+                            for i in range(300)])
+        wrong = (planted != samples[:300]) & ~ambiguous[:300]
+        assert wrong.sum() > 150
