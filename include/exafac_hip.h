@@ -251,6 +251,17 @@ int ex_tns_read(const char* path, int64_t nnz, int nmodes, uint64_t* coords, dou
 /* ---- dense R x R kernels (parity hooks; host in/out) ----------------------- */
 /* compute_gram (common.h:363-403): G = A^T A, A is m x R. */
 int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G);
+/* (no counterpart in the reference) compute_gram (common.h:363-403) that
+ * allocates nothing, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops): A (m x R) and G (R x R) are
+ * DEVICE pointers on the context's device; the work is enqueued on the
+ * context stream and the call returns without waiting (G is ready once that
+ * stream reaches it, e.g. after ex_ctx_sync).  The per-block partial sums use
+ * a static 8 MiB device array, one per GPU, shared by every call on that GPU:
+ * calls that overlap on one GPU (several contexts or host threads) race.  No
+ * size is rejected for R <= 64: the block count is capped so that the
+ * partials fit.  Same kernels as ex_gram; the sum agrees to rounding. */
+int ex_gram_dev_static(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G);
 /* LAPACKE_dsyev('V','U') replacement: eigenvalues ascending, eigenvectors in
  * the columns of V (row-major V[u*R+v]); reads the upper triangle of A. */
 int ex_eigh(ex_ctx* ctx, const double* A, int64_t R, double* lam, double* V);

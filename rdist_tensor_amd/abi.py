@@ -70,6 +70,7 @@ SIGNATURES = {
     "ex_als_fit": [_vp, _d],
     "ex_als_stats_json": [_vp, C.c_char_p, C.POINTER(C.c_size_t)],
     "ex_gram": [_vp, _d, C.c_int64, C.c_int64, _d],
+    "ex_gram_dev_static": [_vp, _vp, C.c_int64, C.c_int64, _vp],
     "ex_eigh": [_vp, _d, C.c_int64, _d, _d],
     "ex_pinv": [_vp, _d, C.c_int64, C.c_double, _d],
     "ex_spd_pinv": [_vp, _d, C.c_int64, C.c_double, _d],

@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -19,7 +20,30 @@
 
 using namespace exafac;
 
+// Scratch of ex_gram_dev_static, the entry point that allocates nothing: the
+// per-block Gram partials live in this file-scope device array (one copy per
+// GPU, held until the process ends).  Host code never uses the array's name
+// as an address: ex_ctx_create looks up each device's copy once
+// (hipGetSymbolAddress, with that device current) and the entry point passes
+// that address to the kernels.
+constexpr size_t kStaticScratchBytes = (size_t)8 << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of `device`'s copy of the scratch array (the device is current).
+void lookup_static_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* p = nullptr;
+    EX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    EX_CHECK(p != nullptr, EX_ERR_HIP, "no device address for the static scratch array");
+    g_scratch_ptr[device] = p;
+}
+
 thread_local std::string g_last_error;
 
 template <class F>
@@ -847,9 +871,11 @@ int ex_ctx_create(int device, ex_ctx** out) {
         int n = 0;
         EX_HIP(hipGetDeviceCount(&n));
         EX_CHECK(device >= 0 && device < n, EX_ERR_ARG, "device index out of range");
+        EX_CHECK(device < kMaxDevices, EX_ERR_UNSUPPORTED, "device index above 63");
         auto c = std::make_unique<ex_ctx>();
         c->c.device = device;
         EX_HIP(hipSetDevice(device));
+        lookup_static_scratch(device);
         EX_HIP(hipGetDeviceProperties(&c->c.prop, device));
         EX_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
         EX_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
@@ -1561,6 +1587,18 @@ int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G) {
         h2d(st, dA.p, A, (size_t)m * R * 8);
         k::gram(st, dA.as<double>(), m, (int)R, dG.as<double>(), part);
         d2h(st, G, dG.p, (size_t)R * R * 8);
+    });
+}
+
+int ex_gram_dev_static(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G) {
+    return guarded([&] {
+        EX_CHECK(ctx && G && (m == 0 || A), EX_ERR_ARG, "null argument");
+        EX_CHECK(R >= 1 && R <= kMaxRank, EX_ERR_UNSUPPORTED, "rank must be in [1, 64]");
+        EX_CHECK(m >= 0, EX_ERR_ARG, "negative row count");
+        void* part = g_scratch_ptr[ctx->c.device];
+        EX_CHECK(part != nullptr, EX_ERR_STATE, "static scratch not set up for this device");
+        EX_HIP(hipSetDevice(ctx->c.device));
+        k::gram_into(ctx->c.stream, A, m, (int)R, G, static_cast<double*>(part), kStaticScratchBytes);
     });
 }
 

@@ -151,6 +151,8 @@ namespace k {
 // dense
 void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part,
           const uint64_t* m_dev = nullptr);
+// G = A^T A with the block partials in `part` (part_bytes; no allocation)
+void gram_into(hipStream_t s, const double* A, int64_t m, int R, double* G, double* part, size_t part_bytes);
 // G = A^T A for FP32 A with FP64 products and sums
 void gram_f32in(hipStream_t s, const float* A, int64_t m, int R, double* G, DevBuf& part,
                 const uint64_t* m_dev = nullptr);

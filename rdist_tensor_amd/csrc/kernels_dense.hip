@@ -685,6 +685,25 @@ static void gram_mfma64(hipStream_t s, const TA* A, int64_t m, int R, double* G,
     gram_final_launch(s, part.as<double>(), nb, R, G);
 }
 
+// G = A^T A with the block partials in a caller-owned buffer of part_bytes:
+// the block count is capped so that the partials fit (fewer, longer blocks;
+// the sum is the same up to its grouping).  Allocates nothing.
+void gram_into(hipStream_t s, const double* A, int64_t m, int R, double* G, double* part, size_t part_bytes) {
+    const int64_t fit = (int64_t)(part_bytes / ((size_t)R * R * sizeof(double)));
+    EX_CHECK(fit >= 1, 4, "Gram partials do not fit the scratch buffer");
+    int nb = (int)grid_for((m + 15) / 16, 4, 1024);
+    if (nb > fit) nb = (int)fit;
+    const dim3 grid(nb), block(kThreads);
+    switch ((R + 15) / 16) {
+        case 1: hipLaunchKernelGGL((gram_mfma64_partial<double, 1>), grid, block, 0, s, A, m, nullptr, R, part); break;
+        case 2: hipLaunchKernelGGL((gram_mfma64_partial<double, 2>), grid, block, 0, s, A, m, nullptr, R, part); break;
+        case 3: hipLaunchKernelGGL((gram_mfma64_partial<double, 3>), grid, block, 0, s, A, m, nullptr, R, part); break;
+        default: hipLaunchKernelGGL((gram_mfma64_partial<double, 4>), grid, block, 0, s, A, m, nullptr, R, part); break;
+    }
+    EX_HIP(hipGetLastError());
+    gram_final_launch(s, part, nb, R, G);
+}
+
 void gram(hipStream_t s, const double* A, int64_t m, int R, double* G, DevBuf& part, const uint64_t* m_dev) {
     gram_mfma64<double>(s, A, m, R, G, part, m_dev);
 }
