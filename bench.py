@@ -447,7 +447,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both")
-    ap.add_argument("--mttkrp-path", choices=["auto", "row_pass", "hit_list"], default="auto",
+    ap.add_argument("--mttkrp-path", choices=["auto", "row_pass", "hit_list", "buckets"], default="auto",
                     help="how the sampled MTTKRP finds the sampled fibers' nonzeros")
     ap.add_argument("--exact-steps", type=int, default=3,
                     help="ExactALS sweeps per precision reported beside the randomized paths (0: skip)")
@@ -478,7 +478,7 @@ def main():
     t0 = time.time()
     T = abi.Tensor(ctx, idx, vals, cfg["dims"])
     T.mttkrp_path({"auto": abi.EX_MTTKRP_AUTO, "row_pass": abi.EX_MTTKRP_ROW_PASS,
-                   "hit_list": abi.EX_MTTKRP_HIT_LIST}[args.mttkrp_path])
+                   "hit_list": abi.EX_MTTKRP_HIT_LIST, "buckets": abi.EX_MTTKRP_BUCKETS}[args.mttkrp_path])
     T.build_lookups()
     ctx.sync()
     log(f"[bench] device {dev}; tensor upload + lookups {time.time() - t0:.1f} s")

@@ -290,7 +290,7 @@ def test_sts_draw_bitexact(ctx, orc, R):
         F.set(k, U[k])
     S = abi.Sampler(ctx, F, abi.EX_SAMPLER_STS_CP)
     so = orc.STS(U)
-    J = 8192
+    J = 20000
     for j in range(3):
         de = rng.random((3, J))
         dr = rng.random((3, J))
