@@ -266,10 +266,13 @@ def run_exact(args, cfg, ctx, T, U0, precision, nnz):
     avg = st["ms"] / st["count"] * 1e-3
     per = float(np.mean([exact_mttkrp_bytes(dims, j, nnz, R, vb) for j in range(N)]))
     achieved = per / avg / 1e9
+    traffic = load_traffic(os.path.join(ROOT, "profiles", "hbm_traffic.md"))
     return dict(value=K / elapsed, ms_per_step=elapsed / K * 1e3,
                 roofline={"kernel": "exact_mttkrp", "bound": "hbm", "achieved": round(achieved, 2),
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                          "traffic": None, "avg_launch_ms": round(avg * 1e3, 4),
+                          "traffic": traffic.get(f"exact_mttkrp/{precision}/{args.config}") if args.scale == 1.0
+                          else None,
+                          "avg_launch_ms": round(avg * 1e3, 4),
                           "algorithmic_bytes_per_launch": int(per)},
                 spans_ms_per_sweep={k: round(v["ms"] / K, 4) for k, v in timing["spans"].items()},
                 fit={"initial": fit0, "final": fit1})
