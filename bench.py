@@ -369,50 +369,119 @@ def run_cpu_baseline(cfg, idx, vals, U0, budget_s=30.0):
                        f"{setup:.1f} s untimed")
 
 
+def batch_sweep_bytes(batch, J, vb, counters, sweeps):
+    """Algorithmic bytes of the batched sweeps, from the batch's counters:
+    per mode step, the sampler state read by the draws (row, score, CDF entry
+    per sample and drawn mode), the samples and weights written, the fiber
+    key of each unique sample, the factor rows gathered into the design, each
+    hit nonzero (row-order position, row, value), the factor rows written and
+    the previous ones cleared, their scores and CDF entries, and one pass over
+    the row stamps of the target mode.  Design / MTTKRP scratch round trips
+    and binary-search paths are implementation traffic, not counted."""
+    nz, ju, rows = counters
+    total = 0.0
+    for q, (dims, idx, vals, R, st) in enumerate(batch):
+        N = len(dims)
+        steps = sweeps * N
+        total += steps * J * ((N - 1) * 20 + N * 4 + 8)
+        total += ju[q] * (8 + (N - 1) * R * vb)
+        total += nz[q] * (8 + vb)
+        total += 2 * rows[q] * R * vb + rows[q] * 20
+        total += sweeps * sum(dims) * 4
+    return total
+
+
 def run_batched(args):
-    """Batched CP-ARLS-LEV (mixed precision): independent tensors, one
-    optimizer and context each, all advanced together by ex_als_run_batch
-    (their streams side by side); compared with the same optimizers run one
-    after another.  A step is one sweep of every tensor."""
+    """Batched CP-ARLS-LEV: independent tensors (mode sizes U[500, 50000],
+    nnz = 20 x the largest, R in {8, 16, 32}, J = 8192) advanced together
+    by ONE kernel launch per sweep (ex_batch_*: one 1024-thread workgroup per
+    tensor, every mode step inside), the launch replayed from a captured HIP
+    graph.  Mixed precision is the headline; the FP64 batch and the earlier
+    multi-stream runner (ex_als_run_batch, one context and stream per
+    optimizer) are reported beside it; the oracle on a sample of the tensors
+    is the CPU baseline.  A step is one sweep of every tensor."""
     from rdist_tensor_amd import abi, synthetic
     t0 = time.time()
     batch = synthetic.make_batch(count=args.batch_count, seed=4, scale=args.scale)
-    log(f"[bench] generated {len(batch)} tensors, {sum(len(b[1]) for b in batch)} nonzeros in "
-        f"{time.time() - t0:.1f} s")
+    nnz_total = sum(len(b[1]) for b in batch)
+    log(f"[bench] generated {len(batch)} tensors, {nnz_total} nonzeros in {time.time() - t0:.1f} s")
     J = 8192
-    objs = []
-    t0 = time.time()
-    for dims, idx, vals, R, st in batch:
-        c = abi.Context(0)
-        T = abi.Tensor(c, idx, vals, dims)
-        F = abi.Factors(c, dims, R)
-        for k, u in enumerate(synthetic.factors(dims, R, st)):
-            F.set(k, u)
-        S = abi.Sampler(c, F, abi.EX_SAMPLER_CP_ARLS_LEV, precision=abi.EX_MIXED, seed=st)
-        A = abi.ALS(c, T, F, S, precision=abi.EX_MIXED)
-        A.init()
-        objs.append((c, T, F, S, A))
-    log(f"[bench] {len(objs)} optimizers ready in {time.time() - t0:.1f} s")
-    opts = [o[4] for o in objs]
-    abi.ALS.run_batch(opts, args.warmup, J)
-    batch_s = abi.ALS.run_batch(opts, args.steps, J)
-    log(f"[bench] batched: {args.steps} sweeps of {len(opts)} tensors in {batch_s:.3f} s")
-    seq_sweeps = max(1, args.steps // 5)
-    t1 = time.perf_counter()
-    for o in opts:
-        o.run(seq_sweeps, J, 10 ** 9)
-    seq_s = time.perf_counter() - t1
-    log(f"[bench] one after another: {seq_sweeps} sweeps of {len(opts)} tensors in {seq_s:.3f} s")
-    eager_rate = args.steps / batch_s
-    rate, mode = eager_rate, "eager"
-    if args.batch_graphs:
-        # every optimizer's sweep captured once (first warm-up round), then replayed
-        for o in opts:
-            o.use_graph(True)
-        abi.ALS.run_batch(opts, max(1, args.warmup), J)
-        graph_s = abi.ALS.run_batch(opts, args.steps, J)
-        log(f"[bench] batched, graph replay: {args.steps} sweeps of {len(opts)} tensors in {graph_s:.3f} s")
-        rate, mode = args.steps / graph_s, "graph replay"
+    ctx = abi.Context(0)
+    results = {}
+    for prec in ("mixed", "fp64"):
+        pc = abi.EX_MIXED if prec == "mixed" else abi.EX_FP64
+        t0 = time.time()
+        objs = []
+        for dims, idx, vals, R, st in batch:
+            T = abi.Tensor(ctx, idx, vals, dims)
+            F = abi.Factors(ctx, dims, R)
+            for k, u in enumerate(synthetic.factors(dims, R, st)):
+                F.set(k, u)
+            S = abi.Sampler(ctx, F, abi.EX_SAMPLER_CP_ARLS_LEV, precision=pc, seed=st)
+            A = abi.ALS(ctx, T, F, S, precision=pc)
+            A.init()
+            objs.append((T, F, S, A))
+        B = abi.Batch([o[3] for o in objs], J)
+        B.use_graph(bool(args.batch_graphs))
+        log(f"[bench] {prec}: {len(objs)} optimizers and the batch ready in {time.time() - t0:.1f} s")
+        B.run(args.warmup)
+        c0 = B.counters()
+        sec = B.run(args.steps)
+        c1 = B.counters()
+        d = tuple(b - a for a, b in zip(c0, c1))
+        fits = np.array([o[3].fit() for o in objs])
+        bad = sum(int(not np.all(np.isfinite(o[1].get(k)))) for o in objs for k in range(3))
+        log(f"[bench] batched {prec}: {args.steps} sweeps of {len(objs)} tensors in {sec:.4f} s; "
+            f"mean fit {np.nanmean(fits):.6f}; factors with non-finite entries {bad}")
+        results[prec] = dict(sec=sec, counters=d, fits=fits, bad=bad)
+        B.close()
+        for o in objs:
+            for x in reversed(o):
+                x.close()
+    # the earlier runner: every optimizer on its own context and stream
+    streams_rate = None
+    if args.batch_compare > 0:
+        objs = []
+        for dims, idx, vals, R, st in batch:
+            c = abi.Context(0)
+            T = abi.Tensor(c, idx, vals, dims)
+            F = abi.Factors(c, dims, R)
+            for k, u in enumerate(synthetic.factors(dims, R, st)):
+                F.set(k, u)
+            S = abi.Sampler(c, F, abi.EX_SAMPLER_CP_ARLS_LEV, precision=abi.EX_MIXED, seed=st)
+            A = abi.ALS(c, T, F, S, precision=abi.EX_MIXED)
+            A.init()
+            objs.append((c, T, F, S, A))
+        abi.ALS.run_batch([o[4] for o in objs], 1, J)
+        s2 = abi.ALS.run_batch([o[4] for o in objs], args.batch_compare, J)
+        streams_rate = args.batch_compare / s2
+        log(f"[bench] multi-stream runner (mixed): {args.batch_compare} sweeps in {s2:.3f} s")
+        for c, *_ in objs:
+            c.close()
+    # CPU baseline: the oracle's ALS on every 32nd tensor, one sweep each, scaled to the batch
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle
+        oracle.build()
+        sample = batch[::32]
+        spent = 0.0
+        for dims, idx, vals, R, st in sample:
+            a = oracle.ALS(idx, vals, dims, synthetic.factors(dims, R, st), "arls", seed=st)
+            t1 = time.perf_counter()
+            a.run(1, J, 10 ** 9)
+            spent += time.perf_counter() - t1
+            log(f"[bench] cpu baseline: tensor dims {dims} R={R} done ({spent:.1f} s)")
+        est = spent * len(batch) / len(sample)
+        cpu = {"value": round(1.0 / est, 6), "unit": "sweeps/s", "cores": int(oracle.lib().orc_num_threads()),
+               "kind": "port",
+               "sample": f"oracle ALS (FP64), one sweep of {len(sample)} of the {len(batch)} tensors (every 32nd), "
+                         f"{spent:.1f} s, scaled by {len(batch)}/{len(sample)}; lookups built untimed"}
+    h = results["mixed"]
+    rate = args.steps / h["sec"]
+    per = batch_sweep_bytes(batch, J, 4, h["counters"], args.steps) / args.steps
+    avg = h["sec"] / args.steps
+    achieved = per / avg / 1e9
+    nz_rate = float(np.sum(h["counters"][0])) / h["sec"]
     out = {
         "metric": "batched CP-ARLS-LEV sweeps/s (one sweep = every tensor of the batch)",
         "value": round(rate, 4),
@@ -420,24 +489,31 @@ def run_batched(args):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 / rate, 3),
+        "ms_per_step": round(1e3 / rate, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+bf16(mfma)/f64 solves",
+        "dtype": "f32 (factors, values, design, MTTKRP) + f64 (Grams, solves, scores)",
         "data": "synthetic",
-        "config": {"workload": f"batched_arls: {len(opts)} 3-mode tensors, mode sizes U[500, 50000] x {args.scale}, "
-                               f"nnz = 20 x largest mode, uniform coordinates, U(0,1) values, R in {{8,16,32}}, "
-                               f"J={J}, CP-ARLS-LEV, mixed precision, seed 4, sweeps by {mode}"},
-        "eager_batched_sweeps_per_s": round(eager_rate, 4),
-        "one_after_another_sweeps_per_s": round(seq_sweeps / seq_s, 4),
-        "batched_vs_one_after_another": round(rate / (seq_sweeps / seq_s), 3),
-        "roofline": None,
-        "cpu_baseline": None,
+        "config": {"workload": f"batched_arls: {len(batch)} 3-mode tensors, mode sizes U[500, 50000] x {args.scale}, "
+                               f"nnz = 20 x largest mode ({nnz_total} in all), uniform coordinates, U(0,1) values, "
+                               f"R in {{8,16,32}}, J={J}, CP-ARLS-LEV, mixed precision, seed 4, one launch per sweep"
+                               f"{' (graph replay)' if args.batch_graphs else ''}"},
+        "roofline": {"kernel": "batch_sweep_kernel<float>", "bound": "hbm", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "avg_launch_ms": round(avg * 1e3, 4), "algorithmic_bytes_per_launch": int(per)},
+        "cpu_baseline": cpu,
+        "nonzeros_iterated_per_s": round(nz_rate, 1),
+        "unique_samples_per_sweep": int(np.sum(h["counters"][1]) / args.steps),
+        "rows_updated_per_sweep": int(np.sum(h["counters"][2]) / args.steps),
+        "mean_fit": {p: float(np.nanmean(r["fits"])) for p, r in results.items()},
+        "max_fit_gap_mixed_vs_fp64": float(np.nanmax(np.abs(results["mixed"]["fits"] - results["fp64"]["fits"]))),
+        "factors_nonfinite": {p: r["bad"] for p, r in results.items()},
+        "fp64_sweeps_per_s": round(args.steps / results["fp64"]["sec"], 4),
+        "multi_stream_runner_sweeps_per_s": round(streams_rate, 4) if streams_rate else None,
     }
     print(json.dumps(out), flush=True)
-    for c, *_ in objs:
-        c.close()
+    ctx.close()
 
 
 def main():
@@ -460,8 +536,10 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256, help="tensors of the batched workload")
     ap.add_argument("--trajectory", type=int, default=1,
                     help="instrumented pass: fit after every sweep and factor checkpoints (mixed vs FP64)")
-    ap.add_argument("--batch-graphs", type=int, default=0,
-                    help="batched workload: replay each optimizer's sweep from a HIP graph (1) or enqueue it (0)")
+    ap.add_argument("--batch-graphs", type=int, default=1,
+                    help="batched workload: replay the one-launch sweep from a captured HIP graph (1) or launch it (0)")
+    ap.add_argument("--batch-compare", type=int, default=2,
+                    help="batched workload: sweeps timed on the multi-stream runner for comparison (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",

@@ -28,6 +28,7 @@ typedef struct ex_tensor ex_tensor;
 typedef struct ex_factors ex_factors;
 typedef struct ex_sampler ex_sampler;
 typedef struct ex_als ex_als;
+typedef struct ex_batch ex_batch;
 
 enum ex_status {
     EX_OK = 0,
@@ -217,6 +218,40 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch);
  * optimizer alone (no fit evaluations, no statistics).  *seconds (optional)
  * receives the wall time from the first launch to the last completion. */
 int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, double* seconds);
+
+/* (no counterpart: the reference runs decompositions one after another)
+ * Many CP-ARLS-LEV decompositions advanced by ONE kernel launch per sweep:
+ * a workgroup of 1024 threads takes whole tensors from a queue and runs
+ * every mode step of AccumulatorStationary::execute_ALS_step
+ * (accumulator_stationary.hpp:139-295) with the CP-ARLS-LEV sampler
+ * (cp_arls_lev.hpp:25-163) as workgroup-level phases.  Each optimizer keeps
+ * its own tensor, factors, sampler and seed; all share one device and one
+ * precision, R <= 32 and 1 <= J <= 8192; no two share a LowRankTensor.
+ * The batch keeps each mode's leverage scores over the rows where the factor
+ * is nonzero (after a step, the rows a sampled fiber's nonzero reached) and
+ * draws from the CDF over them: the row picked for a uniform draw is the one
+ * the full-length CDF picks.  Arithmetic as the optimizers' precision
+ * (EX_MIXED: FP32 factors, values, design and MTTKRP rows; FP64 Grams,
+ * products and sums of FP32 rows, R x R pseudo-inverses -- pivoted Cholesky
+ * when singular -- scores and CDFs; EX_FP64: all FP64).  The factor update
+ * forms U_j = Y Gi diag(1/sigma) and the sampler's Gram as
+ * M^T (Y^T Y) M, so results agree with ex_als_run's to rounding, not bitwise.
+ * ex_batch_create rebuilds every optimizer's sampler state and copies it;
+ * ex_batch_run runs `rounds` sweeps (draws keyed by sweeps 1..rounds, as
+ * ex_als_run keys them), each one launch (or one graph replay after
+ * ex_batch_use_graph(b, 1)); *seconds (optional) receives the wall time of
+ * the launches.  A factor written outside the batch after its creation makes
+ * ex_batch_run fail with EX_ERR_STATE.  After a run the optimizers' own
+ * sampler state is rebuilt before their next draw, so ex_als_run and fits
+ * continue from the batch's factors.  ex_batch_counters fills, per optimizer
+ * (caller's order), the nonzeros iterated, the unique samples and the factor
+ * rows written (the listed rows), each summed over the batch's mode steps
+ * (any pointer may be NULL). */
+int ex_batch_create(ex_als* const* als, int n, int64_t J, ex_batch** out);
+int ex_batch_run(ex_batch* b, int64_t rounds, double* seconds);
+int ex_batch_use_graph(ex_batch* b, int on);
+int ex_batch_counters(ex_batch* b, int64_t* nonzeros, int64_t* unique_samples, int64_t* rows_updated);
+int ex_batch_destroy(ex_batch* b);
 
 /* (no counterpart: sweeps replayed from a captured hipGraph) With on != 0,
  * ex_als_run and ex_als_run_batch run the first sweep at a given J eagerly,

@@ -61,6 +61,11 @@ SIGNATURES = {
     "ex_tensor_mttkrp_path": [_vp, C.c_int],
     "ex_als_run_batch": [_pp, C.c_int, C.c_int64, C.c_int64, _d],
     "ex_als_use_graph": [_vp, C.c_int],
+    "ex_batch_create": [_pp, C.c_int, C.c_int64, _pp],
+    "ex_batch_run": [_vp, C.c_int64, _d],
+    "ex_batch_use_graph": [_vp, C.c_int],
+    "ex_batch_counters": [_vp, _i64, _i64, _i64],
+    "ex_batch_destroy": [_vp],
     "ex_smttkrp_forward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp],
     "ex_smttkrp_backward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp, _vp],
     "ex_als_destroy": [_vp],
@@ -414,3 +419,35 @@ class ALS(_Owned):
         buf = C.create_string_buffer(n.value)
         check(lib().ex_als_stats_json(self.h, buf, C.byref(n)))
         return buf.value.decode()
+
+
+class Batch(_Owned):
+    """CP-ARLS-LEV optimizers advanced together by one kernel launch per sweep
+    (ex_batch_create): same device and precision, R <= 32, J <= 8192."""
+    _destroy = "ex_batch_destroy"
+
+    def __init__(self, optimizers, J):
+        self.keep = tuple(optimizers)
+        arr = (C.c_void_p * len(optimizers))(*[o.h.value for o in optimizers])
+        self.h = C.c_void_p()
+        check(lib().ex_batch_create(arr, len(optimizers), J, C.byref(self.h)))
+        self._register(optimizers[0].ctx)
+
+    def run(self, rounds):
+        """`rounds` sweeps of every optimizer; returns the wall seconds."""
+        sec = C.c_double()
+        check(lib().ex_batch_run(self.h, rounds, C.byref(sec)))
+        return sec.value
+
+    def use_graph(self, on=True):
+        check(lib().ex_batch_use_graph(self.h, 1 if on else 0))
+
+    def counters(self):
+        """(nonzeros iterated, unique samples, factor rows written) per
+        optimizer, summed over the batch's mode steps."""
+        n = len(self.keep)
+        nz = np.zeros(n, dtype=np.int64)
+        ju = np.zeros(n, dtype=np.int64)
+        rows = np.zeros(n, dtype=np.int64)
+        check(lib().ex_batch_counters(self.h, _p(nz, _i64), _p(ju, _i64), _p(rows, _i64)))
+        return nz, ju, rows

@@ -21,9 +21,9 @@ LIB = os.path.join(PKG, "libexafac_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["api.hip", "kernels_dense.hip", "kernels_sample.hip", "kernels_sparse.hip", "kernels_mixed.hip", "sort.hip",
-               "tensor_io.hip"]
-HEADERS = ["common.hpp", "device.hpp", "dsymm.hpp"]
+HIP_SOURCES = ["api.hip", "kernels_dense.hip", "kernels_sample.hip", "kernels_sparse.hip", "kernels_mixed.hip",
+               "kernels_batch.hip", "sort.hip", "tensor_io.hip"]
+HEADERS = ["common.hpp", "device.hpp", "dsymm.hpp", "spd_block.hpp", "jacobi_block.hpp"]
 
 HIP_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -254,14 +254,19 @@ struct ex_factors {
     void sync32_all() {
         for (int k2 = 0; k2 < N; k2++) sync32(k2);
     }
+    // bumped by every write of a factor (a batch checks that nothing else
+    // wrote its factors between its launches)
+    uint64_t version = 0;
     // the FP64 (wrote64) or FP32 (wrote32) copy of `mode` was written
     void wrote64(int mode) {
         stale64[mode] = false;
         if (has32) stale32[mode] = true;
+        version++;
     }
     void wrote32(int mode) {
         stale32[mode] = false;
         stale64[mode] = true;
+        version++;
     }
 };
 
@@ -285,6 +290,9 @@ struct ex_sampler {
     // u64: STS-CP draws given weight 0 since creation (mixed path: a
     // rounding residue routed the draw into a leaf of zero mass)
     DevBuf dropped;
+    // set when a batch advanced the factors: every mode's state is rebuilt
+    // before the next draw or score read
+    bool refresh = false;
 };
 
 struct ex_als {
@@ -375,6 +383,13 @@ void sampler_update_impl(ex_sampler* s, int j) {
         k::unpack_root<double>(st, s->nodes[j].as<double>(), R, s->grams.as<double>() + (int64_t)j * R * R);
     }
     sp.end();
+}
+
+// Rebuilds every mode's state after a batch advanced the factors.
+void sampler_refresh(ex_sampler* s) {
+    if (!s || !s->refresh) return;
+    for (int k2 = 0; k2 < s->f->N; k2++) sampler_update_impl(s, k2);
+    s->refresh = false;
 }
 
 void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const double* draws_row_dev,
@@ -1274,6 +1289,7 @@ int ex_sampler_draw(ex_sampler* s, int64_t J, int mode, uint64_t sweep, const do
         EX_CHECK(s && samples && weights, EX_ERR_ARG, "null argument");
         EX_CHECK(mode >= 0 && mode < s->f->N, EX_ERR_ARG, "mode out of range");
         EX_CHECK(J > 0 && J < (int64_t)1 << 31, EX_ERR_ARG, "sample count J must be in [1, 2^31)");
+        sampler_refresh(s);
         const int N = s->f->N;
         hipStream_t st = s->ctx->c.stream;
         const double* dr = nullptr;
@@ -1325,6 +1341,7 @@ int ex_arls_scores(ex_sampler* s, int mode, double* scores) {
         EX_CHECK(s && scores, EX_ERR_ARG, "null argument");
         EX_CHECK(s->kind == EX_SAMPLER_CP_ARLS_LEV, EX_ERR_STATE, "not a CP-ARLS-LEV sampler");
         EX_CHECK(mode >= 0 && mode < s->f->N, EX_ERR_ARG, "mode out of range");
+        sampler_refresh(s);
         d2h(s->ctx->c.stream, scores, s->scores[mode].p, (size_t)s->f->dims[mode] * 8);
     });
 }
@@ -1410,6 +1427,7 @@ int ex_als_step(ex_als* a, int mode, int64_t J, uint64_t sweep) {
             a->grams_valid = false;
             exact_step_impl(a, mode);
         } else {
+            sampler_refresh(a->s);
             als_step_impl(a, mode, J, sweep);
         }
     });
@@ -1451,6 +1469,7 @@ int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, doubl
         }
         bool graphs = true;
         for (int t = 0; t < n; t++) {
+            sampler_refresh(als[t]->s);
             als_resolve_regions(als[t]);
             EX_HIP(hipStreamSynchronize(als[t]->ctx->c.stream));
             graphs = graphs && als[t]->use_graph && !als[t]->exact;
@@ -1497,6 +1516,325 @@ int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, doubl
     });
 }
 
+// ---------------------------------------------------------------- batched sweeps
+// Many CP-ARLS-LEV decompositions advanced by one kernel launch per sweep
+// (kernels_batch.hip): the descriptors, the per-tensor leverage state and
+// scratch live in one device arena; the launches run on the batch's own
+// stream, ordered after and before the optimizers' context streams.
+struct ex_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<ex_als*> als;          // the caller's order
+    std::vector<int> slot;             // descriptor slot of each optimizer
+    std::vector<uint64_t> version;     // factor versions after the batch's last write
+    std::vector<size_t> stats_off;     // arena offset of each optimizer's counters
+    int precision = 0;
+    int64_t J = 0;
+    int grid = 0;
+    DevBuf desc, ctl, arena;
+    unsigned long long first_sweep = 1;  // host source of the sweep reset
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+void batch_drop_graph(ex_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
+}
+
+// Orders the batch stream after (to_batch) or each context stream after
+// (!to_batch) the other side's queued work.
+void batch_order(ex_batch* b, bool to_batch) {
+    std::vector<ex_ctx*> ctxs;
+    for (ex_als* a : b->als)
+        if (std::find(ctxs.begin(), ctxs.end(), a->ctx) == ctxs.end()) ctxs.push_back(a->ctx);
+    while (b->events.size() < ctxs.size() + 1) {
+        hipEvent_t e;
+        EX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->events.push_back(e);
+    }
+    if (to_batch) {
+        for (size_t i = 0; i < ctxs.size(); i++) {
+            EX_HIP(hipEventRecord(b->events[i], ctxs[i]->c.stream));
+            EX_HIP(hipStreamWaitEvent(b->stream, b->events[i], 0));
+        }
+    } else {
+        hipEvent_t e = b->events[ctxs.size()];
+        EX_HIP(hipEventRecord(e, b->stream));
+        for (ex_ctx* c : ctxs) EX_HIP(hipStreamWaitEvent(c->c.stream, e, 0));
+    }
+}
+
+}  // namespace
+
+int ex_batch_create(ex_als* const* als, int n, int64_t J, ex_batch** out) {
+    return guarded([&] {
+        EX_CHECK(als && n >= 1 && out, EX_ERR_ARG, "no optimizers");
+        EX_CHECK(J >= 1 && J <= kBatchMaxJ, EX_ERR_UNSUPPORTED,
+                 "batched sweeps take 1 <= J <= " + std::to_string(kBatchMaxJ) + " samples");
+        auto b = std::make_unique<ex_batch>();
+        b->J = J;
+        for (int t = 0; t < n; t++) {
+            ex_als* a = als[t];
+            EX_CHECK(a, EX_ERR_ARG, "null optimizer");
+            EX_CHECK(a->initialized, EX_ERR_STATE, "initialize_ground_truth_for_als() has not been called");
+            EX_CHECK(!a->exact && a->s && a->s->kind == EX_SAMPLER_CP_ARLS_LEV, EX_ERR_UNSUPPORTED,
+                     "batched sweeps run CP-ARLS-LEV optimizers");
+            EX_CHECK(a->f->R <= kBatchMaxRank, EX_ERR_UNSUPPORTED,
+                     "batched sweeps take R <= " + std::to_string(kBatchMaxRank));
+            if (t == 0) {
+                b->precision = a->precision;
+                b->device = a->ctx->c.device;
+            }
+            EX_CHECK(a->precision == b->precision, EX_ERR_ARG, "optimizers of one batch share a precision");
+            EX_CHECK(a->ctx->c.device == b->device, EX_ERR_ARG, "optimizers of one batch share a device");
+            for (int u = 0; u < t; u++)
+                EX_CHECK(als[u]->f != a->f, EX_ERR_ARG, "two optimizers of one batch share a LowRankTensor");
+            b->als.push_back(a);
+        }
+        EX_HIP(hipSetDevice(b->device));
+        const bool mixed = b->precision == EX_MIXED;
+        const size_t vb = mixed ? 4 : 8;
+        // current factors and sampler state on every context stream
+        for (ex_als* a : b->als) {
+            if (mixed) a->f->sync32_all();
+            else a->f->sync64_all();
+            for (int k2 = 0; k2 < a->f->N; k2++) {
+                ModeLookup& L = a->t->lk[k2];
+                if (mixed && !L.rval32.p) {
+                    L.rval32.ensure((size_t)(L.nnz > 0 ? L.nnz : 1) * 4);
+                    k::to_f32(a->ctx->c.stream, L.rval.as<double>(), L.nnz, L.rval32.as<float>());
+                }
+            }
+            a->s->refresh = false;
+            for (int k2 = 0; k2 < a->f->N; k2++) sampler_update_impl(a->s, k2);
+        }
+        // descriptor slots: the most expensive tensors first (the queue hands
+        // them out in slot order)
+        std::vector<double> cost((size_t)n);
+        for (int t = 0; t < n; t++) {
+            const ex_als* a = b->als[(size_t)t];
+            double c = (double)a->f->R * a->f->R * (double)J * a->f->N + (double)a->t->nnz;
+            for (int k2 = 0; k2 < a->f->N; k2++) c += (double)a->f->dims[k2] * a->f->R;
+            cost[(size_t)t] = c;
+        }
+        std::vector<int> order((size_t)n);
+        for (int t = 0; t < n; t++) order[(size_t)t] = t;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[(size_t)x] > cost[(size_t)y]; });
+        b->slot.assign((size_t)n, 0);
+        for (int q = 0; q < n; q++) b->slot[(size_t)order[(size_t)q]] = q;
+        // arena layout
+        auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        struct Plan {
+            size_t samples, weights, sd, kd, wd, dg, dm, Yc, hu, hstamp, rstamp, cnt, ssum, stats;
+            size_t list[kMaxModes], csc[kMaxModes], ccdf[kMaxModes], rowptr[kMaxModes];
+        };
+        std::vector<Plan> plan((size_t)n);
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            const size_t o = off;
+            off += al(bytes > 0 ? bytes : 1);
+            return o;
+        };
+        for (int t = 0; t < n; t++) {
+            const ex_als* a = b->als[(size_t)t];
+            const int N = a->f->N, R = a->f->R;
+            int64_t Imax = 1;
+            for (int k2 = 0; k2 < N; k2++) Imax = std::max(Imax, a->f->dims[k2]);
+            const size_t nnz = (size_t)std::max<int64_t>(a->t->nnz, 1);
+            Plan& P = plan[(size_t)t];
+            P.samples = take((size_t)J * N * 4);
+            P.weights = take((size_t)J * 8);
+            P.sd = take((size_t)J * N * 4);
+            P.kd = take((size_t)J * 8);
+            P.wd = take((size_t)J * 8);
+            P.dg = take((size_t)J * R * vb);
+            P.dm = take((size_t)J * R * vb);
+            P.Yc = take((size_t)Imax * R * vb);
+            P.hu = take(nnz * 4);
+            P.hstamp = take(nnz * 4);
+            P.rstamp = take((size_t)Imax * 4);
+            P.cnt = take(kMaxModes * 4);
+            P.ssum = take(kMaxModes * 8);
+            P.stats = take(24);
+            for (int k2 = 0; k2 < N; k2++) {
+                P.list[k2] = take((size_t)a->f->dims[k2] * 4);
+                P.csc[k2] = take((size_t)a->f->dims[k2] * 8);
+                P.ccdf[k2] = take((size_t)a->f->dims[k2] * 8);
+                P.rowptr[k2] = take((size_t)(a->f->dims[k2] + 1) * 4);
+            }
+        }
+        EX_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        b->arena.ensure(off);
+        b->desc.ensure(sizeof(BatchTensor) * (size_t)n);
+        b->ctl.ensure(sizeof(BatchCtl));
+        EX_HIP(hipMemsetAsync(b->arena.p, 0, off, b->stream));
+        EX_HIP(hipMemsetAsync(b->ctl.p, 0, sizeof(BatchCtl), b->stream));
+        unsigned char* base = b->arena.as<unsigned char>();
+        std::vector<BatchTensor> host((size_t)n);
+        b->stats_off.assign((size_t)n, 0);
+        for (int t = 0; t < n; t++) {
+            ex_als* a = b->als[(size_t)t];
+            const Plan& P = plan[(size_t)t];
+            BatchTensor& T = host[(size_t)b->slot[(size_t)t]];
+            std::memset(&T, 0, sizeof(T));
+            T.N = a->f->N;
+            T.R = a->f->R;
+            T.nnz = a->t->nnz;
+            T.seed = a->s->seed;
+            for (int k2 = 0; k2 < T.N; k2++) {
+                BatchMode& M = T.m[k2];
+                const ModeLookup& L = a->t->lk[k2];
+                M.U = mixed ? a->f->U32[k2].p : a->f->U[k2].p;
+                M.I = a->f->dims[k2];
+                M.fkey = L.fkey.as<uint64_t>();
+                M.nfib = L.nfib;
+                M.fptr = L.fptr.as<uint32_t>();
+                M.kpos = L.kpos.as<uint32_t>();
+                M.rrow = L.rrow.as<uint32_t>();
+                M.rval = mixed ? L.rval32.p : L.rval.p;
+                M.rowptr = reinterpret_cast<uint32_t*>(base + P.rowptr[k2]);
+                for (int q = 0; q < kMaxModes; q++) M.shift[q] = q < T.N ? L.kl.shift[q] : -1;
+                M.list = reinterpret_cast<uint32_t*>(base + P.list[k2]);
+                M.csc = reinterpret_cast<double*>(base + P.csc[k2]);
+                M.ccdf = reinterpret_cast<double*>(base + P.ccdf[k2]);
+            }
+            T.sigma = a->f->sigma.as<double>();
+            T.cnt = reinterpret_cast<uint32_t*>(base + P.cnt);
+            T.ssum = reinterpret_cast<double*>(base + P.ssum);
+            T.stats = reinterpret_cast<unsigned long long*>(base + P.stats);
+            T.samples = reinterpret_cast<uint32_t*>(base + P.samples);
+            T.weights = reinterpret_cast<double*>(base + P.weights);
+            T.sd = reinterpret_cast<uint32_t*>(base + P.sd);
+            T.kd = reinterpret_cast<uint64_t*>(base + P.kd);
+            T.wd = reinterpret_cast<double*>(base + P.wd);
+            T.dg = base + P.dg;
+            T.dm = base + P.dm;
+            T.Yc = base + P.Yc;
+            T.hu = reinterpret_cast<uint32_t*>(base + P.hu);
+            T.hstamp = reinterpret_cast<uint32_t*>(base + P.hstamp);
+            T.rstamp = reinterpret_cast<uint32_t*>(base + P.rstamp);
+            b->stats_off[(size_t)t] = P.stats;
+        }
+        // the sampler state (every row listed) after the context streams' work
+        batch_order(b.get(), true);
+        EX_HIP(hipMemcpyAsync(b->desc.p, host.data(), sizeof(BatchTensor) * (size_t)n, hipMemcpyHostToDevice,
+                              b->stream));
+        k::batch_init(b->stream, b->desc.as<BatchTensor>(), n);
+        std::vector<uint32_t> counts((size_t)n * kMaxModes, 0);
+        for (int t = 0; t < n; t++) {
+            ex_als* a = b->als[(size_t)t];
+            const BatchTensor& T = host[(size_t)b->slot[(size_t)t]];
+            for (int k2 = 0; k2 < T.N; k2++) {
+                const size_t I = (size_t)a->f->dims[k2];
+                EX_HIP(hipMemcpyAsync(T.m[k2].csc, a->s->scores[k2].p, I * 8, hipMemcpyDeviceToDevice, b->stream));
+                EX_HIP(hipMemcpyAsync(T.m[k2].ccdf, a->s->cdf[k2].p, I * 8, hipMemcpyDeviceToDevice, b->stream));
+                counts[(size_t)t * kMaxModes + k2] = (uint32_t)I;
+            }
+            EX_HIP(hipMemcpyAsync(T.ssum, a->s->sums.p, (size_t)T.N * 8, hipMemcpyDeviceToDevice, b->stream));
+            EX_HIP(hipMemcpyAsync(T.cnt, counts.data() + (size_t)t * kMaxModes, kMaxModes * 4, hipMemcpyHostToDevice,
+                                  b->stream));
+        }
+        EX_HIP(hipStreamSynchronize(b->stream));
+        // one workgroup per CU at most (1024 threads and about 120 KiB of LDS each)
+        const int cus = std::max(1, b->als[0]->ctx->c.prop.multiProcessorCount);
+        b->grid = std::min(n, cus);
+        b->version.assign((size_t)n, 0);
+        for (int t = 0; t < n; t++) b->version[(size_t)t] = b->als[(size_t)t]->f->version;
+        *out = b.release();
+    });
+}
+
+int ex_batch_use_graph(ex_batch* b, int on) {
+    return guarded([&] {
+        EX_CHECK(b, EX_ERR_ARG, "null batch");
+        EX_HIP(hipSetDevice(b->device));
+        EX_HIP(hipStreamSynchronize(b->stream));
+        b->use_graph = on != 0;
+        if (!b->use_graph) batch_drop_graph(b);
+    });
+}
+
+int ex_batch_run(ex_batch* b, int64_t rounds, double* seconds) {
+    return guarded([&] {
+        EX_CHECK(b, EX_ERR_ARG, "null batch");
+        EX_CHECK(rounds >= 0, EX_ERR_ARG, "negative round count");
+        EX_HIP(hipSetDevice(b->device));
+        const int n = (int)b->als.size();
+        for (int t = 0; t < n; t++)
+            EX_CHECK(b->als[(size_t)t]->f->version == b->version[(size_t)t], EX_ERR_STATE,
+                     "a factor of the batch was written outside it since its last sweep: create a new batch");
+        batch_order(b, true);
+        // sweeps are numbered from 1 in every run, as ex_als_run numbers them
+        EX_HIP(hipMemcpyAsync(b->ctl.p, &b->first_sweep, sizeof(b->first_sweep), hipMemcpyHostToDevice,
+                              b->stream));
+        const int vb = b->precision == EX_MIXED ? 4 : 8;
+        BatchTensor* desc = b->desc.as<BatchTensor>();
+        BatchCtl* ctl = b->ctl.as<BatchCtl>();
+        if (b->use_graph && !b->gexec && rounds > 0) {
+            EX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            k::batch_sweep(b->stream, desc, n, b->J, ctl, b->grid, vb);
+            EX_HIP(hipStreamEndCapture(b->stream, &b->graph));
+            EX_HIP(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        }
+        EX_HIP(hipStreamSynchronize(b->stream));
+        auto t0 = std::chrono::steady_clock::now();
+        for (int64_t r = 0; r < rounds; r++) {
+            if (b->gexec) EX_HIP(hipGraphLaunch(b->gexec, b->stream));
+            else k::batch_sweep(b->stream, desc, n, b->J, ctl, b->grid, vb);
+        }
+        EX_HIP(hipStreamSynchronize(b->stream));
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rounds > 0)
+            for (int t = 0; t < n; t++) {
+                ex_als* a = b->als[(size_t)t];
+                for (int k2 = 0; k2 < a->f->N; k2++) {
+                    if (b->precision == EX_MIXED) a->f->wrote32(k2);
+                    else a->f->wrote64(k2);
+                }
+                a->s->refresh = true;
+                a->grams_valid = false;
+                b->version[(size_t)t] = a->f->version;
+            }
+        batch_order(b, false);
+    });
+}
+
+int ex_batch_counters(ex_batch* b, int64_t* nonzeros, int64_t* unique_samples, int64_t* rows_updated) {
+    return guarded([&] {
+        EX_CHECK(b, EX_ERR_ARG, "null batch");
+        EX_HIP(hipSetDevice(b->device));
+        const size_t n = b->als.size();
+        std::vector<unsigned long long> v(3);
+        for (size_t t = 0; t < n; t++) {
+            EX_HIP(hipMemcpyAsync(v.data(), b->arena.as<unsigned char>() + b->stats_off[t], 24,
+                                  hipMemcpyDeviceToHost, b->stream));
+            EX_HIP(hipStreamSynchronize(b->stream));
+            if (nonzeros) nonzeros[t] = (int64_t)v[0];
+            if (unique_samples) unique_samples[t] = (int64_t)v[1];
+            if (rows_updated) rows_updated[t] = (int64_t)v[2];
+        }
+    });
+}
+
+int ex_batch_destroy(ex_batch* b) {
+    return guarded([&] {
+        if (!b) return;
+        (void)hipSetDevice(b->device);
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        batch_drop_graph(b);
+        for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+    });
+}
+
 int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
     return guarded([&] {
         EX_CHECK(a, EX_ERR_ARG, "null optimizer");
@@ -1504,6 +1842,7 @@ int ex_als_run(ex_als* a, int64_t rounds, int64_t J, int64_t epoch) {
         EX_CHECK(rounds >= 0, EX_ERR_ARG, "negative round count");
         EX_CHECK(epoch >= 1, EX_ERR_ARG, "epoch_interval must be positive");
         hipStream_t st = a->ctx->c.stream;
+        sampler_refresh(a->s);
         als_resolve_regions(a);
         for (double& x : a->t_region) x = 0.0;
         a->grams_valid = false;

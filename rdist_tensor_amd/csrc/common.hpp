@@ -288,6 +288,71 @@ size_t exclusive_scan_u32(hipStream_t s, void* temp, size_t temp_bytes, const ui
 size_t exclusive_scan_u64(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* in, uint64_t* out, int64_t n);
 }  // namespace k
 
+// ---------------------------------------------------------------- batched sweeps
+// Device descriptors of the batched CP-ARLS-LEV sweep (one workgroup runs
+// whole tensors).  V = float (mixed precision) or double (FP64) for factor
+// rows, nonzero values, design rows and MTTKRP rows.
+constexpr int kBatchMaxRank = 32;
+constexpr int kBatchMaxJ = 8192;
+
+struct BatchMode {
+    void* U;                  // V[I x R]: the factor
+    int64_t I;
+    // lookup with this mode as the target (ModeLookup)
+    const uint64_t* fkey;     // nfib distinct fiber keys, ascending
+    int64_t nfib;
+    const uint32_t* fptr;     // nfib + 1 key-order offsets
+    const uint32_t* kpos;     // row-order position of each key-ordered nonzero
+    const uint32_t* rrow;     // row of each row-order position
+    const void* rval;         // V value of each row-order position
+    uint32_t* rowptr;         // I + 1: first row-order position of each row
+    int shift[kMaxModes];     // fiber key layout (-1 for this mode)
+    // leverage state over the rows where the factor is nonzero
+    uint32_t* list;           // rows, ascending (capacity I)
+    double* csc;              // their leverage scores
+    double* ccdf;             // CDF over them (last entry 1)
+};
+
+struct BatchTensor {
+    int N, R;
+    int64_t nnz;
+    uint64_t seed;
+    BatchMode m[kMaxModes];
+    double* sigma;            // R: column norms of the last updated factor
+    uint32_t* cnt;            // [kMaxModes] listed rows per mode
+    double* ssum;             // [kMaxModes] score sums
+    unsigned long long* stats;  // running totals: [0] nonzeros iterated, [1] unique samples, [2] rows updated
+    // scratch
+    uint32_t* samples;        // J x N
+    double* weights;          // J
+    uint32_t* sd;             // J x N unique tuples, first-occurrence order
+    uint64_t* kd;             // J their fiber keys
+    double* wd;               // J weight x multiplicity
+    void* dg;                 // V[J x R] design rows (Gram)
+    void* dm;                 // V[J x R] design rows (MTTKRP)
+    void* Yc;                 // V[Imax x R] MTTKRP rows of the listed rows
+    uint32_t* hu;             // nnz: design row of a hit position
+    uint32_t* hstamp;         // nnz: step stamp of a hit position
+    uint32_t* rstamp;         // Imax: step stamp of a hit row
+};
+
+// Read by every workgroup at launch start; the last workgroup to finish
+// advances sweep and epoch and resets the queue.
+struct BatchCtl {
+    unsigned long long sweep;  // keys the draws of the launch
+    unsigned int epoch;        // launches since creation (step stamps)
+    unsigned int next;         // tensor queue
+    unsigned int done;         // finished workgroups
+    unsigned int pad;
+};
+
+namespace k {
+// list = iota and rowptr for every mode of every tensor
+void batch_init(hipStream_t s, const BatchTensor* ts, int n);
+// one sweep of every tensor (grid workgroups); vbytes 4 (mixed) or 8 (FP64)
+void batch_sweep(hipStream_t s, const BatchTensor* ts, int n, int64_t J, BatchCtl* ctl, int grid, int vbytes);
+}  // namespace k
+
 inline unsigned grid_for(int64_t work, int per_block, int64_t cap = 1 << 20) {
     int64_t g = (work + per_block - 1) / per_block;
     if (g < 1) g = 1;

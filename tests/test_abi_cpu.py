@@ -54,6 +54,9 @@ def test_errors_are_status_codes_without_gpu():
     assert L.ex_tensor_norm(None, None) == abi.EX_ERR_ARG
     assert L.ex_als_run(None, 1, 1, 1) == abi.EX_ERR_ARG
     assert b"null" in L.ex_last_error()
+    assert L.ex_batch_run(None, 1, None) == abi.EX_ERR_ARG
+    assert L.ex_batch_create(None, 0, 1024, C.byref(h)) == abi.EX_ERR_ARG
+    assert L.ex_batch_destroy(None) == abi.EX_OK
 
 
 def test_pymodule_surface():
