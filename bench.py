@@ -470,12 +470,12 @@ def run_batched(args):
             t1 = time.perf_counter()
             a.run(1, J, 10 ** 9)
             spent += time.perf_counter() - t1
-            log(f"[bench] cpu baseline: tensor dims {dims} R={R} done ({spent:.1f} s)")
+            log(f"[bench] cpu baseline: tensor dims {dims} R={R} done ({spent:.3f} s)")
         est = spent * len(batch) / len(sample)
         cpu = {"value": round(1.0 / est, 6), "unit": "sweeps/s", "cores": int(oracle.lib().orc_num_threads()),
                "kind": "port",
                "sample": f"oracle ALS (FP64), one sweep of {len(sample)} of the {len(batch)} tensors (every 32nd), "
-                         f"{spent:.1f} s, scaled by {len(batch)}/{len(sample)}; lookups built untimed"}
+                         f"{spent:.3f} s, scaled by {len(batch)}/{len(sample)}; lookups built untimed"}
     h = results["mixed"]
     rate = args.steps / h["sec"]
     per = batch_sweep_bytes(batch, J, 4, h["counters"], args.steps) / args.steps

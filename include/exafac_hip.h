@@ -283,6 +283,22 @@ int ex_tns_scan(const char* path, int64_t* nnz, int* nmodes);
 int ex_tns_read(const char* path, int64_t nnz, int nmodes, uint64_t* coords, double* vals, uint64_t* min_idx,
                 uint64_t* max_idx);
 
+/* ---- launch variants (no reference counterpart) --------------------------- */
+/* Compiled alternatives for the STS-CP draw (lanes per sample, vectors per
+ * lane, threads per workgroup, per precision and rank class) and the
+ * sampled-MTTKRP row pass (lanes per hit), chosen by the autotuner
+ * rdist_tensor_amd/autotune.py, which times every variant on the GPU and
+ * writes tuning.txt beside the library.  The first ex_ctx_create of a
+ * process loads that file ($EXAFAC_TUNING names another file; "off" keeps
+ * the defaults).  Every variant computes the same draws and sums up to
+ * rounding (FP64 draws stay index-exact).  Knobs are numbered 0..n-1;
+ * variant 0 is the built-in default.  A file line is "name value" ('#'
+ * starts a comment); an unknown name or an out-of-range value is an error. */
+int ex_tuning_count(int* n);
+int ex_tuning_info(int knob, char* name, size_t len, int* variants, int* value);
+int ex_tuning_set(const char* name, int value);
+int ex_tuning_load(const char* path);
+
 /* ---- dense R x R kernels (parity hooks; host in/out) ----------------------- */
 /* compute_gram (common.h:363-403): G = A^T A, A is m x R. */
 int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G);

@@ -66,6 +66,10 @@ SIGNATURES = {
     "ex_batch_use_graph": [_vp, C.c_int],
     "ex_batch_counters": [_vp, _i64, _i64, _i64],
     "ex_batch_destroy": [_vp],
+    "ex_tuning_count": [C.POINTER(C.c_int)],
+    "ex_tuning_info": [C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "ex_tuning_set": [C.c_char_p, C.c_int],
+    "ex_tuning_load": [C.c_char_p],
     "ex_smttkrp_forward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp],
     "ex_smttkrp_backward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp, _vp],
     "ex_als_destroy": [_vp],
@@ -127,6 +131,27 @@ def f64(a):
 
 def u32(a):
     return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def tuning():
+    """{knob name: (variants, current value)} of the launch variants."""
+    n = C.c_int()
+    check(lib().ex_tuning_count(C.byref(n)))
+    out = {}
+    for q in range(n.value):
+        buf = C.create_string_buffer(64)
+        nv, v = C.c_int(), C.c_int()
+        check(lib().ex_tuning_info(q, buf, 64, C.byref(nv), C.byref(v)))
+        out[buf.value.decode()] = (nv.value, v.value)
+    return out
+
+
+def tuning_set(name, value):
+    check(lib().ex_tuning_set(name.encode(), value))
+
+
+def tuning_load(path):
+    check(lib().ex_tuning_load(os.fsencode(path)))
 
 
 class _Owned:

@@ -4,7 +4,10 @@
 // Reference flow: ALS_Optimizer::execute_ALS_rounds (als_optimizer.hpp:40-124)
 // over AccumulatorStationary::execute_ALS_step (accumulator_stationary.hpp:139-295).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <dlfcn.h>
+#include <fstream>
 #include <exception>
 #include <thread>
 #include <cmath>
@@ -45,6 +48,67 @@ void lookup_static_scratch(int device) {
 }
 
 thread_local std::string g_last_error;
+
+// Launch variants (common.hpp TuneKnob): names, variant counts, current values.
+const TuneInfo kTuneTable[kTuneKnobs] = {
+    {"sts_draw.fp64.r16", 3}, {"sts_draw.fp64.r32", 4}, {"sts_draw.fp32.r16", 3},
+    {"sts_draw.fp32.r32", 4}, {"sts_draw.fp32.r64", 2}, {"row_scan.fp64", 2},
+    {"row_scan.fp32", 2},
+};
+std::atomic<int> g_tune[kTuneKnobs];
+std::once_flag g_tune_once;
+
+int tune_index(const std::string& name) {
+    for (int q = 0; q < kTuneKnobs; q++)
+        if (name == kTuneTable[q].name) return q;
+    return -1;
+}
+
+// "name value" lines ('#' starts a comment); every name must be a knob and
+// every value one of its variants.
+void load_tuning_file(const std::string& path) {
+    std::ifstream in(path);
+    EX_CHECK(in.good(), EX_ERR_ARG, "cannot open tuning file " + path);
+    std::string line;
+    int ln = 0;
+    while (std::getline(in, line)) {
+        ln++;
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line.resize(h);
+        std::istringstream ss(line);
+        std::string name;
+        if (!(ss >> name)) continue;
+        int v = -1;
+        EX_CHECK((bool)(ss >> v), EX_ERR_ARG, path + ":" + std::to_string(ln) + ": expected 'name value'");
+        const int q = tune_index(name);
+        EX_CHECK(q >= 0, EX_ERR_ARG, path + ":" + std::to_string(ln) + ": unknown knob " + name);
+        EX_CHECK(v >= 0 && v < kTuneTable[q].variants, EX_ERR_ARG,
+                 path + ":" + std::to_string(ln) + ": " + name + " has no variant " + std::to_string(v));
+        g_tune[q].store(v);
+    }
+}
+
+// tuning.txt beside the library (or $EXAFAC_TUNING; "off" keeps the
+// defaults), read once per process by the first context.
+void load_default_tuning() {
+    std::call_once(g_tune_once, [] {
+        const char* env = std::getenv("EXAFAC_TUNING");
+        std::string path;
+        if (env && *env) {
+            if (std::string(env) == "off") return;
+            path = env;
+        } else {
+            Dl_info info;
+            if (!dladdr(reinterpret_cast<void*>(&load_tuning_file), &info) || !info.dli_fname) return;
+            path = info.dli_fname;
+            const size_t sl = path.rfind('/');
+            path = (sl == std::string::npos ? std::string(".") : path.substr(0, sl)) + "/tuning.txt";
+            std::ifstream probe(path);
+            if (!probe.good()) return;
+        }
+        load_tuning_file(path);
+    });
+}
 
 template <class F>
 int guarded(F&& f) {
@@ -99,6 +163,12 @@ std::string json_list(const std::vector<T>& v) {
 }
 
 }  // namespace
+
+namespace exafac {
+const TuneInfo& tune_info(int knob) { return kTuneTable[knob]; }
+int tune_value(int knob) { return g_tune[knob].load(std::memory_order_relaxed); }
+void tune_select(int knob, int value) { g_tune[knob].store(value); }
+}  // namespace exafac
 
 struct ex_ctx {
     Ctx c;
@@ -887,6 +957,7 @@ int ex_ctx_create(int device, ex_ctx** out) {
         EX_HIP(hipGetDeviceCount(&n));
         EX_CHECK(device >= 0 && device < n, EX_ERR_ARG, "device index out of range");
         EX_CHECK(device < kMaxDevices, EX_ERR_UNSUPPORTED, "device index above 63");
+        load_default_tuning();
         auto c = std::make_unique<ex_ctx>();
         c->c.device = device;
         EX_HIP(hipSetDevice(device));
@@ -1914,6 +1985,43 @@ int ex_als_stats_json(ex_als* a, char* buf, size_t* len) {
 }
 
 // ---------------------------------------------------------------- dense hooks
+int ex_tuning_count(int* n) {
+    return guarded([&] {
+        EX_CHECK(n, EX_ERR_ARG, "null argument");
+        *n = kTuneKnobs;
+    });
+}
+
+int ex_tuning_info(int knob, char* name, size_t len, int* variants, int* value) {
+    return guarded([&] {
+        EX_CHECK(knob >= 0 && knob < kTuneKnobs, EX_ERR_ARG, "knob index out of range");
+        if (name && len) {
+            std::strncpy(name, kTuneTable[knob].name, len - 1);
+            name[len - 1] = 0;
+        }
+        if (variants) *variants = kTuneTable[knob].variants;
+        if (value) *value = g_tune[knob].load();
+    });
+}
+
+int ex_tuning_set(const char* name, int value) {
+    return guarded([&] {
+        EX_CHECK(name, EX_ERR_ARG, "null argument");
+        const int q = tune_index(name);
+        EX_CHECK(q >= 0, EX_ERR_ARG, std::string("unknown knob ") + name);
+        EX_CHECK(value >= 0 && value < kTuneTable[q].variants, EX_ERR_ARG,
+                 std::string(name) + " has no variant " + std::to_string(value));
+        g_tune[q].store(value);
+    });
+}
+
+int ex_tuning_load(const char* path) {
+    return guarded([&] {
+        EX_CHECK(path, EX_ERR_ARG, "null argument");
+        load_tuning_file(path);
+    });
+}
+
 int ex_gram(ex_ctx* ctx, const double* A, int64_t m, int64_t R, double* G) {
     return guarded([&] {
         EX_CHECK(ctx && G && (m == 0 || A), EX_ERR_ARG, "null argument");

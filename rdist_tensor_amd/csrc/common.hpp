@@ -288,6 +288,29 @@ size_t exclusive_scan_u32(hipStream_t s, void* temp, size_t temp_bytes, const ui
 size_t exclusive_scan_u64(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* in, uint64_t* out, int64_t n);
 }  // namespace k
 
+// ---------------------------------------------------------------- launch variants
+// Compiled alternatives an offline autotuner chooses between
+// (rdist_tensor_amd/autotune.py measures them and writes tuning.txt beside
+// the library, which the first context loads).  Value 0 is the built-in
+// default; every variant computes the same quantities.
+enum TuneKnob {
+    kTuneDrawF64R16,   // STS-CP draw, FP64 nodes, R <= 16: (lanes per sample, vectors per lane, threads)
+    kTuneDrawF64R32,   // FP64, 16 < R <= 32
+    kTuneDrawF32R16,   // FP32 nodes, R <= 16
+    kTuneDrawF32R32,   // FP32, 16 < R <= 32
+    kTuneDrawF32R64,   // FP32, 45 < R <= 64
+    kTuneScanF64,      // sampled-MTTKRP row pass, FP64: lanes per hit x 1 or x 2
+    kTuneScanF32,      // FP32
+    kTuneKnobs
+};
+struct TuneInfo {
+    const char* name;
+    int variants;
+};
+const TuneInfo& tune_info(int knob);
+int tune_value(int knob);
+void tune_select(int knob, int value);
+
 // ---------------------------------------------------------------- batched sweeps
 // Device descriptors of the batched CP-ARLS-LEV sweep (one workgroup runs
 // whole tensors).  V = float (mixed precision) or double (FP64) for factor

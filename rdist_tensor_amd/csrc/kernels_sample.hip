@@ -715,21 +715,58 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
     for (int k = 0; k < N; k++)
         if (k != j && ((uintptr_t)modes[k].rows % (size_t)(vw * node_bytes)) != 0) a.vec_leaf = 0;
 #define DRAW_LAUNCH(TN, G, NV, NT)                                                                     \
-    hipLaunchKernelGGL((sts_draw_kernel<TN, G, NV, NT>), dim3(grid_for(J, (NT / 64) * (64 / G), 1LL << 30)), \
-                       dim3(NT), 0, s, a)
+    do {                                                                                               \
+        static_assert(draw_rmax(G, NV, Vec<TN>::W) >= 16, "variant below the smallest rank class");   \
+        EX_CHECK(R <= draw_rmax(G, NV, Vec<TN>::W), 1, "draw variant too narrow for R");              \
+        hipLaunchKernelGGL((sts_draw_kernel<TN, G, NV, NT>), dim3(grid_for(J, (NT / 64) * (64 / G), 1LL << 30)), \
+                           dim3(NT), 0, s, a);                                                        \
+    } while (0)
     // lanes per sample G and W-wide vectors per lane NV: G * NV * W >= R (R + 1) / 2;
     // NT threads per workgroup (1024 where the registers of four waves per
-    // SIMD suffice: more samples share each staged copy of the top levels)
+    // SIMD suffice: more samples share each staged copy of the top levels).
+    // Variant 0 of each rank class is the default; the others are the
+    // alternatives the autotuner measures (tune_value).
     if (node_bytes == 8) {
-        if (R <= 16) DRAW_LAUNCH(double, 16, 5, 1024);
-        else if (R <= 32) DRAW_LAUNCH(double, 32, 9, 1024);
-        else if (R <= 45) DRAW_LAUNCH(double, 64, 9, 1024);
-        else DRAW_LAUNCH(double, 64, 17, 256);
+        if (R <= 16) {
+            switch (tune_value(kTuneDrawF64R16)) {
+                case 1: DRAW_LAUNCH(double, 8, 9, 1024); break;
+                case 2: DRAW_LAUNCH(double, 32, 3, 1024); break;
+                default: DRAW_LAUNCH(double, 16, 5, 1024); break;
+            }
+        } else if (R <= 32) {
+            switch (tune_value(kTuneDrawF64R32)) {
+                case 1: DRAW_LAUNCH(double, 16, 17, 1024); break;
+                case 2: DRAW_LAUNCH(double, 64, 5, 1024); break;
+                case 3: DRAW_LAUNCH(double, 32, 9, 256); break;
+                default: DRAW_LAUNCH(double, 32, 9, 1024); break;
+            }
+        } else if (R <= 45) {
+            DRAW_LAUNCH(double, 64, 9, 1024);
+        } else {
+            DRAW_LAUNCH(double, 64, 17, 256);
+        }
     } else {
-        if (R <= 16) DRAW_LAUNCH(float, 16, 3, 1024);
-        else if (R <= 32) DRAW_LAUNCH(float, 8, 17, 1024);
-        else if (R <= 45) DRAW_LAUNCH(float, 32, 9, 1024);
-        else DRAW_LAUNCH(float, 64, 9, 1024);
+        if (R <= 16) {
+            switch (tune_value(kTuneDrawF32R16)) {
+                case 1: DRAW_LAUNCH(float, 8, 5, 1024); break;
+                case 2: DRAW_LAUNCH(float, 32, 2, 1024); break;
+                default: DRAW_LAUNCH(float, 16, 3, 1024); break;
+            }
+        } else if (R <= 32) {
+            switch (tune_value(kTuneDrawF32R32)) {
+                case 1: DRAW_LAUNCH(float, 16, 9, 1024); break;
+                case 2: DRAW_LAUNCH(float, 32, 5, 1024); break;
+                case 3: DRAW_LAUNCH(float, 8, 17, 256); break;
+                default: DRAW_LAUNCH(float, 8, 17, 1024); break;
+            }
+        } else if (R <= 45) {
+            DRAW_LAUNCH(float, 32, 9, 1024);
+        } else {
+            switch (tune_value(kTuneDrawF32R64)) {
+                case 1: DRAW_LAUNCH(float, 32, 17, 1024); break;
+                default: DRAW_LAUNCH(float, 64, 9, 1024); break;
+            }
+        }
     }
     EX_HIP(hipGetLastError());
 }

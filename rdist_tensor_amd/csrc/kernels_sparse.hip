@@ -1850,6 +1850,15 @@ int group_lanes(int R) {
     return G;
 }
 
+// Lanes per hit of the sampled-MTTKRP row pass: group_lanes, doubled (at
+// most 32) when the autotuner chose the wider variant.
+template <typename V>
+int scan_lanes(int R) {
+    int G = group_lanes<V>(R);
+    if (tune_value(sizeof(V) == 8 ? kTuneScanF64 : kTuneScanF32) == 1 && G < 32) G *= 2;
+    return G;
+}
+
 // f(std::integral_constant<int, G>) for the runtime lane count G.
 template <typename F>
 void with_group(int G, F f) {
@@ -2041,7 +2050,7 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
                            L.fkey.as<uint64_t>(), L.nfib, L.cmap.as<uint64_t>(), L.filt.as<uint32_t>(), fsel);
         const dim3 scan_grid(grid_for(nch, kScanThreads / 64, num_cus(s)));   // one workgroup per CU (LDS)
         unsigned long long* part = count_parts(L, counts, scan_grid.x);
-        with_group(group_lanes<V>(R), [&](auto gc) {
+        with_group(scan_lanes<V>(R), [&](auto gc) {
             hipLaunchKernelGGL((row_scan_reduce<V, false, decltype(gc)::value>), scan_grid, dim3(kScanThreads), 0, s,
                                L.rrow.as<uint32_t>(), L.rfib.as<uint32_t>(), vals, L.nnz, nch,
                                L.cmap.as<uint64_t>(), L.filt.as<uint32_t>(), design, ExactSrc<V>{}, R, Y, vec, flag,

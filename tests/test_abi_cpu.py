@@ -102,3 +102,34 @@ def test_bench_refuses_multi_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_tuning_knobs_and_file(tmp_path):
+    """Launch-variant knobs: names and variant counts, set / load with
+    errors for unknown names, out-of-range values and malformed lines (host
+    only: no GPU call)."""
+    from rdist_tensor_amd import abi
+    t = abi.tuning()
+    assert {"sts_draw.fp64.r32", "sts_draw.fp32.r32", "row_scan.fp32"} <= set(t)
+    before = {k: v for k, (n, v) in t.items()}
+    try:
+        abi.tuning_set("sts_draw.fp32.r32", 2)
+        assert abi.tuning()["sts_draw.fp32.r32"][1] == 2
+        for bad in [("no_such_knob", 0), ("sts_draw.fp32.r32", 99), ("sts_draw.fp32.r32", -1)]:
+            with pytest.raises(abi.ExafacError) as e:
+                abi.tuning_set(*bad)
+            assert e.value.code == abi.EX_ERR_ARG
+        f = tmp_path / "tuning.txt"
+        f.write_text("# comment\nsts_draw.fp64.r32 3   # trailing comment\n\nrow_scan.fp64 1\n")
+        abi.tuning_load(f)
+        assert abi.tuning()["sts_draw.fp64.r32"][1] == 3 and abi.tuning()["row_scan.fp64"][1] == 1
+        for text in ["row_scan.fp64 7\n", "bogus 0\n", "row_scan.fp64\n"]:
+            f.write_text(text)
+            with pytest.raises(abi.ExafacError) as e:
+                abi.tuning_load(f)
+            assert e.value.code == abi.EX_ERR_ARG
+        with pytest.raises(abi.ExafacError):
+            abi.tuning_load(tmp_path / "missing.txt")
+    finally:
+        for k, v in before.items():
+            abi.tuning_set(k, v)
