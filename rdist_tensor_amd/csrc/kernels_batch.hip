@@ -41,14 +41,20 @@ static_assert(kBatchMaxJ < (1 << 16), "table entries pack a 16-bit index and a 1
 
 // offsets in the shared region (bytes) of the phases that run after the
 // deduplication
-constexpr int kOffStage = 0;                     // Gram staging: 64 rows x R doubles (16 KiB)
-constexpr int kOffPart = 16 << 10;               // Gram partials: 1024 doubles
+constexpr int kGramRows = 256;                   // rows per staged Gram chunk
+constexpr int kOffStage = 0;                     // Gram staging: kGramRows x R doubles (64 KiB)
+constexpr int kOffPart = 64 << 10;               // column-norm partials (FP64 update): 1024 doubles
 constexpr int kOffSolve = 24 << 10;              // R x R solver scratch (SpdShared + perm, or EighShared)
 constexpr int kOffGY = 80 << 10;                 // Y^T Y (R x R doubles)
 constexpr int kOffTmp = 88 << 10;                // R x R temporary
 static_assert(kOffSolve + (int)sizeof(SpdShared) + 128 <= kOffGY, "solver scratch overlaps Y^T Y");
 static_assert(kOffSolve + (int)sizeof(EighShared) <= kOffGY, "solver scratch overlaps Y^T Y");
 static_assert(kOffTmp + kRR * 8 <= kBigBytes, "shared region too small");
+static_assert(kOffStage + kGramRows * kBatchMaxRank * 8 <= kOffPart, "Gram staging overlaps its partials");
+static_assert(kOffPart + kRR * 8 <= kOffGY, "partials overlap Y^T Y");
+// block_gram's partials: 1024 / ceil(R/2)^2 groups of R^2 doubles, at most
+// 4 * R^2 * 8 (R even) or about that (R odd) -- within the staging area
+static_assert(4 * kRR * 8 <= kOffPart - kOffStage, "Gram partials do not fit the staging area");
 static_assert((4 << kTableBits) <= kBigBytes, "deduplication table does not fit");
 
 struct Lds {
@@ -273,7 +279,9 @@ __device__ uint32_t dedup_phase(const BatchTensor& T, int j, int64_t J, const Ld
 }
 
 // KRP gather (accumulator_stationary.hpp:210-249): dg = prod_{k != j}
-// U_k[s_k] sqrt(w), dm = dg sqrt(w), in V arithmetic.
+// U_k[s_k] sqrt(w), dm = dg sqrt(w), in V arithmetic.  A thread takes kLane
+// rows at a time so that their index loads, then their factor-row loads,
+// are in flight together.
 template <typename V>
 __device__ void design_phase(const BatchTensor& T, int j, uint32_t Ju) {
     const int N = T.N, R = T.R;
@@ -281,35 +289,55 @@ __device__ void design_phase(const BatchTensor& T, int j, uint32_t Ju) {
     V* dg = static_cast<V*>(T.dg);
     V* dm = static_cast<V*>(T.dm);
     if (rl < rows)
-        for (uint32_t c = rl; c < Ju; c += rows) {
-            V d = (V)1;
+        for (uint32_t c0 = rl; c0 < Ju; c0 += (uint32_t)rows * kLane) {
+            V d[kLane];
+            uint32_t idx[kLane];
+#pragma unroll
+            for (int b = 0; b < kLane; b++) d[b] = (V)1;
             for (int k = 0; k < N; k++) {
                 if (k == j) continue;
-                d *= static_cast<const V*>(T.m[k].U)[(int64_t)T.sd[(int64_t)c * N + k] * R + r];
+                const V* U = static_cast<const V*>(T.m[k].U);
+#pragma unroll
+                for (int b = 0; b < kLane; b++) {
+                    const uint32_t c = c0 + (uint32_t)b * rows;
+                    idx[b] = c < Ju ? T.sd[(int64_t)c * N + k] : 0u;
+                }
+#pragma unroll
+                for (int b = 0; b < kLane; b++) d[b] *= U[(int64_t)idx[b] * R + r];
             }
-            const V sw = (V)sqrt(T.wd[c]);
-            d *= sw;
-            dg[(int64_t)c * R + r] = d;
-            dm[(int64_t)c * R + r] = d * sw;
+#pragma unroll
+            for (int b = 0; b < kLane; b++) {
+                const uint32_t c = c0 + (uint32_t)b * rows;
+                if (c >= Ju) continue;
+                const V sw = (V)sqrt(T.wd[c]);
+                const V v = d[b] * sw;
+                dg[(int64_t)c * R + r] = v;
+                dm[(int64_t)c * R + r] = v * sw;
+            }
         }
     __syncthreads();
 }
 
 // G (R x R, LDS, both triangles) = sum over the rows of A (rows idx[0, rows)
 // when idx is given, else rows 0 .. rows-1) of a^T a with FP64 products and
-// sums: 64-row chunks staged in LDS; 1024 / R^2 thread groups take
-// interleaved rows of a chunk; their partials are added in group order.
+// sums: kGramRows-row chunks staged in LDS (each thread's loads of a chunk in
+// flight together); a thread owns a 2 x 2 block of G (four products for four
+// LDS reads per row) and 1024 / ceil(R/2)^2 thread groups take interleaved
+// rows of a chunk; their partials, written over the staging area once the
+// last chunk is done, are added in group order.  Entries (a, b) and (b, a)
+// are the same products summed in the same order, so G is exactly symmetric.
+// The solver's scratch overlaps the staging area: it is dead once G is formed.
 template <typename V>
 __device__ void block_gram(const V* __restrict__ A, const uint32_t* __restrict__ idx, uint32_t rows, int R,
                            double* G, const Lds& s) {
     double* stage = reinterpret_cast<double*>(s.big + kOffStage);
-    double* part = reinterpret_cast<double*>(s.big + kOffPart);
-    const int RR = R * R, groups = kBT / RR;
-    const int t = threadIdx.x, g = t / RR, e = t - g * RR, a = e / R, b = e - a * R;
+    const int H = (R + 1) / 2, HH = H * H, groups = kBT / HH;
+    const int t = threadIdx.x, g = t / HH, e = t - g * HH, a0 = 2 * (e / H), b0 = 2 * (e - (e / H) * H);
     const bool on = g < groups;
-    double acc = 0.0;
-    for (uint32_t r0 = 0; r0 < rows; r0 += 64) {
-        const int nr = rows - r0 < 64u ? (int)(rows - r0) : 64;
+    const bool a1 = a0 + 1 < R, b1 = b0 + 1 < R;
+    double acc00 = 0.0, acc01 = 0.0, acc10 = 0.0, acc11 = 0.0;
+    for (uint32_t r0 = 0; r0 < rows; r0 += kGramRows) {
+        const int nr = rows - r0 < (uint32_t)kGramRows ? (int)(rows - r0) : kGramRows;
         for (int q = t; q < nr * R; q += kBT) {
             const int rr = q / R;
             const int64_t row = idx ? (int64_t)idx[r0 + rr] : (int64_t)r0 + rr;
@@ -317,15 +345,31 @@ __device__ void block_gram(const V* __restrict__ A, const uint32_t* __restrict__
         }
         __syncthreads();
         if (on)
-            for (int r = g; r < nr; r += groups) acc += stage[r * R + a] * stage[r * R + b];
+            for (int r = g; r < nr; r += groups) {
+                const double* x = stage + r * R;
+                const double xa0 = x[a0], xa1 = a1 ? x[a0 + 1] : 0.0;
+                const double xb0 = x[b0], xb1 = b1 ? x[b0 + 1] : 0.0;
+                acc00 += xa0 * xb0;
+                acc01 += xa0 * xb1;
+                acc10 += xa1 * xb0;
+                acc11 += xa1 * xb1;
+            }
         __syncthreads();
     }
-    if (on) part[t] = acc;
+    double* part = stage;   // groups x R x R partials
+    const int RR = R * R;
+    if (on) {
+        double* pg = part + (int64_t)g * RR;
+        pg[a0 * R + b0] = acc00;
+        if (b1) pg[a0 * R + b0 + 1] = acc01;
+        if (a1) pg[(a0 + 1) * R + b0] = acc10;
+        if (a1 && b1) pg[(a0 + 1) * R + b0 + 1] = acc11;
+    }
     __syncthreads();
-    if (t < RR) {
+    for (int o = t; o < RR; o += kBT) {
         double sum = 0.0;
-        for (int q = 0; q < groups; q++) sum += part[q * RR + t];
-        G[t] = sum;
+        for (int q = 0; q < groups; q++) sum += part[(int64_t)q * RR + o];
+        G[o] = sum;
     }
     __syncthreads();
 }
@@ -441,25 +485,52 @@ __device__ uint32_t list_phase(const BatchTensor& T, int j, uint32_t stamp, cons
     return run;
 }
 
+// Lanes per row of the row-wise phases: a power of two >= R, so that a
+// row's lanes sit inside one wavefront.
+__device__ __forceinline__ int row_lanes(int R) {
+    int G = 1;
+    while (G < R) G <<= 1;
+    return G;
+}
+
 // Sampled MTTKRP, part 2 (sort_lookup.hpp:238-375): Y row of each listed row
-// = sum over its stamped positions, in row order, of x * dm[design row].
+// = sum over its stamped positions, in row order, of x * dm[design row].  The
+// row's lanes test G positions at a time (one coalesced load of stamps); the
+// hits among them, found by a ballot, are added in ascending position.
 template <typename V>
 __device__ void y_phase(const BatchTensor& T, int j, uint32_t L, uint32_t stamp) {
-    const int R = T.R;
-    const int rows = kBT / R, rl = threadIdx.x / R, r = threadIdx.x - rl * R;
+    const int R = T.R, G = row_lanes(R);
+    const int rows = kBT / G, rl = threadIdx.x / G, r = threadIdx.x - rl * G;
+    const int gbase = (threadIdx.x & 63) - r;
+    const uint64_t gmask = G >= 64 ? ~0ull : ((1ull << G) - 1ull);
     const BatchMode& M = T.m[j];
     const V* val = static_cast<const V*>(M.rval);
     const V* dm = static_cast<const V*>(T.dm);
     V* Y = static_cast<V*>(T.Yc);
-    if (rl < rows)
-        for (uint32_t q = rl; q < L; q += rows) {
+    for (uint32_t q0 = 0; q0 < L; q0 += rows) {
+        const uint32_t q = q0 + rl;
+        uint32_t p0 = 0, p1 = 0;
+        if (q < L) {
             const uint32_t row = M.list[q];
-            const uint32_t p1 = M.rowptr[row + 1];
-            V acc = (V)0;
-            for (uint32_t p = M.rowptr[row]; p < p1; p++)
-                if (T.hstamp[p] == stamp) acc += val[p] * dm[(int64_t)T.hu[p] * R + r];
-            Y[(int64_t)q * R + r] = acc;
+            p0 = M.rowptr[row];
+            p1 = M.rowptr[row + 1];
         }
+        V acc = (V)0;
+        for (uint32_t p = p0; p < p1; p += G) {
+            const uint32_t pp = p + r;
+            const bool hit = pp < p1 && T.hstamp[pp] == stamp;
+            uint64_t m = (__ballot(hit) >> gbase) & gmask;
+            while (m) {
+                const int b = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const uint32_t ph = p + (uint32_t)b;
+                const uint32_t u = T.hu[ph];
+                const V x = val[ph];
+                if (r < R) acc += x * dm[(int64_t)u * R + r];
+            }
+        }
+        if (q < L && r < R) Y[(int64_t)q * R + r] = acc;
+    }
     __syncthreads();
 }
 
@@ -559,28 +630,33 @@ __device__ void update_fp64_phase(const BatchTensor& T, int j, uint32_t L, const
 }
 
 // Sampler update (cp_arls_lev.hpp:25-34): Gp = pinv(U_j^T U_j) (in s.Gi),
-// score of each listed row u^T Gp u (FP64 from V rows), their sum and the
-// CDF of score / sum over the listed rows in ascending order (last entry 1).
+// score of each listed row u^T Gp u (FP64 from V rows; the row's lanes hold
+// one entry each, t_c = sum_b u_b Gp[b][c] over broadcasts, then a butterfly
+// sum of u_c t_c), their sum and the CDF of score / sum over the listed rows
+// in ascending order (last entry 1).
 template <typename V>
 __device__ void scores_phase(const BatchTensor& T, int j, uint32_t L, const Lds& s) {
-    const int R = T.R;
+    const int R = T.R, G = row_lanes(R);
+    const int rows = kBT / G, rl = threadIdx.x / G, r = threadIdx.x - rl * G;
+    const int gbase = (threadIdx.x & 63) - r;
     const BatchMode& M = T.m[j];
     const V* U = static_cast<const V*>(M.U);
     const double* g = s.Gi;
-    // thread t scores the rows [t P, t P + P) and sums them in order
+    const int rc = r < R ? r : 0;
+    for (uint32_t q0 = 0; q0 < L; q0 += rows) {
+        const uint32_t q = q0 + rl;
+        const double ac = (q < L && r < R) ? (double)U[(int64_t)M.list[q] * R + r] : 0.0;
+        double tc = 0.0;
+        for (int b = 0; b < R; b++) tc += __shfl(ac, gbase + b, 64) * g[b * R + rc];
+        double v = r < R ? ac * tc : 0.0;
+        for (int off = G >> 1; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (q < L && r == 0) M.csc[q] = v;
+    }
+    __syncthreads();
+    // thread t sums the scores [t P, t P + P) in order, then the workgroup
     const uint32_t P = (L + kBT - 1) / kBT;
     double part = 0.0;
-    for (uint32_t q = threadIdx.x * P, q1 = q + P < L ? q + P : L; q < q1; q++) {
-        const V* a = U + (int64_t)M.list[q] * R;
-        double r = 0.0;
-        for (int c = 0; c < R; c++) {
-            double tt = 0.0;
-            for (int b = 0; b < R; b++) tt += (double)a[b] * g[b * R + c];
-            r += (double)a[c] * tt;
-        }
-        M.csc[q] = r;
-        part += r;
-    }
+    for (uint32_t q = threadIdx.x * P, q1 = q + P < L ? q + P : L; q < q1; q++) part += M.csc[q];
     const double S = block_sum_f64(part, s.wsd);
     double carry = 0.0;
     for (uint32_t c0 = 0; c0 < L; c0 += kBT * kChunk) {
