@@ -9,179 +9,221 @@ namespace exafac {
 namespace {
 
 // ---------------------------------------------------------------- Jacobi
-// Round-robin ("circle method") pairing: position 0 fixed, the rest rotate.
-__device__ __forceinline__ void rr_pair(int n, int round, int i, int& a, int& b) {
-    int pa = i, pb = n - 1 - i;
-    a = pa == 0 ? 0 : ((pa - 1 + round) % (n - 1)) + 1;
-    b = pb == 0 ? 0 : ((pb - 1 + round) % (n - 1)) + 1;
+// Pair i (p < q) of round `round` of the round-robin ("circle method")
+// schedule over ne (even) indices: index 0 fixed, the others shifted by
+// `round`.  Both shifted positions are below 2 (ne - 1), so the modulo is a
+// conditional subtraction.
+__device__ __forceinline__ void rr_pq(int ne, int round, int i, int& p, int& q) {
+    const int m = ne - 1;
+    int a = 0;
+    if (i != 0) {
+        a = i - 1 + round;
+        a = (a >= m ? a - m : a) + 1;
+    }
+    int b = ne - 2 - i + round;
+    b = (b >= m ? b - m : b) + 1;
+    p = a < b ? a : b;
+    q = a < b ? b : a;
 }
 
 struct JacobiShared {
-    double c[kMaxRank / 2], s[kMaxRank / 2], t[kMaxRank / 2];
-    double apq[kMaxRank / 2], app[kMaxRank / 2], aqq[kMaxRank / 2];
-    int p[kMaxRank / 2], q[kMaxRank / 2], rot[kMaxRank / 2];
     int rank[kMaxRank];
-    int rotated;
+    int conv[2];   // per sweep (alternating): whether any pair rotated
 };
 
-__device__ __forceinline__ double symA(const double* A, int R, int a, int b) {
-    return a <= b ? A[tri_index(R, a, b)] : A[tri_index(R, b, a)];
+__device__ __forceinline__ int sym_index(int R, int a, int b) {
+    return a <= b ? tri_index(R, a, b) : tri_index(R, b, a);
 }
 
-// Jacobi runs on one workgroup of kJacobiThreads threads.
+// Jacobi runs on one workgroup of kJacobiThreads threads (one wavefront per
+// SIMD: the solver is bound by dependent FP64 latency, not by issue).
 constexpr int kJacobiThreads = 256;
 
-// New value of element (r, c), r < c, r in pair X, c in pair Y (X != Y):
-// rotation X applied to the rows, then rotation Y to the columns.  v[i][k] =
-// A[{p_X, q_X}[i]][{p_Y, q_Y}[k]]; ir / ic = position of r / c in its pair.
-__device__ __forceinline__ double rot_elem(int rX, double cX, double sX, int rY, double cY, double sY, int ir, int ic,
-                                           double v00, double v01, double v10, double v11) {
+// Rotation of pair (p, q) from its entries of the pre-round matrix: c > 0,
+// s = t c when it rotates; c = -1 when it does not (|a_pq| at most
+// eps sqrt(|a_pp a_qq|), or q the padding index n of an odd order).
+struct JacobiRot {
+    double c, s, t;
+};
+
+__device__ __forceinline__ JacobiRot jacobi_rot(double app, double aqq, double apq, bool valid) {
 #pragma clang fp contract(off)
-    double nr0, nr1;
-    if (rX) {
-        if (ir == 0) { nr0 = cX * v00 - sX * v10; nr1 = cX * v01 - sX * v11; }
-        else { nr0 = sX * v00 + cX * v10; nr1 = sX * v01 + cX * v11; }
-    } else {
-        nr0 = ir == 0 ? v00 : v10;
-        nr1 = ir == 0 ? v01 : v11;
+    JacobiRot r{-1.0, 0.0, 0.0};
+    const double thr = 2.220446049250313e-16 * sqrt(fabs(app) * fabs(aqq));
+    if (valid && apq != 0.0 && fabs(apq) > thr) {
+        const double tau = (aqq - app) / (2.0 * apq);
+        const double rt = sqrt(1.0 + tau * tau);
+        const double t = tau >= 0.0 ? 1.0 / (tau + rt) : -1.0 / (-tau + rt);
+        const double c = 1.0 / sqrt(1.0 + t * t);
+        r.c = c;
+        r.s = t * c;
+        r.t = t;
     }
-    if (!rY) return ic == 0 ? nr0 : nr1;
-    return ic == 0 ? cY * nr0 - sY * nr1 : sY * nr0 + cY * nr1;
+    return r;
+}
+
+// Rows of V each thread rotates: kJacobiThreads / 64 wavefronts, each with
+// 64 / npair row slots of npair lanes (lane l rotates columns of pair
+// l % npair, the pair whose rotation it holds).
+__host__ __device__ constexpr int jacobi_vstride(int npair) { return (kJacobiThreads / 64) * (64 / npair); }
+
+// Row i of the 2x2 product J^T w (J the rotation c, s of a pair; w[0], w[1]
+// the pair's p and q entries), as written out: c w0 - s w1 for i = 0, s w0 +
+// c w1 for i = 1, w_i unchanged without rotation.  (c w0 - s w1 = c w0 +
+// (-s) w1 exactly, so both rows are one product-sum with chosen factors.)
+__device__ __forceinline__ double rot_row(bool rot, double c, double s, int i, double w0, double w1) {
+#pragma clang fp contract(off)
+    const double m0 = i == 0 ? c : s, m1 = i == 0 ? -s : c;
+    return rot ? m0 * w0 + m1 * w1 : (i == 0 ? w0 : w1);
 }
 
 // Cyclic Jacobi on the packed upper triangle A (LDS) with eigenvectors V
-// (LDS, R x R row-major).  The same arithmetic as the oracle's orc_eigh:
-// rotation parameters from the pre-round matrix, each upper element rotated
-// rows-then-columns, the exact 2x2 diagonal, upper triangle authoritative.
-// Each thread owns NB fixed 2x2 blocks of pair indices (P <= Q) and NV fixed
-// (row, pair) items of V; the round-robin pair schedule is tabulated in LDS
-// once.  Afterwards sh.rank[v] is the ascending position of A[v][v] (ties by
-// index).
-template <int NB, int NV>
-__device__ void jacobi_block(double* A, double* V, int R, JacobiShared& sh, uint16_t* sched) {
+// (LDS, R x R row-major); A2 (LDS, packed upper) is the second buffer of the
+// matrix.  The same arithmetic as the oracle's orc_eigh: rotation parameters
+// from the pre-round matrix, each upper element rotated rows-then-columns,
+// the exact 2x2 diagonal, upper triangle authoritative.
+//
+// One barrier per round.  Lane l of each wavefront computes the rotation of
+// pair l % npair itself.  Each thread owns NBL fixed 2x2 blocks of pair
+// indices (P <= Q) and NV rows of V: it reads the blocks' entries and its V
+// entries before the rotations are known (the round's pairs follow from the
+// schedule), takes the rotations of P and Q from lanes P and Q by shuffles,
+// and writes the four post-round elements of each block to the other
+// buffer, every element through the same select-based sequence (both
+// rotation orders formed, the element's own order chosen) so that the
+// wavefront does not diverge.  The convergence flag of each sweep is
+// published by thread 0 in LDS; threads past kJacobiThreads only meet the
+// barriers.  Afterwards A holds the result and sh.rank[v] is the ascending
+// position of A[v][v] (ties by index).
+template <int NBL, int NV>
+__device__ void jacobi_block(double* A, double* A2, double* V, int R, JacobiShared& sh) {
 #pragma clang fp contract(off)
-    const int tid = threadIdx.x;
-    const bool act = tid < kJacobiThreads;   // threads beyond only meet the barriers
-    const int n = R, ne = n + (n & 1), npair = ne / 2;
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    const bool act = tid < kJacobiThreads;   // wavefront-uniform
+    const int n = R, ne = n + (n & 1), npair = ne / 2, m = ne - 1;
     const int nblocks = npair * (npair + 1) / 2;
-    for (int e = tid; act && e < (ne - 1) * npair; e += kJacobiThreads) {
-        int a, b;
-        rr_pair(ne, e / npair, e % npair, a, b);
-        sched[e] = (uint16_t)((a < b ? a : b) | ((a < b ? b : a) << 8));
-    }
-    int bP[NB], bQ[NB];
+    int bP[NBL], bQ[NBL];
 #pragma unroll
-    for (int t = 0; t < NB; t++) {
-        int bi = tid + kJacobiThreads * t, P = 0;
-        bP[t] = -1;
-        bQ[t] = -1;
+    for (int b = 0; b < NBL; b++) {
+        int bi = tid + b * kJacobiThreads, P = 0;
+        bP[b] = -1;
+        bQ[b] = 0;
         if (act && bi < nblocks) {
             while (bi >= npair - P) { bi -= npair - P; P++; }
-            bP[t] = P;
-            bQ[t] = P + bi;
+            bP[b] = P;
+            bQ[b] = P + bi;
         }
     }
-    int vk[NV], vi[NV];
-#pragma unroll
-    for (int u = 0; u < NV; u++) {
-        int it = tid + kJacobiThreads * u;
-        vk[u] = act && it < n * npair ? it / npair : -1;
-        vi[u] = it < n * npair ? it % npair : 0;
-    }
+    const int mine_pair = lane % npair;
+    const int vslots = 64 / npair, vstride = jacobi_vstride(npair);
+    const int vrow = act && lane < vslots * npair ? wave * vslots + lane / npair : n;
     for (int e = tid; act && e < n * n; e += kJacobiThreads) V[e] = (e / n == e % n) ? 1.0 : 0.0;
     __syncthreads();
-    for (int sweep = 0; sweep < 40; sweep++) {
-        if (tid == 0) sh.rotated = 0;
-        __syncthreads();
-        for (int round = 0; round < ne - 1; round++) {
-            if (tid < npair) {
-                const uint16_t pq = sched[round * npair + tid];
-                const int p = pq & 0xFF, q = pq >> 8;
-                sh.p[tid] = p;
-                sh.q[tid] = q;
-                int r = 0;
-                if (q < n) {
-                    double apq = A[tri_index(R, p, q)], app = A[tri_index(R, p, p)], aqq = A[tri_index(R, q, q)];
-                    double thr = 2.220446049250313e-16 * sqrt(fabs(app) * fabs(aqq));
-                    if (apq != 0.0 && fabs(apq) > thr) {
-                        double tau = (aqq - app) / (2.0 * apq);
-                        double rt = sqrt(1.0 + tau * tau);
-                        double t = tau >= 0.0 ? 1.0 / (tau + rt) : -1.0 / (-tau + rt);
-                        double c = 1.0 / sqrt(1.0 + t * t);
-                        sh.c[tid] = c;
-                        sh.s[tid] = t * c;
-                        sh.t[tid] = t;
-                        sh.apq[tid] = apq;
-                        sh.app[tid] = app;
-                        sh.aqq[tid] = aqq;
-                        r = 1;
-                        sh.rotated = 1;
-                    }
-                }
-                sh.rot[tid] = r;
+    double* cur = A;
+    double* nxt = A2;
+    bool any = false;
+    int sweep = 0, round = 0;
+    for (;;) {
+        if (act) {
+            // the round's pairs, and every read of the round, first
+            int pl, ql;
+            rr_pq(ne, round, mine_pair, pl, ql);
+            const int qlc = ql < n ? ql : pl;
+            int pa[NBL], qa[NBL], pb[NBL], qb[NBL];
+            double w[NBL][4];   // A[{pa, qa}[i]][{pb, qb}[k]] at 2 i + k
+#pragma unroll
+            for (int b = 0; b < NBL; b++) {
+                rr_pq(ne, round, bP[b] < 0 ? 0 : bP[b], pa[b], qa[b]);
+                rr_pq(ne, round, bQ[b], pb[b], qb[b]);
+                const int a1 = qa[b] < n ? qa[b] : pa[b], a2 = qb[b] < n ? qb[b] : pb[b];
+                w[b][0] = cur[sym_index(R, pa[b], pb[b])];
+                w[b][1] = cur[sym_index(R, pa[b], a2)];
+                w[b][2] = cur[sym_index(R, a1, pb[b])];
+                w[b][3] = cur[sym_index(R, a1, a2)];
             }
-            __syncthreads();
-            double nv[NB][4];
-            int ni[NB][4];
+            double vp[NV], vq[NV];
 #pragma unroll
-            for (int t = 0; t < NB; t++) {
+            for (int u = 0; u < NV; u++) {
+                const int kk = vrow + vstride * u < n ? vrow + vstride * u : 0;
+                vp[u] = V[kk * n + pl];
+                vq[u] = V[kk * n + qlc];
+            }
+            const JacobiRot mine = jacobi_rot(cur[tri_index(R, pl, pl)], cur[tri_index(R, qlc, qlc)],
+                                              cur[tri_index(R, pl, qlc)], ql < n);
+            any |= __ballot(mine.c > 0.0) != 0;
 #pragma unroll
-                for (int u = 0; u < 4; u++) ni[t][u] = -1;
-                const int P = bP[t], Q = bQ[t];
-                if (P < 0) continue;
-                // every parameter this block needs, loaded in one batch
-                const int rP = sh.rot[P], rQ = sh.rot[Q];
-                const int p1 = sh.p[P], q1 = sh.q[P], p2 = sh.p[Q], q2 = sh.q[Q];
-                const double cP = sh.c[P], sP = sh.s[P], cQ = sh.c[Q], sQ = sh.s[Q];
-                if (!rP && !rQ) continue;
-                if (P == Q) {
-                    const double tt = sh.t[P], apq = sh.apq[P], app = sh.app[P], aqq = sh.aqq[P];
-                    ni[t][0] = tri_index(R, p1, p1);
-                    nv[t][0] = app - tt * apq;
-                    ni[t][1] = tri_index(R, q1, q1);
-                    nv[t][1] = aqq + tt * apq;
-                    ni[t][2] = tri_index(R, p1, q1);
-                    nv[t][2] = 0.0;
-                    continue;
+            for (int b = 0; b < NBL; b++) {
+                const int P = bP[b] < 0 ? 0 : bP[b], Q = bQ[b];
+                const double cP = __shfl(mine.c, P, 64), sP = __shfl(mine.s, P, 64), tP = __shfl(mine.t, P, 64);
+                const double cQ = __shfl(mine.c, Q, 64), sQ = __shfl(mine.s, Q, 64);
+                const bool rP = cP > 0.0, rQ = cQ > 0.0;
+                const bool vqa = qa[b] < n, vqb = qb[b] < n;
+                double x[2][2];
+                x[0][0] = w[b][0];
+                x[0][1] = vqb ? w[b][1] : 0.0;
+                x[1][0] = vqa ? w[b][2] : 0.0;
+                x[1][1] = (vqa && vqb) ? w[b][3] : 0.0;
+                // fa: rows by P's rotation, then columns by Q's (element
+                // stored as a row of pair P); fb: rows by Q's, then columns
+                // by P's (stored as a row of pair Q)
+                double ra_[2][2], fa[2][2], rb_[2][2], fb[2][2];
+#pragma unroll
+                for (int k = 0; k < 2; k++) {
+                    ra_[0][k] = rot_row(rP, cP, sP, 0, x[0][k], x[1][k]);
+                    ra_[1][k] = rot_row(rP, cP, sP, 1, x[0][k], x[1][k]);
+                    rb_[0][k] = rot_row(rQ, cQ, sQ, 0, x[k][0], x[k][1]);
+                    rb_[1][k] = rot_row(rQ, cQ, sQ, 1, x[k][0], x[k][1]);
                 }
-                const bool vq1 = q1 < n, vq2 = q2 < n;
-                const double v00 = symA(A, R, p1, p2);
-                const double v01 = vq2 ? symA(A, R, p1, q2) : 0.0;
-                const double v10 = vq1 ? symA(A, R, q1, p2) : 0.0;
-                const double v11 = (vq1 && vq2) ? symA(A, R, q1, q2) : 0.0;
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    fa[i][0] = rot_row(rQ, cQ, sQ, 0, ra_[i][0], ra_[i][1]);
+                    fa[i][1] = rot_row(rQ, cQ, sQ, 1, ra_[i][0], ra_[i][1]);
+                    fb[i][0] = rot_row(rP, cP, sP, 0, rb_[i][0], rb_[i][1]);
+                    fb[i][1] = rot_row(rP, cP, sP, 1, rb_[i][0], rb_[i][1]);
+                }
+                // one pair (P == Q): (p, p), (p, q), (q, q) take the exact 2x2 result
+                const bool diag = P == Q;
+                const double d_pp = rP ? x[0][0] - tP * x[0][1] : x[0][0];
+                const double d_qq = rP ? x[1][1] + tP * x[0][1] : x[1][1];
+                const double d_pq = rP ? 0.0 : x[0][1];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const int ir = u >> 1, ic = u & 1;
-                    const int ra = ir ? q1 : p1, cb = ic ? q2 : p2;
-                    if (ra >= n || cb >= n) continue;
-                    double val;
-                    if (ra < cb) val = rot_elem(rP, cP, sP, rQ, cQ, sQ, ir, ic, v00, v01, v10, v11);
-                    else val = rot_elem(rQ, cQ, sQ, rP, cP, sP, ic, ir, v00, v10, v01, v11);
-                    ni[t][u] = ra < cb ? tri_index(R, ra, cb) : tri_index(R, cb, ra);
-                    nv[t][u] = val;
+                    const int ra = ir ? qa[b] : pa[b], cb = ic ? qb[b] : pb[b];
+                    const double off = ra < cb ? fa[ir][ic] : fb[ic][ir];
+                    const double dv = u == 0 ? d_pp : (u == 3 ? d_qq : d_pq);
+                    if (bP[b] >= 0 && ra < n && cb < n && !(diag && u == 2))
+                        nxt[sym_index(R, ra, cb)] = diag ? dv : off;
                 }
             }
+            if (mine.c > 0.0) {
 #pragma unroll
-            for (int u = 0; u < NV; u++) {
-                const int kk = vk[u], i = vi[u];
-                const int ri = sh.rot[i], p = sh.p[i], q = sh.q[i];
-                const double c = sh.c[i], s = sh.s[i];
-                if (kk < 0 || !ri) continue;
-                const double vp = V[kk * n + p], vq = V[kk * n + q];
-                V[kk * n + p] = c * vp - s * vq;
-                V[kk * n + q] = s * vp + c * vq;
+                for (int u = 0; u < NV; u++) {
+                    const int kk = vrow + vstride * u;
+                    if (kk < n) {
+                        V[kk * n + pl] = mine.c * vp[u] - mine.s * vq[u];
+                        V[kk * n + ql] = mine.s * vp[u] + mine.c * vq[u];
+                    }
+                }
             }
-            __syncthreads();
-#pragma unroll
-            for (int t = 0; t < NB; t++)
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-                    if (ni[t][u] >= 0) A[ni[t][u]] = nv[t][u];
-            __syncthreads();
         }
-        const int r = sh.rotated;
+        const bool last = round + 1 == m;
+        if (last && tid == 0) sh.conv[sweep & 1] = any ? 1 : 0;
         __syncthreads();
-        if (!r) break;
+        double* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        round = last ? 0 : round + 1;
+        if (last) {
+            // every thread reads the same flag: the exit is uniform
+            const int rotated = sh.conv[sweep & 1];
+            any = false;
+            if (!rotated || ++sweep == 40) break;
+        }
+    }
+    if (cur != A) {
+        for (int e = tid; e < tri_size(R); e += blockDim.x) A[e] = cur[e];
+        __syncthreads();
     }
     if (tid < n) {
         const double lv = A[tri_index(R, tid, tid)];
@@ -207,16 +249,19 @@ __device__ void load_upper(const double* M, int R, double* A) {
     __syncthreads();
 }
 
-struct EighShared {
-    double A[kMaxRank * (kMaxRank + 1) / 2];
-    double V[kMaxRank * kMaxRank];
+template <int RM>
+struct EighSharedT {
+    double A[RM * (RM + 1) / 2];
+    double A2[RM * (RM + 1) / 2];
+    double V[RM * RM];
     JacobiShared j;
-    uint16_t sched[(kMaxRank - 1) * (kMaxRank / 2)];
 };
+typedef EighSharedT<kMaxRank> EighShared;
 
 // compute_pinv_square: W = V diag(1/sqrt(lam)) for lam > tol else 0 (columns in
 // ascending eigenvalue order), out = W W^T summed over ascending columns.
-__device__ void pinv_from_jacobi(EighShared& sh, int R, double tol, double* __restrict__ out) {
+template <class SH>
+__device__ void pinv_from_jacobi(SH& sh, int R, double tol, double* __restrict__ out) {
 #pragma clang fp contract(off)
     // reuse V in place as W with sorted columns: build into A-sized temp? Use a
     // column-scale factor per original column, then sum in sorted order.

@@ -48,7 +48,8 @@ constexpr int kOffSolve = 24 << 10;              // R x R solver scratch (SpdSha
 constexpr int kOffGY = 80 << 10;                 // Y^T Y (R x R doubles)
 constexpr int kOffTmp = 88 << 10;                // R x R temporary
 static_assert(kOffSolve + (int)sizeof(SpdShared) + 128 <= kOffGY, "solver scratch overlaps Y^T Y");
-static_assert(kOffSolve + (int)sizeof(EighShared) <= kOffGY, "solver scratch overlaps Y^T Y");
+static_assert(kOffSolve + (int)sizeof(EighSharedT<kBatchMaxRank>) <= kOffGY, "solver scratch overlaps Y^T Y");
+static_assert(kBT >= kJacobiThreads, "the Jacobi solver needs kJacobiThreads threads");
 static_assert(kOffTmp + kRR * 8 <= kBigBytes, "shared region too small");
 static_assert(kOffStage + kGramRows * kBatchMaxRank * 8 <= kOffPart, "Gram staging overlaps its partials");
 static_assert(kOffPart + kRR * 8 <= kOffGY, "partials overlap Y^T Y");
@@ -395,10 +396,9 @@ __device__ void block_spd_pinv(const double* M, int R, double tol_rel, double* o
 // compute_pinv_square (common.h:405-443) as the FP64 path computes it: cyclic
 // Jacobi eigendecomposition, eigenvalues <= tol (absolute) dropped.
 __device__ void block_pinv_jacobi(const double* M, int R, double tol, double* out, const Lds& s) {
-    EighShared& sh = *reinterpret_cast<EighShared*>(s.big + kOffSolve);
+    EighSharedT<kBatchMaxRank>& sh = *reinterpret_cast<EighSharedT<kBatchMaxRank>*>(s.big + kOffSolve);
     load_upper(M, R, sh.A);
-    if (R <= 16) jacobi_block<1, 1>(sh.A, sh.V, R, sh.j, sh.sched);
-    else jacobi_block<1, 2>(sh.A, sh.V, R, sh.j, sh.sched);
+    jacobi_block<1, 2>(sh.A, sh.A2, sh.V, R, sh.j);
     pinv_from_jacobi(sh, R, tol, out);
     __syncthreads();
 }

@@ -95,13 +95,13 @@ __global__ __launch_bounds__(kThreads) void gram_mfma64_partial(const TA* __rest
 }
 
 // Eigendecomposition of blockIdx.x-th matrix: lam ascending, V columns.
-template <int NB, int NV>
+template <int NBL, int NV>
 __global__ __launch_bounds__(kJacobiThreads) void eigh_kernel(const double* __restrict__ Min, int R,
                                                         double* __restrict__ lam, double* __restrict__ Vout) {
     __shared__ EighShared sh;
     const int64_t R2 = (int64_t)R * R;
     load_upper(Min + blockIdx.x * R2, R, sh.A);
-    jacobi_block<NB, NV>(sh.A, sh.V, R, sh.j, sh.sched);
+    jacobi_block<NBL, NV>(sh.A, sh.A2, sh.V, R, sh.j);
     for (int o = threadIdx.x; o < R * R; o += blockDim.x) {
         int u = o / R, v = o % R;
         Vout[blockIdx.x * R2 + u * R + sh.j.rank[v]] = sh.V[u * R + v];
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kJacobiThreads) void eigh_kernel(const double* __re
     if (threadIdx.x < R) lam[blockIdx.x * R + sh.j.rank[threadIdx.x]] = sh.A[tri_index(R, threadIdx.x, threadIdx.x)];
 }
 
-template <int NB, int NV>
+template <int NBL, int NV>
 __global__ __launch_bounds__(kJacobiThreads) void pinv_kernel(const double* __restrict__ Min, int R, double tol,
                                                         double* __restrict__ out, const int* __restrict__ only_if,
                                                         int relative) {
@@ -130,14 +130,14 @@ __global__ __launch_bounds__(kJacobiThreads) void pinv_kernel(const double* __re
         __syncthreads();
         tol *= dmax;
     }
-    jacobi_block<NB, NV>(sh.A, sh.V, R, sh.j, sh.sched);
+    jacobi_block<NBL, NV>(sh.A, sh.A2, sh.V, R, sh.j);
     pinv_from_jacobi(sh, R, tol, out + blockIdx.x * R2);
 }
 
 // ---------------------------------------------------------------- STS computeM
 // sts_cp.hpp:80-188 for target mode j, one workgroup.  grams: N x R x R
 // (full symmetric), M: (N+2) x R x R, lam: (N+1) x R.
-template <int NB, int NV>
+template <int NBL, int NV>
 __global__ __launch_bounds__(kJacobiThreads) void sts_compute_m_kernel(int N, int j, int R, const double* __restrict__ grams,
                                                                  double* __restrict__ M, double* __restrict__ lam) {
 #pragma clang fp contract(off)
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kJacobiThreads) void sts_compute_m_kernel(int N, in
     __syncthreads();
     // eigendecomposition of the Hadamard product; slot N = V diag(lam^-1/2), columns reversed
     load_upper(M, R, sh.A);
-    jacobi_block<NB, NV>(sh.A, sh.V, R, sh.j, sh.sched);
+    jacobi_block<NBL, NV>(sh.A, sh.A2, sh.V, R, sh.j);
     if (tid < R) ord[sh.j.rank[tid]] = tid;
     __syncthreads();
     if (tid < R) lam[sh.j.rank[tid]] = sh.A[tri_index(R, tid, tid)];
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kJacobiThreads) void sts_compute_m_kernel(int N, in
         if (k == j) continue;
         if (k < N) {
             load_upper(M + k * R2, R, sh.A);
-            jacobi_block<NB, NV>(sh.A, sh.V, R, sh.j, sh.sched);
+            jacobi_block<NBL, NV>(sh.A, sh.A2, sh.V, R, sh.j);
             if (tid < R) ord[sh.j.rank[tid]] = tid;
             __syncthreads();
             if (tid < R) lam[k * R + sh.j.rank[tid]] = sh.A[tri_index(R, tid, tid)];
@@ -488,17 +488,12 @@ void col_sumsq_final_launch(hipStream_t s, const double* part, int nb, int R, do
     EX_HIP(hipGetLastError());
 }
 
-// Jacobi work per thread: (pair blocks, V items) = (1, 1) for R <= 16,
-// (1, 2) for R <= 32, (3, 8) for R <= 64.
-int jacobi_nb(int R) { return R <= 16 ? 1 : (R <= 32 ? 2 : 3); }
-
+// Jacobi work per thread: (2x2 blocks, V rows) = (1, 2) for R <= 32, (3, 8)
+// for R <= 64.
 #define JACOBI_DISPATCH(R, KERNEL, GRID, STREAM, ...)                                                     \
     do {                                                                                                  \
-        switch (jacobi_nb(R)) {                                                                           \
-            case 1: hipLaunchKernelGGL((KERNEL<1, 1>), GRID, dim3(kJacobiThreads), 0, STREAM, __VA_ARGS__); break; \
-            case 2: hipLaunchKernelGGL((KERNEL<1, 2>), GRID, dim3(kJacobiThreads), 0, STREAM, __VA_ARGS__); break; \
-            default: hipLaunchKernelGGL((KERNEL<3, 8>), GRID, dim3(kJacobiThreads), 0, STREAM, __VA_ARGS__); break; \
-        }                                                                                                 \
+        if ((R) <= 32) hipLaunchKernelGGL((KERNEL<1, 2>), GRID, dim3(kJacobiThreads), 0, STREAM, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERNEL<3, 8>), GRID, dim3(kJacobiThreads), 0, STREAM, __VA_ARGS__);        \
     } while (0)
 
 void eigh(hipStream_t s, const double* A, int R, double* lam, double* V, int ncopies) {

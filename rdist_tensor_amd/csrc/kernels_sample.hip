@@ -185,13 +185,16 @@ template <> struct Vec<double> {
 };
 
 // LDS of one draw block: the packed-upper (a, b) table, and per sample the
-// running Hadamard product h, the current query and the leaf's q^2.
-template <typename TN, int SLOTS, int RMAX>
+// running Hadamard product h, the current query and the leaf's q^2, in rows
+// of RP >= RMAX elements.  With RP = RMAX + 1 the groups of a wavefront,
+// which read the same element of their own rows together, fall on different
+// banks; with RP = RMAX (a power of two) every other group shares banks.
+template <typename TN, int SLOTS, int RMAX, int RP>
 struct DrawShared {
     uint16_t pairs[RMAX * (RMAX + 1) / 2];
-    TN h[SLOTS][RMAX];
-    TN x[SLOTS][RMAX];
-    double q2[SLOTS][RMAX];
+    TN h[SLOTS][RP];
+    TN x[SLOTS][RP];
+    double q2[SLOTS][RP];
 };
 
 // Largest rank whose packed triangle fits the NV W-wide vectors of G lanes
@@ -403,6 +406,10 @@ constexpr int stage_levels(int nodes) {
     while (((2 << sl) - 1) <= nodes / 2) sl++;
     return sl;
 }
+// Staged nodes per tree when the per-sample state takes `shared` bytes.
+constexpr int draw_nstage(size_t shared, int NT, int stride, int bytes) {
+    return (1 << stage_levels(shared < draw_lds(NT) ? (int)((draw_lds(NT) - shared) / (stride * bytes)) : 0)) - 1;
+}
 template <typename TN, int G, int NV, int NT>
 struct DrawCfg {
     static constexpr int W = Vec<TN>::W;
@@ -410,10 +417,14 @@ struct DrawCfg {
     static constexpr int RMAX = draw_rmax(G, NV, W);
     static constexpr int SLOTS = (NT / 64) * SPW;
     static constexpr int STRIDE = cx_stride(RMAX, (int)sizeof(TN));
-    static constexpr size_t SHARED = sizeof(DrawShared<TN, SLOTS, RMAX>);
-    static constexpr int BUDGET =
-        SHARED < draw_lds(NT) ? (int)((draw_lds(NT) - SHARED) / (STRIDE * sizeof(TN))) : 0;
-    static constexpr int NSTAGE = (1 << stage_levels(BUDGET)) - 1;   // staged nodes per tree
+    // padded rows unless the padding would cost a staged tree level
+    static constexpr int RP =
+        draw_nstage(sizeof(DrawShared<TN, SLOTS, RMAX, RMAX + 1>), NT, STRIDE, (int)sizeof(TN)) ==
+                draw_nstage(sizeof(DrawShared<TN, SLOTS, RMAX, RMAX>), NT, STRIDE, (int)sizeof(TN))
+            ? RMAX + 1
+            : RMAX;
+    typedef DrawShared<TN, SLOTS, RMAX, RP> Shared;
+    static constexpr int NSTAGE = draw_nstage(sizeof(Shared), NT, STRIDE, (int)sizeof(TN));
 };
 
 // STS-CP KRPDrawSamples (sts_cp.hpp:228-286): one sample per group of G
@@ -431,7 +442,7 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
     constexpr int SPW = C::SPW;
     constexpr int RMAX = C::RMAX;
     constexpr int NST = C::NSTAGE;
-    __shared__ DrawShared<TN, C::SLOTS, RMAX> sh;
+    __shared__ typename C::Shared sh;
     __shared__ __attribute__((aligned(16))) TN stage[2][NST > 0 ? NST * C::STRIDE : 1];
     typedef typename Vec<TN>::type VT;
     const int R = a.R, N = a.N, j = a.j;
