@@ -123,7 +123,10 @@ int ex_tensor_mttkrp_path(ex_tensor* t, int path);
  * data_ptr()); factors and grads are host arrays of N such pointers
  * (I_k x R row-major FP64).  samples: Ju x N deduplicated tuples (column
  * `mode` ignored); weights: Ju.  Both calls wait for their work on the
- * context stream before returning.
+ * context stream before returning.  Deduplication is checked on the device
+ * (a hashed count of the fiber keys): when two rows share every non-target
+ * coordinate the outputs are filled with NaN and the call returns
+ * EX_ERR_ARG.
  * Forward: Y (I_mode x R, overwritten) = sum over the nonzeros x of each
  * sampled fiber c of x * (prod_{k != mode} U_k[s_c,k] * sqrt(w_c)) * sqrt(w_c).
  * Backward: given G = dL/dY (I_mode x R), adds dL/dU_k into grads[k] for
@@ -134,6 +137,20 @@ int ex_smttkrp_forward(ex_tensor* t, int mode, int64_t Ju, const uint32_t* sampl
                        const double* const* factors, int64_t R, double* Y);
 int ex_smttkrp_backward(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samples, const double* weights,
                         const double* const* factors, int64_t R, const double* G, double* const* grads);
+/* Stream-ordered variants (no reference counterpart; for callers that keep
+ * their work on their own stream, e.g. torch's current stream, with no
+ * device-wide synchronisation): the same computation enqueued on `stream`
+ * (a hipStream_t of the context's device; NULL = the null stream, which is
+ * torch's default stream) after the work already there, returning without
+ * waiting.  Duplicate samples
+ * are not reported by the return code: the outputs come out NaN.  The
+ * temporaries live in the tensor (grown on demand, which drains the device
+ * once): calls on one tensor must be ordered, e.g. on one stream. */
+int ex_smttkrp_forward_async(ex_tensor* t, void* stream, int mode, int64_t Ju, const uint32_t* samples,
+                             const double* weights, const double* const* factors, int64_t R, double* Y);
+int ex_smttkrp_backward_async(ex_tensor* t, void* stream, int mode, int64_t Ju, const uint32_t* samples,
+                              const double* weights, const double* const* factors, int64_t R, const double* G,
+                              double* const* grads);
 
 /* ---- factors (LowRankTensor) ----------------------------------------- */
 /* LowRankTensor(R, TensorGrid&) -- low_rank_tensor.hpp:16-25: N factor

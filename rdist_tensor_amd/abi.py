@@ -72,6 +72,8 @@ SIGNATURES = {
     "ex_tuning_load": [C.c_char_p],
     "ex_smttkrp_forward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp],
     "ex_smttkrp_backward": [_vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp, _vp],
+    "ex_smttkrp_forward_async": [_vp, _vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp],
+    "ex_smttkrp_backward_async": [_vp, _vp, C.c_int, C.c_int64, _vp, _vp, _vp, C.c_int64, _vp, _vp],
     "ex_als_destroy": [_vp],
     "ex_als_init_ground_truth": [_vp],
     "ex_als_step": [_vp, C.c_int, C.c_int64, C.c_uint64],

@@ -1,8 +1,10 @@
 """Differentiable sampled MTTKRP for first-order CP fitting.
 
 `SampledMTTKRP` is a torch.autograd.Function over the C-ABI entry points
-ex_smttkrp_forward / ex_smttkrp_backward (HIP kernels; PyTorch supplies the
-device memory and nothing else).  For a target mode j, deduplicated sample
+ex_smttkrp_forward_async / ex_smttkrp_backward_async (HIP kernels enqueued
+on torch's current stream, nothing waited for; PyTorch supplies the device
+memory and the stream).  Duplicate sample tuples are caught on the device
+and turn the outputs into NaN.  For a target mode j, deduplicated sample
 tuples s_c (c < J_u) with weights w_c and factors U_0 .. U_{N-1}:
 
     D_c = (prod_{k != j} U_k[s_c,k] * sqrt(w_c)) * sqrt(w_c)        (design row)
@@ -64,9 +66,11 @@ class SampledMTTKRP(torch.autograd.Function):
         weights = weights.contiguous()
         U = [f.detach().contiguous() for f in factors]
         Y = torch.empty((int(tensor.dims[mode]), R), dtype=torch.float64, device=samples.device)
-        torch.cuda.synchronize(samples.device)   # the library runs on its own stream
-        abi.check(abi.lib().ex_smttkrp_forward(tensor.h, mode, samples.shape[0], samples.data_ptr(),
-                                               weights.data_ptr(), _ptr_array(U), R, Y.data_ptr()))
+        # enqueued on torch's current stream, after the work that produced
+        # the inputs; nothing is waited for (duplicate samples give NaN)
+        stream = torch.cuda.current_stream(samples.device).cuda_stream
+        abi.check(abi.lib().ex_smttkrp_forward_async(tensor.h, stream, mode, samples.shape[0], samples.data_ptr(),
+                                                     weights.data_ptr(), _ptr_array(U), R, Y.data_ptr()))
         ctx.tensor = tensor
         ctx.mode = mode
         ctx.save_for_backward(samples, weights, *U)
@@ -81,10 +85,10 @@ class SampledMTTKRP(torch.autograd.Function):
         grads = [torch.zeros_like(u) if (k != mode and ctx.needs_input_grad[4 + k]) else None
                  for k, u in enumerate(U)]
         if any(g is not None for g in grads):
-            torch.cuda.synchronize(G.device)
-            abi.check(abi.lib().ex_smttkrp_backward(ctx.tensor.h, mode, samples.shape[0], samples.data_ptr(),
-                                                    weights.data_ptr(), _ptr_array(U), R, G.data_ptr(),
-                                                    _ptr_array(grads)))
+            stream = torch.cuda.current_stream(G.device).cuda_stream
+            abi.check(abi.lib().ex_smttkrp_backward_async(ctx.tensor.h, stream, mode, samples.shape[0],
+                                                          samples.data_ptr(), weights.data_ptr(), _ptr_array(U), R,
+                                                          G.data_ptr(), _ptr_array(grads)))
         return (None, None, None, None, *grads)
 
 

@@ -268,6 +268,14 @@ struct Span {
     }
 };
 
+// Temporaries of the differentiable sampled MTTKRP (grown on demand, kept
+// with the tensor so that a call enqueued on a caller's stream allocates
+// nothing once they are large enough).
+struct AdjScratch {
+    DevBuf kd, sd, wd, dg, dm, dD, table, work, cnt;
+    DevBuf tmp[4];
+};
+
 struct ex_tensor {
     ex_ctx* ctx = nullptr;
     int N = 0;
@@ -279,6 +287,7 @@ struct ex_tensor {
     bool built = false;
     ModeLookup lk[kMaxModes];
     DevBuf tmp[4];
+    AdjScratch adj;
 };
 
 struct ex_factors {
@@ -922,16 +931,67 @@ double als_fit_impl(ex_als* a) {
 
 // Keys of device sample tuples for `mode`, checked distinct (the sampled
 // MTTKRP sums each fiber once per sample row).
-void smttkrp_keys(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samples, DevBuf& kd) {
-    hipStream_t st = t->ctx->c.stream;
-    const ModeLookup& L = t->lk[mode];
-    kd.ensure((size_t)(Ju > 0 ? Ju : 1) * 8);
-    k::keys_from_coords(st, samples, Ju, L.kl, kd.as<uint64_t>(), nullptr);
-    std::vector<uint64_t> keys((size_t)Ju);
-    d2h(st, keys.data(), kd.p, (size_t)Ju * 8);
-    std::sort(keys.begin(), keys.end());
-    EX_CHECK(std::adjacent_find(keys.begin(), keys.end()) == keys.end(), EX_ERR_ARG,
-             "samples must be deduplicated: two rows share every non-target coordinate");
+// The samples' fiber keys in their order (t->adj.kd) and, at t->adj.cnt,
+// the device count of distinct keys, which equals Ju exactly when the
+// samples are deduplicated; enqueued on st.
+void smttkrp_keys(ex_tensor* t, hipStream_t st, int mode, int64_t Ju, const uint32_t* samples,
+                  const double* weights) {
+    AdjScratch& a = t->adj;
+    const KeyLayout& kl = t->lk[mode].kl;
+    const size_t J1 = (size_t)(Ju > 0 ? Ju : 1);
+    a.kd.ensure(J1 * 8);
+    a.sd.ensure(J1 * t->N * 4);
+    a.wd.ensure(J1 * 8);
+    a.cnt.ensure(32);
+    uint64_t* cnt = a.cnt.as<uint64_t>();
+    if (kl.bits <= 63)
+        k::dedup_hash(st, samples, weights, Ju, t->N, kl, a.sd.as<uint32_t>(), a.wd.as<double>(), a.kd.as<uint64_t>(),
+                      a.table, a.work, cnt, nullptr);
+    else
+        k::dedup(st, samples, weights, Ju, t->N, kl, a.sd.as<uint32_t>(), a.wd.as<double>(), a.kd.as<uint64_t>(),
+                 a.tmp, cnt, nullptr);
+}
+
+// Forward on stream st (nothing waited for); Y is NaN when two samples share
+// a fiber key.
+void smttkrp_forward_on(ex_tensor* t, hipStream_t st, int mode, int64_t Ju, const uint32_t* samples,
+                        const double* weights, const double* const* factors, int R, double* Y) {
+    AdjScratch& a = t->adj;
+    ModeLookup& L = t->lk[mode];
+    const int64_t Ij = t->dims[mode];
+    const size_t J1 = (size_t)(Ju > 0 ? Ju : 1);
+    if (Ju > 0) smttkrp_keys(t, st, mode, Ju, samples, weights);
+    a.dg.ensure(J1 * R * 8);
+    a.dm.ensure(J1 * R * 8);
+    k::design(st, samples, weights, Ju, t->N, mode, R, factors, a.dg.as<double>(), a.dm.as<double>());
+    k::sampled_mttkrp<double>(st, L, L.rval.as<double>(), a.kd.as<uint64_t>(), Ju, a.dm.as<double>(), R, Y, Ij,
+                              a.tmp);
+    if (Ju > 0) k::poison_unless(st, a.cnt.as<uint64_t>(), Ju, Y, Ij * R);
+}
+
+// Backward on stream st (nothing waited for); every requested gradient is NaN
+// when two samples share a fiber key.
+void smttkrp_backward_on(ex_tensor* t, hipStream_t st, int mode, int64_t Ju, const uint32_t* samples,
+                         const double* weights, const double* const* factors, int R, const double* G,
+                         double* const* grads) {
+    AdjScratch& a = t->adj;
+    if (Ju == 0) return;
+    smttkrp_keys(t, st, mode, Ju, samples, weights);
+    a.dD.ensure((size_t)Ju * R * 8);
+    double* gp[kMaxModes] = {};
+    for (int k2 = 0; k2 < t->N; k2++) gp[k2] = k2 == mode ? nullptr : grads[k2];
+    k::smttkrp_adjoint(st, t->lk[mode], a.kd.as<uint64_t>(), samples, weights, Ju, t->N, mode, R, factors, G, gp,
+                       a.dD.as<double>());
+    for (int k2 = 0; k2 < t->N; k2++)
+        if (gp[k2]) k::poison_unless(st, a.cnt.as<uint64_t>(), Ju, gp[k2], t->dims[k2] * R);
+}
+
+// Whether the last smttkrp_keys found the samples deduplicated (waits for st).
+bool smttkrp_distinct(ex_tensor* t, hipStream_t st, int64_t Ju) {
+    if (Ju == 0) return true;
+    uint64_t c = 0;
+    d2h(st, &c, t->adj.cnt.p, 8);
+    return (int64_t)c == Ju;
 }
 
 void smttkrp_check(ex_tensor* t, int mode, int64_t Ju, const void* samples, const void* weights,
@@ -1158,17 +1218,9 @@ int ex_smttkrp_forward(ex_tensor* t, int mode, int64_t Ju, const uint32_t* sampl
         EX_HIP(hipSetDevice(t->ctx->c.device));
         tensor_build_lookups(t);
         hipStream_t st = t->ctx->c.stream;
-        ModeLookup& L = t->lk[mode];
-        const int64_t Ij = t->dims[mode];
-        DevBuf kd, dg, dm;
-        DevBuf tmp[4];
-        smttkrp_keys(t, mode, Ju, samples, kd);
-        dg.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
-        dm.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
-        k::design(st, samples, weights, Ju, t->N, mode, (int)R, factors, dg.as<double>(), dm.as<double>());
-        k::sampled_mttkrp<double>(st, L, L.rval.as<double>(), kd.as<uint64_t>(), Ju, dm.as<double>(), (int)R, Y,
-                                  Ij, tmp);
-        EX_HIP(hipStreamSynchronize(st));
+        smttkrp_forward_on(t, st, mode, Ju, samples, weights, factors, (int)R, Y);
+        EX_CHECK(smttkrp_distinct(t, st, Ju), EX_ERR_ARG,
+                 "samples must be deduplicated: two rows share every non-target coordinate");
     });
 }
 
@@ -1180,14 +1232,37 @@ int ex_smttkrp_backward(ex_tensor* t, int mode, int64_t Ju, const uint32_t* samp
         EX_HIP(hipSetDevice(t->ctx->c.device));
         tensor_build_lookups(t);
         hipStream_t st = t->ctx->c.stream;
-        DevBuf kd, dD;
-        smttkrp_keys(t, mode, Ju, samples, kd);
-        dD.ensure((size_t)(Ju > 0 ? Ju : 1) * R * 8);
-        double* gp[kMaxModes] = {};
-        for (int k2 = 0; k2 < t->N; k2++) gp[k2] = k2 == mode ? nullptr : grads[k2];
-        k::smttkrp_adjoint(st, t->lk[mode], kd.as<uint64_t>(), samples, weights, Ju, t->N, mode, (int)R, factors, G,
-                           gp, dD.as<double>());
+        smttkrp_backward_on(t, st, mode, Ju, samples, weights, factors, (int)R, G, grads);
+        EX_CHECK(smttkrp_distinct(t, st, Ju), EX_ERR_ARG,
+                 "samples must be deduplicated: two rows share every non-target coordinate");
         EX_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int ex_smttkrp_forward_async(ex_tensor* t, void* stream, int mode, int64_t Ju, const uint32_t* samples,
+                             const double* weights, const double* const* factors, int64_t R, double* Y) {
+    return guarded([&] {
+        smttkrp_check(t, mode, Ju, samples, weights, factors, R);
+        EX_CHECK(Y, EX_ERR_ARG, "null output");
+        EX_HIP(hipSetDevice(t->ctx->c.device));
+        tensor_build_lookups(t);
+        hipStream_t st = static_cast<hipStream_t>(stream);   // NULL: the null stream
+        smttkrp_forward_on(t, st, mode, Ju, samples, weights, factors, (int)R, Y);
+        EX_HIP(hipGetLastError());
+    });
+}
+
+int ex_smttkrp_backward_async(ex_tensor* t, void* stream, int mode, int64_t Ju, const uint32_t* samples,
+                              const double* weights, const double* const* factors, int64_t R, const double* G,
+                              double* const* grads) {
+    return guarded([&] {
+        smttkrp_check(t, mode, Ju, samples, weights, factors, R);
+        EX_CHECK(G && grads, EX_ERR_ARG, "null gradient argument");
+        EX_HIP(hipSetDevice(t->ctx->c.device));
+        tensor_build_lookups(t);
+        hipStream_t st = static_cast<hipStream_t>(stream);   // NULL: the null stream
+        smttkrp_backward_on(t, st, mode, Ju, samples, weights, factors, (int)R, G, grads);
+        EX_HIP(hipGetLastError());
     });
 }
 
@@ -1546,6 +1621,18 @@ int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, doubl
             graphs = graphs && als[t]->use_graph && !als[t]->exact;
         }
         auto t0 = std::chrono::steady_clock::now();
+        // with graphs, every optimizer's first sweep runs here, one optimizer
+        // after the other (for one without a graph at this J: an eager sweep,
+        // then the capture of one), so no capture overlaps another thread's
+        // launches; the threads below only replay graphs
+        int64_t first = 1;
+        if (graphs && rounds >= 1) {
+            for (int t = 0; t < n; t++) {
+                EX_HIP(hipSetDevice(als[t]->ctx->c.device));
+                als_sweep(als[t], J, 1);
+            }
+            first = 2;
+        }
         // host threads enqueue the optimizers' mode steps (optimizer t on
         // thread t mod T), interleaved across each thread's optimizers: every
         // context's stream carries its own chain and the streams run side by
@@ -1556,7 +1643,7 @@ int ex_als_run_batch(ex_als* const* als, int n, int64_t rounds, int64_t J, doubl
         for (int th = 0; th < T; th++)
             pool.emplace_back([&, th] {
                 try {
-                    for (int64_t round = 1; round <= rounds; round++) {
+                    for (int64_t round = first; round <= rounds; round++) {
                         if (graphs) {
                             // one launch of each optimizer's captured sweep
                             for (int t = th; t < n; t += T) {

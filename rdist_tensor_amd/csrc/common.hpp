@@ -243,6 +243,9 @@ void exact_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const V* const* U
 void smttkrp_adjoint(hipStream_t s, const ModeLookup& L, const uint64_t* kd, const uint32_t* sd, const double* wd,
                      int64_t Ju, int N, int j, int R, const double* const* U, const double* G,
                      double* const* grads, double* dD);
+// out[0..n) = NaN unless *count == want: a device-side argument check whose
+// failure the caller sees in the output, with nothing waited for.
+void poison_unless(hipStream_t s, const uint64_t* count, int64_t want, double* out, int64_t n);
 void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
              const double* sigma, int R, double* out, DevBuf& part);
 // final stages shared with the mixed-precision kernels

@@ -1528,6 +1528,14 @@ int64_t num_cus(hipStream_t s) {
     return n > 0 ? n : 1;
 }
 
+__global__ __launch_bounds__(kThreads) void poison_unless_kernel(const uint64_t* __restrict__ count, uint64_t want,
+                                                                 double* __restrict__ out, int64_t n) {
+    if (*count == want) return;
+    const double nan = __longlong_as_double(0x7ff8000000000000ll);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = nan;
+}
+
 uint64_t read_u64(hipStream_t s, const uint64_t* d) {
     uint64_t v = 0;
     EX_HIP(hipMemcpyAsync(&v, d, sizeof(v), hipMemcpyDeviceToHost, s));
@@ -1770,6 +1778,13 @@ __global__ __launch_bounds__(kThreads) void scatter_adjoint(const uint32_t* __re
             atomicAdd(g.G[k] + (int64_t)sd[c * a.N + k] * a.R + r, base * (prod * sw * sw));
         }
     }
+}
+
+void poison_unless(hipStream_t s, const uint64_t* count, int64_t want, double* out, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(poison_unless_kernel, dim3(grid_for(n, kThreads, 4096)), dim3(kThreads), 0, s, count,
+                       (uint64_t)want, out, n);
+    EX_HIP(hipGetLastError());
 }
 
 void smttkrp_adjoint(hipStream_t s, const ModeLookup& L, const uint64_t* kd, const uint32_t* sd, const double* wd,

@@ -116,6 +116,86 @@ def test_backward_matches_dense_gradient(ctx, orc, N, R):
             assert err < 1e-12
 
 
+def test_stream_ordered_entries_match_synchronous(ctx, orc):
+    """ex_smttkrp_forward_async / _backward_async on a side torch stream (the
+    autograd path, nothing waited for) give the synchronous entries' Y bitwise
+    and their gradients within 1e-12 (atomic adds in the adjoint), over three
+    modes in a row on one tensor (the temporaries it keeps are reused)."""
+    import torch
+    from rdist_tensor_amd import abi
+    from rdist_tensor_amd.autograd import _ptr_array
+    N, R = 3, 16
+    rng, dims, idx, vals, T, U, Ut = setup_case(ctx, orc, N, R, 77)
+    side = torch.cuda.Stream()
+    for mode in range(N):
+        sd, wd = make_samples(orc, rng, idx, dims, mode, 3000, 1000)
+        sdt = torch.tensor(sd.astype(np.int32), device="cuda")
+        wdt = torch.tensor(wd, device="cuda")
+        G = torch.tensor(rng.standard_normal((dims[mode], R)), device="cuda")
+        Ys = torch.empty((dims[mode], R), dtype=torch.float64, device="cuda")
+        gs = [torch.zeros_like(u) if k != mode else None for k, u in enumerate(Ut)]
+        torch.cuda.synchronize()
+        abi.check(abi.lib().ex_smttkrp_forward(T.h, mode, len(sd), sdt.data_ptr(), wdt.data_ptr(), _ptr_array(Ut),
+                                               R, Ys.data_ptr()))
+        abi.check(abi.lib().ex_smttkrp_backward(T.h, mode, len(sd), sdt.data_ptr(), wdt.data_ptr(), _ptr_array(Ut),
+                                                R, G.data_ptr(), _ptr_array(gs)))
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            Ya = torch.empty((dims[mode], R), dtype=torch.float64, device="cuda")
+            ga = [torch.zeros_like(u) if k != mode else None for k, u in enumerate(Ut)]
+            abi.check(abi.lib().ex_smttkrp_forward_async(T.h, side.cuda_stream, mode, len(sd), sdt.data_ptr(),
+                                                         wdt.data_ptr(), _ptr_array(Ut), R, Ya.data_ptr()))
+            abi.check(abi.lib().ex_smttkrp_backward_async(T.h, side.cuda_stream, mode, len(sd), sdt.data_ptr(),
+                                                          wdt.data_ptr(), _ptr_array(Ut), R, G.data_ptr(),
+                                                          _ptr_array(ga)))
+            Yh = Ya.cpu()
+            gh = [g.cpu() if g is not None else None for g in ga]
+        assert torch.equal(Yh, Ys.cpu())
+        for k in range(N):
+            if k != mode:
+                err = rel_err(gh[k].numpy(), gs[k].cpu().numpy())
+                print(f"mode {mode}: async dL/dU_{k} rel err {err:.2e}")
+                assert err < 1e-12
+
+
+def test_duplicate_samples_rejected(ctx, orc):
+    """Samples sharing every non-target coordinate: the synchronous entries
+    return EX_ERR_ARG (outputs NaN), the stream-ordered ones fill their
+    outputs with NaN; a deduplicated call on the same tensor afterwards is
+    exact again."""
+    import torch
+    from rdist_tensor_amd import abi
+    from rdist_tensor_amd.autograd import _ptr_array
+    N, R = 3, 8
+    rng, dims, idx, vals, T, U, Ut = setup_case(ctx, orc, N, R, 78)
+    mode = 1
+    sd, wd = make_samples(orc, rng, idx, dims, mode, 500, 100)
+    dup = np.concatenate([sd, sd[:1]]).copy()
+    dup[-1, mode] = (dup[-1, mode] + 1) % dims[mode]   # the target column differs, the fiber is the same
+    wdup = np.concatenate([wd, wd[:1]])
+    sdt = torch.tensor(dup.astype(np.int32), device="cuda")
+    wdt = torch.tensor(wdup, device="cuda")
+    Y = torch.zeros((dims[mode], R), dtype=torch.float64, device="cuda")
+    rc = abi.lib().ex_smttkrp_forward(T.h, mode, len(dup), sdt.data_ptr(), wdt.data_ptr(), _ptr_array(Ut), R,
+                                      Y.data_ptr())
+    assert rc == abi.EX_ERR_ARG and "deduplicated" in abi.lib().ex_last_error().decode()
+    assert bool(torch.isnan(Y).all())
+    G = torch.ones((dims[mode], R), dtype=torch.float64, device="cuda")
+    ga = [torch.zeros_like(u) if k != mode else None for k, u in enumerate(Ut)]
+    abi.check(abi.lib().ex_smttkrp_backward_async(T.h, None, mode, len(dup), sdt.data_ptr(), wdt.data_ptr(),
+                                                  _ptr_array(Ut), R, G.data_ptr(), _ptr_array(ga)))
+    torch.cuda.synchronize()
+    assert all(bool(torch.isnan(g).all()) for g in ga if g is not None)
+    # deduplicated again: matches the oracle
+    D, _ = orc.design(sd, wd, mode, U)
+    Yo, _ = orc.Lookup(idx, vals, mode, dims).sampled_mttkrp(sd, D)
+    s2 = torch.tensor(sd.astype(np.int32), device="cuda")
+    w2 = torch.tensor(wd, device="cuda")
+    abi.check(abi.lib().ex_smttkrp_forward(T.h, mode, len(sd), s2.data_ptr(), w2.data_ptr(), _ptr_array(Ut), R,
+                                           Y.data_ptr()))
+    assert rel_err(Y.cpu().numpy(), Yo) < 1e-12
+
+
 def test_backward_central_differences(ctx, orc):
     import torch
     from rdist_tensor_amd.autograd import SampledMTTKRP
