@@ -244,8 +244,13 @@ __global__ __launch_bounds__(T) void spd_inverse_kernel(const double* __restrict
 // factorisations M = sum_v w_v w_v^T (rows w_v): slot 0 = (Hadamard product)^-1
 // (Gauss-Jordan, pivots <= 1e-8 dropped), slot N rows from the Cholesky factor
 // of slot 0, slots k rows from the Cholesky factor of (later Grams (.) slot 0).
-// Any such factorisation gives the same row-sampling distribution.  Every
-// thread reads and writes only its own elements of each slot.
+// Any such factorisation gives the same row-sampling distribution.  The
+// factorisations are independent once slot 0 is known: workgroup 0 writes
+// slot 0, the Hadamard product (slot N + 1) and slot N, workgroup q >= 1 the
+// q-th slot k of N - 1 .. 1 (k != j); every workgroup forms the products and
+// the Gauss-Jordan inverse itself (the same arithmetic, so the same values)
+// and keeps its slot's product of the later Grams in registers.  Every thread
+// reads and writes only its own elements of each slot.
 template <int RPT, int T = kSpdThreads>
 __global__ __launch_bounds__(T) void sts_compute_m_mixed(int N, int j, int R,
                                                                    const double* __restrict__ grams,
@@ -253,13 +258,21 @@ __global__ __launch_bounds__(T) void sts_compute_m_mixed(int N, int j, int R,
     __shared__ SpdShared sh;
     const SpdLayout L(R);
     const int R2 = R * R;
-    double a[RPT], s0[RPT];
+    // this workgroup's slot: N for workgroup 0, else the q-th k of N-1 .. 1 (k != j)
+    int slot = N;
+    for (int k = N - 1, q = 0; k > 0 && blockIdx.x > 0; k--) {
+        if (k == j) continue;
+        if (++q == (int)blockIdx.x) {
+            slot = k;
+            break;
+        }
+    }
+    double a[RPT], s0[RPT], later[RPT];
 #pragma unroll
-    for (int m = 0; m < RPT; m++) a[m] = 1.0;
-    if (L.on)
-#pragma unroll
-        for (int m = 0; m < RPT; m++)
-            if (L.row(m) < R) M[N * R2 + L.row(m) * R + L.j] = 1.0;
+    for (int m = 0; m < RPT; m++) {
+        a[m] = 1.0;
+        later[m] = 0.0;
+    }
     for (int k = N - 1; k >= 0; k--) {
         if (k == j) continue;
         if (L.on)
@@ -268,12 +281,12 @@ __global__ __launch_bounds__(T) void sts_compute_m_mixed(int N, int j, int R,
                 const int o = L.row(m) * R + L.j;
                 if (L.row(m) < R) {
                     a[m] *= grams[k * R2 + o];
-                    M[k * R2 + o] = a[m];
+                    if (k == slot) later[m] = a[m];
                 }
             }
     }
-    // a: the full Hadamard product (slot 0; slot 1's copy when j == 0)
-    if (L.on)
+    // a: the full Hadamard product
+    if (blockIdx.x == 0 && L.on)
 #pragma unroll
         for (int m = 0; m < RPT; m++)
             if (L.row(m) < R) M[(N + 1) * R2 + L.row(m) * R + L.j] = a[m];
@@ -284,25 +297,19 @@ __global__ __launch_bounds__(T) void sts_compute_m_mixed(int N, int j, int R,
 #pragma unroll
     for (int m = 0; m < RPT; m++) {
         s0[m] = a[m];
-        if (L.on && L.row(m) < R) M[L.row(m) * R + L.j] = a[m];
+        if (blockIdx.x == 0 && L.on && L.row(m) < R) M[L.row(m) * R + L.j] = a[m];
+    }
+    if (slot != N) {
+        // slot k: rows = columns of chol(later Grams (.) slot 0)
+#pragma unroll
+        for (int m = 0; m < RPT; m++) a[m] = (L.on && L.row(m) < R) ? later[m] * s0[m] : 0.0;
     }
     // slot N: rows = columns of chol(slot 0)
     spd_cholesky_regs<RPT>(sh, L, a, R, 1e-12 * spd_max_diag<RPT>(sh, L, a, R));
     if (L.on)
 #pragma unroll
         for (int m = 0; m < RPT; m++)
-            if (L.row(m) < R) M[N * R2 + L.j * R + L.row(m)] = a[m];
-    for (int k = N - 1; k > 0; k--) {
-        if (k == j) continue;
-#pragma unroll
-        for (int m = 0; m < RPT; m++)
-            a[m] = (L.on && L.row(m) < R) ? M[k * R2 + L.row(m) * R + L.j] * s0[m] : 0.0;
-        spd_cholesky_regs<RPT>(sh, L, a, R, 1e-12 * spd_max_diag<RPT>(sh, L, a, R));
-        if (L.on)
-#pragma unroll
-            for (int m = 0; m < RPT; m++)
-                if (L.row(m) < R) M[k * R2 + L.j * R + L.row(m)] = a[m];
-    }
+            if (L.row(m) < R) M[slot * R2 + L.j * R + L.row(m)] = a[m];
 }
 
 // ---------------------------------------------------------------- FP32 row kernels
@@ -454,12 +461,13 @@ void tree_build_f32(hipStream_t s, const float* U, const TreeGeom& g, int R, flo
 }
 
 // rows per thread: 256 / R row groups cover R rows
-#define RM_DISPATCH(R, KERNEL, STREAM, ...)                                                                  \
+#define RM_DISPATCH_GRID(R, KERNEL, GRID, STREAM, ...)                                                      \
     do {                                                                                                    \
-        if ((R) <= 16) hipLaunchKernelGGL((KERNEL<1>), dim3(1), dim3(kSpdThreads), 0, STREAM, __VA_ARGS__);  \
-        else if ((R) <= 35) hipLaunchKernelGGL((KERNEL<5>), dim3(1), dim3(kSpdThreads), 0, STREAM, __VA_ARGS__); \
-        else hipLaunchKernelGGL((KERNEL<16>), dim3(1), dim3(kSpdThreads), 0, STREAM, __VA_ARGS__);          \
+        if ((R) <= 16) hipLaunchKernelGGL((KERNEL<1>), GRID, dim3(kSpdThreads), 0, STREAM, __VA_ARGS__);     \
+        else if ((R) <= 35) hipLaunchKernelGGL((KERNEL<5>), GRID, dim3(kSpdThreads), 0, STREAM, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERNEL<16>), GRID, dim3(kSpdThreads), 0, STREAM, __VA_ARGS__);             \
     } while (0)
+#define RM_DISPATCH(R, KERNEL, STREAM, ...) RM_DISPATCH_GRID(R, KERNEL, dim3(1), STREAM, __VA_ARGS__)
 
 // R <= 32: one element per thread (1024 threads), else the 256-thread tiling.
 static void spd_inverse_launch(hipStream_t s, const double* M, int R, double tol_rel, double* out, int* dropped,
@@ -485,8 +493,11 @@ void spd_pinv(hipStream_t s, const double* M, int R, double tol_rel, double* out
 void sts_compute_m_mixed_launch(hipStream_t s, int N, int j, int R, const double* grams, double* M) {
     // R <= 32: one element per thread (1024 threads; faster than 256 threads
     // with five rows each, and than one wavefront with sixteen)
-    if (R <= 32) hipLaunchKernelGGL((sts_compute_m_mixed<1, 1024>), dim3(1), dim3(1024), 0, s, N, j, R, grams, M);
-    else RM_DISPATCH(R, sts_compute_m_mixed, s, N, j, R, grams, M);
+    // one workgroup per factorisation: slot N and every slot k in 1 .. N-1, k != j
+    int jobs = 1;
+    for (int k = 1; k < N; k++) jobs += k != j ? 1 : 0;
+    if (R <= 32) hipLaunchKernelGGL((sts_compute_m_mixed<1, 1024>), dim3(jobs), dim3(1024), 0, s, N, j, R, grams, M);
+    else RM_DISPATCH_GRID(R, sts_compute_m_mixed, dim3(jobs), s, N, j, R, grams, M);
     EX_HIP(hipGetLastError());
 }
 

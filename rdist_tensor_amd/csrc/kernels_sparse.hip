@@ -157,22 +157,67 @@ __device__ __forceinline__ uint32_t dh_slot(uint64_t key, int tbits) {
     return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - tbits));
 }
 
+// Samples of one workgroup of dh_insert, and the slots of its LDS table
+// (load <= 1/2).
+constexpr int kDhIns = 256;
+constexpr int kDhLocal = 2 * kDhIns;
+
+// Inserts every sample's key: first within the workgroup (an LDS table
+// holds each distinct key's multiplicity and smallest sample index), then
+// one global insert per distinct key of the workgroup, so that a key drawn
+// many times costs one global atomic per workgroup instead of one per draw.
+// slotof[i] = the global slot of sample i's key.
 __global__ __launch_bounds__(kThreads) void dh_insert(const uint32_t* __restrict__ samples, int64_t J, KeyArgs ka,
                                                       int tbits, unsigned long long* __restrict__ hkey,
                                                       uint32_t* __restrict__ hfirst, uint32_t* __restrict__ hcount,
                                                       uint32_t* __restrict__ slotof) {
-    const uint32_t mask = (1u << tbits) - 1u;
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < J; i += (int64_t)gridDim.x * blockDim.x) {
+    constexpr int PER = kDhIns / kThreads;
+    __shared__ unsigned long long lkey[kDhLocal];
+    __shared__ uint32_t lfirst[kDhLocal], lcount[kDhLocal], lslot[kDhLocal];
+    for (int h = threadIdx.x; h < kDhLocal; h += kThreads) {
+        lkey[h] = 0ull;
+        lfirst[h] = ~0u;
+        lcount[h] = 0u;
+    }
+    __syncthreads();
+    const int64_t b = (int64_t)blockIdx.x * kDhIns;
+    uint32_t mine[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int64_t i = b + q * kThreads + threadIdx.x;
+        mine[q] = 0u;
+        if (i >= J) continue;
         const unsigned long long key = pack(samples + i * ka.N, ka) + 1ull;
+        uint32_t h = dh_slot(key, 31) & (kDhLocal - 1);
+        while (true) {
+            const unsigned long long prev = atomicCAS(lkey + h, 0ull, key);
+            if (prev == 0ull || prev == key) break;
+            h = (h + 1u) & (kDhLocal - 1);
+        }
+        atomicMin(lfirst + h, (uint32_t)i);
+        atomicAdd(lcount + h, 1u);
+        mine[q] = h;
+    }
+    __syncthreads();
+    const uint32_t mask = (1u << tbits) - 1u;
+    for (int hl = threadIdx.x; hl < kDhLocal; hl += kThreads) {
+        const unsigned long long key = lkey[hl];
+        if (key == 0ull) continue;
         uint32_t h = dh_slot(key, tbits);
         while (true) {
             const unsigned long long prev = atomicCAS(hkey + h, 0ull, key);
             if (prev == 0ull || prev == key) break;
             h = (h + 1u) & mask;
         }
-        atomicMax(hfirst + h, ~(uint32_t)i);
-        atomicAdd(hcount + h, 1u);
-        slotof[i] = h;
+        atomicMax(hfirst + h, ~lfirst[hl]);
+        atomicAdd(hcount + h, lcount[hl]);
+        lslot[hl] = h;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int64_t i = b + q * kThreads + threadIdx.x;
+        if (i < J) slotof[i] = lslot[mine[q]];
     }
 }
 
@@ -1702,7 +1747,7 @@ int64_t dedup_hash(hipStream_t s, const uint32_t* samples, const double* weights
     uint32_t* slotof = (uint32_t*)(cnt + 2);
     uint32_t* tcount = slotof + J;
     const KeyArgs ka = key_args(kl);
-    hipLaunchKernelGGL(dh_insert, dim3(grid_for(J, kThreads, 16384)), dim3(kThreads), 0, s, samples, J, ka, tbits,
+    hipLaunchKernelGGL(dh_insert, dim3((unsigned)((J + kDhIns - 1) / kDhIns)), dim3(kThreads), 0, s, samples, J, ka, tbits,
                        hkey, hfirst, hcount, slotof);
     hipLaunchKernelGGL(dh_count, dim3((unsigned)ntiles), dim3(kThreads), 0, s, slotof, J, hfirst, tcount);
     hipLaunchKernelGGL(dh_scatter, dim3((unsigned)ntiles), dim3(kThreads), 0, s, samples, weights, J, ka, slotof, tcount,
