@@ -501,7 +501,9 @@ def run_batched(args):
                                f"{' (graph replay)' if args.batch_graphs else ''}"},
         "roofline": {"kernel": "batch_sweep_kernel<float>", "bound": "hbm", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "avg_launch_ms": round(avg * 1e3, 4), "algorithmic_bytes_per_launch": int(per)},
+                     "traffic": (load_traffic(os.path.join(ROOT, "profiles", "hbm_traffic.md"))
+                                 .get("batch_sweep_kernel/mixed/batched_arls") if args.scale == 1.0 and args.batch_count == 256 else None),
+                     "avg_launch_ms": round(avg * 1e3, 4), "algorithmic_bytes_per_launch": int(per)},
         "cpu_baseline": cpu,
         "nonzeros_iterated_per_s": round(nz_rate, 1),
         "unique_samples_per_sweep": int(np.sum(h["counters"][1]) / args.steps),
