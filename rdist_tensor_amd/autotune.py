@@ -32,6 +32,7 @@ KNOBS = {
     "sts_draw.fp64.r16": ("sts_1e6x1e6x1e5", 16, "fp64", "sts_draw"),
     "sts_draw.fp32.r16": ("sts_1e6x1e6x1e5", 16, "mixed", "sts_draw"),
     "sts_draw.fp32.r64": ("sts_1e7x1e6x1e4x1e2", 64, "mixed", "sts_draw"),
+    "sts_first.fp32": ("sts_1e6x1e6x1e5", 32, "mixed", "sts_draw"),
 }
 SCALE = {"sts_1e6x1e6x1e5": 1.0, "sts_1e7x1e6x1e4x1e2": 0.02}
 

@@ -53,7 +53,7 @@ thread_local std::string g_last_error;
 const TuneInfo kTuneTable[kTuneKnobs] = {
     {"sts_draw.fp64.r16", 3}, {"sts_draw.fp64.r32", 4}, {"sts_draw.fp32.r16", 3},
     {"sts_draw.fp32.r32", 4}, {"sts_draw.fp32.r64", 2}, {"row_scan.fp64", 2},
-    {"row_scan.fp32", 2},
+    {"row_scan.fp32", 2}, {"sts_first.fp32", 2},
 };
 std::atomic<int> g_tune[kTuneKnobs];
 std::once_flag g_tune_once;
@@ -369,6 +369,7 @@ struct ex_sampler {
     // u64: STS-CP draws given weight 0 since creation (mixed path: a
     // rounding residue routed the draw into a leaf of zero mass)
     DevBuf dropped;
+    DevBuf first_cdf;   // mixed: prefix sums of the first non-target mode's leaf masses
     // set when a batch advanced the factors: every mode's state is rebuilt
     // before the next draw or score read
     bool refresh = false;
@@ -510,8 +511,18 @@ void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const do
             m[k2].ge = s->ge;
         }
         Span sd(s->ctx, "sts_draw");
+        // mixed: the first non-target mode's draws search prefix sums of its
+        // leaf masses (its running product is 1, so R queries cover them all)
+        const double* first_cdf = nullptr;
+        if (mixed && tune_value(kTuneFirstF32) == 0) {
+            const int k0 = j == 0 ? 1 : 0;
+            const size_t L0 = s->g[k0].leaf_count;
+            s->first_cdf.ensure((size_t)R * (L0 + (L0 + 31) / 32) * 8);
+            k::sts_first_cdf(st, m[k0], R, s->first_cdf.as<double>());
+            first_cdf = s->first_cdf.as<double>();
+        }
         k::sts_draw(st, N, j, R, J, m, s->pinvp.as<double>(), s->seed, sweep, draws_row_dev, draws_eig_dev, samples,
-                    weights, mixed ? 4 : 8, s->sweep_dev, s->dropped.as<unsigned long long>());
+                    weights, mixed ? 4 : 8, s->sweep_dev, s->dropped.as<unsigned long long>(), first_cdf);
         sd.end();
     }
 }

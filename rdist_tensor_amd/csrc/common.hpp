@@ -199,11 +199,22 @@ struct StsModeArgs {
     TreeGeom ge;
 };
 // node_bytes = 8: FP64 rows and row-tree nodes; 4: FP32 (the mixed path).
+// first_cdf (FP32 only): the prefix sums of sts_first_cdf for the first
+// non-target mode, whose draws then search them instead of descending the
+// trees (nullptr: every mode descends).
 void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* modes,
               const double* pinv_packed, uint64_t seed, uint64_t sweep,
               const double* draws_row, const double* draws_eig, uint32_t* samples, double* weights,
               int node_bytes = 8, const uint64_t* sweep_dev = nullptr,
-              unsigned long long* dropped = nullptr);
+              unsigned long long* dropped = nullptr, const double* first_cdf = nullptr);
+// The first non-target mode's draw distribution in closed form: with the
+// running product h = 1 the row query is an eigen row w_v, so the leaf
+// masses m_v(l) = w_v^T G_l w_v (FP32 row-tree leaf Grams) give, per
+// component v, prefix sums over the leaves in two levels (FP64): within
+// each tile of 32 leaves, cdf[v L + l], and over the NT = ceil(L / 32) tile
+// totals, cdf[R L + v NT + q]; the last is v's total.  R <= 64; cdf holds
+// R (L + NT) doubles.
+void sts_first_cdf(hipStream_t s, const StsModeArgs& m, int R, double* cdf);
 void pack_upper(hipStream_t s, const double* M, int R, double* packed);
 // sparse
 void keys_from_coords(hipStream_t s, const uint32_t* coords, int64_t n, const KeyLayout& kl, uint64_t* keys,
@@ -304,6 +315,7 @@ enum TuneKnob {
     kTuneDrawF32R64,   // FP32, 45 < R <= 64
     kTuneScanF64,      // sampled-MTTKRP row pass, FP64: lanes per hit x 1 or x 2
     kTuneScanF32,      // FP32
+    kTuneFirstF32,     // FP32 STS-CP draw, first non-target mode: prefix sums (0) or tree descents (1)
     kTuneKnobs
 };
 struct TuneInfo {

@@ -144,6 +144,7 @@ struct StsArgs {
     double* weights;
     unsigned long long* dropped;   // when set, draws given weight 0 are counted here
     int vec_leaf;           // leaf rows read as W-wide vectors (R % W == 0, aligned factors)
+    const double* first_cdf;   // FP32: prefix sums for the first non-target mode (sts_first_cdf), or null
 };
 
 // Lane exchange inside rows of 16 lanes through DPP (CTRL: quad_perm,
@@ -356,6 +357,27 @@ __device__ __forceinline__ int leaf_pick(const double* q2, int F, double frac, i
     return F - 1;
 }
 
+// First index in [0, n) whose cumulative value exceeds t (cum ascending,
+// cum[n - 1] > t), the G lanes of a group probing G positions per step; every
+// lane of the group returns it.  With cum[n - 1] <= t (no mass) it returns
+// n - 1.
+template <int G>
+__device__ __forceinline__ int64_t group_upper(const double* __restrict__ cum, int64_t n, double t, int l, int grp) {
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1);
+    int64_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int64_t step = (hi - lo + G - 1) / G;
+        int64_t idx = lo + (int64_t)(l + 1) * step - 1;
+        if (idx > hi - 1) idx = hi - 1;
+        const uint64_t bits = (__ballot(cum[idx] > t) >> (grp * G)) & gmask;
+        const int first = bits ? __ffsll((unsigned long long)bits) - 1 : G - 1;
+        const int64_t nhi = lo + (int64_t)(first + 1) * step;
+        lo += (int64_t)first * step;
+        hi = nhi < hi ? nhi : hi;
+    }
+    return lo;
+}
+
 // x^T G x of node c of a tree whose first nst nodes are staged in LDS
 // (snodes, same stride S): the staged copy when it holds c, else the node in
 // global memory.  The nodes of the top levels are staged, so every group of
@@ -444,6 +466,7 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
     constexpr int NST = C::NSTAGE;
     __shared__ typename C::Shared sh;
     __shared__ __attribute__((aligned(16))) TN stage[2][NST > 0 ? NST * C::STRIDE : 1];
+    __shared__ double s_ecdf[kMaxRank];   // first mode by prefix sums: components' cumulative totals
     typedef typename Vec<TN>::type VT;
     const int R = a.R, N = a.N, j = a.j;
     const int P = tri_size(R);
@@ -452,6 +475,17 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
         int ea, eb;
         tri_ab(R, e, ea, eb);
         sh.pairs[e] = (uint16_t)(ea | (eb << 8));
+    }
+    // the first non-target mode: prefix sums instead of descents when given
+    const int k0 = j == 0 ? 1 : 0;
+    if (a.first_cdf != nullptr && threadIdx.x < 64) {
+        const int64_t L0 = a.m[k0].g.leaf_count, NT0 = (L0 + 31) / 32;
+        double x = (int)threadIdx.x < R ? a.first_cdf[(int64_t)R * L0 + (int64_t)threadIdx.x * NT0 + NT0 - 1] : 0.0;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y = __shfl_up(x, d, 64);
+            if ((int)threadIdx.x >= d) x += y;
+        }
+        if ((int)threadIdx.x < R) s_ecdf[threadIdx.x] = x;
     }
     __syncthreads();
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
@@ -474,38 +508,64 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
         const TN* rows_k = static_cast<const TN*>(md.rows);
         const TN* eig_k = static_cast<const TN*>(md.eig);
         const TN* nodes_k = static_cast<const TN*>(md.nodes);
+        const bool by_cdf = a.first_cdf != nullptr && k == k0;   // the same in every thread
         // stage the top levels of both trees (S is a multiple of W)
         const int ne = (int)md.ge.node_count < NST ? (int)md.ge.node_count : NST;
         const int nr = (int)md.g.node_count < NST ? (int)md.g.node_count : NST;
-        __syncthreads();
-        if (NST > 0) {
-            const int ve = ne * S / W, vr = nr * S / W;
-            for (int i = threadIdx.x; i < ve + vr; i += NT) {
-                if (i < ve) reinterpret_cast<VT*>(stage[0])[i] = reinterpret_cast<const VT*>(eig_k)[i];
-                else reinterpret_cast<VT*>(stage[1])[i - ve] = reinterpret_cast<const VT*>(nodes_k)[i - ve];
+        if (!by_cdf) {
+            __syncthreads();
+            if (NST > 0) {
+                const int ve = ne * S / W, vr = nr * S / W;
+                for (int i = threadIdx.x; i < ve + vr; i += NT) {
+                    if (i < ve) reinterpret_cast<VT*>(stage[0])[i] = reinterpret_cast<const VT*>(eig_k)[i];
+                    else reinterpret_cast<VT*>(stage[1])[i - ve] = reinterpret_cast<const VT*>(nodes_k)[i - ve];
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
         if (!active) continue;
-        // eigen tree: query h, F = 1
-        outer_g<TN, G, NV>(sh.pairs, hs, P, l, xx);
         const double ue = a.draws_eig ? a.draws_eig[(int64_t)k * a.J + s]
                                       : uniform01(a.seed, stream_id(sweep, j, k, 1), s);
-        double low, high;
-        int64_t c = descend_g<TN, G, NV>(eig_k, stage[0], ne, md.ge, S, P, xx, l, ue, low, high);
-        const int64_t v = md.ge.leaf_idx(c);
-        eig_idx = (uint32_t)v;
-        for (int b = l; b < R; b += G) xs[b] = (TN)((double)hs[b] * md.w[v * R + b]);
-        wave_lds_sync();
-        // row tree: query sh = h * w_v, F = R
-        outer_g<TN, G, NV>(sh.pairs, xs, P, l, xx);
         const double ur = a.draws_row ? a.draws_row[(int64_t)k * a.J + s]
                                       : uniform01(a.seed, stream_id(sweep, j, k, 2), s);
-        c = descend_g<TN, G, NV>(nodes_k, stage[1], nr, md.g, S, P, xx, l, ur, low, high);
-        const int64_t leaf = md.g.leaf_idx(c);
+        int64_t v, leaf;
+        double frac;
+        if (by_cdf) {
+            // component v with probability proportional to its total, then
+            // the leaf at ur x total: the tile from the tiles' cumulative
+            // totals, the leaf from the tile's own prefix sums
+            const int64_t L = md.g.leaf_count, NTl = (L + 31) / 32;
+            v = group_upper<G>(s_ecdf, R, ue * s_ecdf[R - 1], l, grp);
+            const double* ttv = a.first_cdf + (int64_t)R * L + v * NTl;
+            const double t = ur * ttv[NTl - 1];
+            const int64_t tile = group_upper<G>(ttv, NTl, t, l, grp);
+            const double base = tile > 0 ? ttv[tile - 1] : 0.0;
+            const int64_t cnt = L - 32 * tile < 32 ? L - 32 * tile : 32;
+            const double* lpt = a.first_cdf + v * L + 32 * tile;
+            const int64_t jj = group_upper<G>(lpt, cnt, t - base, l, grp);
+            leaf = 32 * tile + jj;
+            const double lo = jj > 0 ? lpt[jj - 1] : 0.0;
+            frac = (t - base - lo) / (lpt[jj] - lo);
+            eig_idx = (uint32_t)v;
+            for (int b = l; b < R; b += G) xs[b] = (TN)((double)hs[b] * md.w[v * R + b]);
+            wave_lds_sync();
+        } else {
+            // eigen tree: query h, F = 1
+            outer_g<TN, G, NV>(sh.pairs, hs, P, l, xx);
+            double low, high;
+            int64_t c = descend_g<TN, G, NV>(eig_k, stage[0], ne, md.ge, S, P, xx, l, ue, low, high);
+            v = md.ge.leaf_idx(c);
+            eig_idx = (uint32_t)v;
+            for (int b = l; b < R; b += G) xs[b] = (TN)((double)hs[b] * md.w[v * R + b]);
+            wave_lds_sync();
+            // row tree: query sh = h * w_v, F = R
+            outer_g<TN, G, NV>(sh.pairs, xs, P, l, xx);
+            c = descend_g<TN, G, NV>(nodes_k, stage[1], nr, md.g, S, P, xx, l, ur, low, high);
+            leaf = md.g.leaf_idx(c);
+            frac = (ur - low) / (high - low);
+        }
         const int F = (int)md.g.F;
         const int rows = (int)((md.g.n - leaf * F) < F ? (md.g.n - leaf * F) : F);
-        const double frac = (ur - low) / (high - low);
         // q_r = U[leaf F + r] . sh for the leaf's real rows (0 for the rest),
         // q_r^2 into LDS; the cumulative sums run in row order
         const TN* blk = rows_k + leaf * F * R;
@@ -537,6 +597,128 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
         a.samples[s * N + j] = eig_idx;
         a.weights[s] = dropped ? 0.0 : (double)R / (q * (double)a.J);
         if (dropped && a.dropped) atomicAdd(a.dropped, 1ull);
+    }
+}
+
+// ---------------------------------------------------------------- first mode by prefix sums
+typedef float fx4 __attribute__((ext_vector_type(4)));
+typedef float fx16 __attribute__((ext_vector_type(16)));
+
+// Leaf masses of the first non-target mode, m_v(l) = w_v^T G_c w_v for the
+// leaf node c of leaf l and component v < R (FP32 packed row-tree nodes), as
+// the product A B on v_mfma_f32_32x32x2f32 with A[v][e] = (a == b ? 1 : 2)
+// w_v[a] w_v[b] for the packed element e = (a, b) and B[e][l] = G_c[e].  One
+// workgroup per tile of 32 leaves (in leaf order) x 32 components; its four
+// wavefronts take the K blocks of 32 packed elements in turn (block kb to
+// wavefront kb mod 4) and their partial products are added in wavefront
+// order.  In step t of a block lane i + 32 h supplies A[32 vt + i][e] and
+// B[e][32 lt + i] for e = 32 kb + 16 h + t (16 consecutive node elements per
+// lane, read as float4; elements past the packed size count zero); output:
+// lane j + 32 h, register r holds component 32 vt + 8 (r / 4) + 4 h + r % 4
+// of leaf 32 lt + j.  Written, per component v and tile: the tile's
+// inclusive prefix sums lp[v L + l] (FP64, within the tile) and its total
+// tt[v NT + lt] (NT = tiles).
+__global__ __launch_bounds__(kThreads) void first_mass_kernel(const float* __restrict__ nodes, TreeGeom g, int R,
+                                                              int S, const double* __restrict__ w,
+                                                              double* __restrict__ lp, double* __restrict__ tt) {
+    __shared__ float wt[kMaxRank * 64];   // wt[a * 64 + v] = w_v[a]
+    __shared__ uint16_t pairs[kMaxRank * (kMaxRank + 1) / 2 + 32];
+    __shared__ float part[kThreads / 64 - 1][16][64];
+    const int P = tri_size(R);
+    const int kblocks = (P + 31) / 32;
+    for (int o = threadIdx.x; o < kMaxRank * 64; o += blockDim.x) {
+        const int a2 = o / 64, v = o % 64;
+        wt[o] = (a2 < R && v < R) ? (float)w[v * R + a2] : 0.0f;
+    }
+    for (int e = threadIdx.x; e < kblocks * 32; e += blockDim.x) {
+        int ea = 0, eb = 0;
+        if (e < P) tri_ab(R, e, ea, eb);
+        pairs[e] = (uint16_t)(ea | (eb << 8));
+    }
+    __syncthreads();
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64, i = lane % 32, h = lane / 32;
+    const int64_t L = g.leaf_count, NT = (L + 31) / 32;
+    const int vtiles = (R + 31) / 32;
+    const int64_t lt = blockIdx.x / vtiles;
+    const int vt = (int)(blockIdx.x % vtiles);
+    // this lane's leaf (column of B) and its node: the bottom-level leaves
+    // come first in leaf order, then those one level up
+    const int64_t li = lt * 32 + i;
+    const bool lok = li < L;
+    const int64_t nbottom = (int64_t)g.node_count - g.nodes_upto_lfill;
+    const int64_t lc = lok ? li : 0;
+    const int64_t c = lc < nbottom ? (int64_t)g.nodes_upto_lfill + lc : lc + g.complete_level_offset;
+    const float* node = nodes + c * S;
+    const int v = 32 * vt + i;   // this lane's component (row of A)
+    fx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+    for (int kb = wv; kb < kblocks; kb += kThreads / 64) {
+        const int e0 = 32 * kb + 16 * h;
+        float bv[16];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const fx4 x = *reinterpret_cast<const fx4*>(node + e0 + 4 * q);
+#pragma unroll
+            for (int u = 0; u < 4; u++) bv[4 * q + u] = (lok && e0 + 4 * q + u < P) ? x[u] : 0.0f;
+        }
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+            const uint32_t pr = pairs[e0 + t];
+            const int ea = pr & 0xFF, eb = pr >> 8;
+            const float x = wt[ea * 64 + v] * wt[eb * 64 + v];
+            const float av = e0 + t < P ? (ea == eb ? x : 2.0f * x) : 0.0f;
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv[t], acc, 0, 0, 0);
+        }
+    }
+    if (wv > 0)
+#pragma unroll
+        for (int r = 0; r < 16; r++) part[wv - 1][r][lane] = acc[r];
+    __syncthreads();
+    if (wv != 0) return;
+    const int cnt = (int)(L - 32 * lt < 32 ? L - 32 * lt : 32);
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        float m = acc[r];
+        for (int q = 0; q < kThreads / 64 - 1; q++) m += part[q][r][lane];
+        // inclusive prefix over the tile's leaves j = lane % 32 (32-lane segments)
+        double x = i < cnt ? (double)m : 0.0;
+        for (int d = 1; d < 32; d <<= 1) {
+            const double y = __shfl_up(x, d, 32);
+            if (i >= d) x += y;
+        }
+        const int vv = 32 * vt + 8 * (r / 4) + 4 * h + r % 4;
+        if (vv < R && i < cnt) {
+            lp[(int64_t)vv * L + li] = x;
+            if (i == cnt - 1) tt[(int64_t)vv * NT + lt] = x;
+        }
+    }
+}
+
+// In place, per component (workgroup) v: tt[v NT + q] = the tile totals
+// summed over tiles <= q (FP64, in tile order, 1024 at a time).
+__global__ __launch_bounds__(1024) void first_scan_kernel(double* __restrict__ tt, int64_t NT) {
+    __shared__ double wsum[16];
+    double* x = tt + (int64_t)blockIdx.x * NT;
+    const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
+    double offset = 0.0;
+    for (int64_t b0 = 0; b0 < NT; b0 += 1024) {
+        const int64_t q = b0 + threadIdx.x;
+        const double val = q < NT ? x[q] : 0.0;
+        double inc = val;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += y;
+        }
+        if (lane == 63) wsum[wv] = inc;
+        __syncthreads();
+        double pre = offset;
+        for (int w2 = 0; w2 < wv; w2++) pre += wsum[w2];
+        if (q < NT) x[q] = pre + inc;
+        double tot = 0.0;
+        for (int w2 = 0; w2 < 16; w2++) tot += wsum[w2];
+        __syncthreads();
+        offset += tot;
     }
 }
 
@@ -650,6 +832,17 @@ __global__ __launch_bounds__(kThreads) void arls_draw_kernel(ArlsArgs a) {
 
 namespace k {
 
+void sts_first_cdf(hipStream_t s, const StsModeArgs& m, int R, double* cdf) {
+    EX_CHECK(R >= 1 && R <= kMaxRank, 1, "rank out of range");
+    const int64_t L = m.g.leaf_count, NT = (L + 31) / 32;
+    double* lp = cdf;
+    double* tt = cdf + (int64_t)R * L;
+    hipLaunchKernelGGL(first_mass_kernel, dim3((unsigned)(NT * ((R + 31) / 32))), dim3(kThreads), 0, s,
+                       static_cast<const float*>(m.nodes), m.g, R, tri_stride32(R), m.w, lp, tt);
+    hipLaunchKernelGGL(first_scan_kernel, dim3(R), dim3(1024), 0, s, tt, NT);
+    EX_HIP(hipGetLastError());
+}
+
 template <typename T>
 void level_sums(hipStream_t s, const TreeGeom& g, int S, T* nodes) {
     const int64_t first_leaf = (int64_t)g.node_count - g.leaf_count;
@@ -709,7 +902,8 @@ void tree_build(hipStream_t s, const double* U, const TreeGeom& g, int R, double
 
 void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* modes, const double* pinv_packed,
               uint64_t seed, uint64_t sweep, const double* draws_row, const double* draws_eig, uint32_t* samples,
-              double* weights, int node_bytes, const uint64_t* sweep_dev, unsigned long long* dropped) {
+              double* weights, int node_bytes, const uint64_t* sweep_dev, unsigned long long* dropped,
+              const double* first_cdf) {
     EX_CHECK(node_bytes == 8 || node_bytes == 4, 1, "node precision must be FP64 or FP32");
     StsArgs a;
     for (int k = 0; k < N; k++) a.m[k] = modes[k];
@@ -719,6 +913,7 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
     a.draws_row = draws_row; a.draws_eig = draws_eig;
     a.samples = samples; a.weights = weights;
     a.dropped = dropped;
+    a.first_cdf = node_bytes == 4 ? first_cdf : nullptr;
     // vector reads of leaf rows: R a multiple of the vector width and every
     // drawn factor aligned to it
     const int vw = node_bytes == 8 ? 2 : 4;

@@ -5,7 +5,9 @@ quantities as the default (ex_tuning_*):
   workgroup): index-exact against the oracle with injected uniforms, as the
   default is (draws inside a compared boundary's rounding bound exempt);
 * mixed STS-CP draw variants: chi-square, total variation and weights
-  against exact KRP leverage, as the default is;
+  against exact KRP leverage, as the default is (sts_first.fp32: the first
+  non-target mode drawn from prefix sums of its leaf masses, or by tree
+  descents like the other modes);
 * row-pass variants (lanes per hit): the FP64 sampled MTTKRP against the
   oracle (1e-12 relative, hits exact); the mixed ALS sweep's fit within
   1e-5 of the default variant's (FP32 sums grouped differently)."""
@@ -64,7 +66,8 @@ def test_fp64_draw_variants_bitexact(ctx, orc, knob, name, variant, R):
 
 @pytest.mark.parametrize("name,variant,R", [("sts_draw.fp32.r16", v, 16) for v in range(3)] +
                          [("sts_draw.fp32.r32", v, 32) for v in range(4)] +
-                         [("sts_draw.fp32.r64", v, 64) for v in range(2)])
+                         [("sts_draw.fp32.r64", v, 64) for v in range(2)] +
+                         [("sts_first.fp32", v, R) for v in range(2) for R in (16, 32, 64)])
 def test_mixed_draw_variants_distribution(ctx, knob, name, variant, R):
     from rdist_tensor_amd import abi
     knob(name, variant)
