@@ -51,8 +51,8 @@ thread_local std::string g_last_error;
 
 // Launch variants (common.hpp TuneKnob): names, variant counts, current values.
 const TuneInfo kTuneTable[kTuneKnobs] = {
-    {"sts_draw.fp64.r16", 3}, {"sts_draw.fp64.r32", 4}, {"sts_draw.fp32.r16", 3},
-    {"sts_draw.fp32.r32", 4}, {"sts_draw.fp32.r64", 2}, {"row_scan.fp64", 2},
+    {"sts_draw.fp64.r16", 4}, {"sts_draw.fp64.r32", 5}, {"sts_draw.fp32.r16", 4},
+    {"sts_draw.fp32.r32", 5}, {"sts_draw.fp32.r64", 3}, {"row_scan.fp64", 2},
     {"row_scan.fp32", 2}, {"sts_first.fp32", 2},
 };
 std::atomic<int> g_tune[kTuneKnobs];

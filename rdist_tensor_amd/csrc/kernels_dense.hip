@@ -56,15 +56,24 @@ __global__ __launch_bounds__(kThreads) void gram_mfma64_partial(const TA* __rest
         for (int tb = 0; tb < NT; tb++) acc[ta][tb] = f64x4{0.0, 0.0, 0.0, 0.0};
     const int64_t nchunks = (m + 15) / 16;
     for (int64_t ch = (int64_t)blockIdx.x * 4 + wave; ch < nchunks; ch += (int64_t)gridDim.x * 4) {
-        double x[4][NT];
+        // unconditional loads (element 0 in place of a missing one, then
+        // zeroed), all in flight together
+        TA y[4][NT];
 #pragma unroll
         for (int st = 0; st < 4; st++) {
             const int64_t r = ch * 16 + 4 * st + kr;
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 const int c = 16 * t + kc;
-                x[st][t] = (r < m && c < R) ? (double)A[r * R + c] : 0.0;
+                y[st][t] = A[(r < m && c < R) ? r * R + c : 0];
             }
+        }
+        double x[4][NT];
+#pragma unroll
+        for (int st = 0; st < 4; st++) {
+            const int64_t r = ch * 16 + 4 * st + kr;
+#pragma unroll
+            for (int t = 0; t < NT; t++) x[st][t] = (r < m && 16 * t + kc < R) ? (double)y[st][t] : 0.0;
         }
 #pragma unroll
         for (int st = 0; st < 4; st++)

@@ -23,17 +23,23 @@ __device__ __forceinline__ f32x4 mfma_bf16(const bf16x8& a, const bf16x8& b, con
 }
 
 // Load the 8 K-values this lane contributes for column `col` from rows
-// r0 + 8 (lane/16) + i (zero beyond `rows` or past R).
+// r0 + 8 (lane/16) + i (zero beyond `rows` or past R).  The loads are
+// unconditional (element (r0, 0) in place of a missing one, then zeroed), so
+// that all eight are in flight together.
 __device__ __forceinline__ bf16x8 load_k8(const float* __restrict__ A, int64_t r0, int rows, int R, int col,
                                           int lane) {
-    bf16x8 v;
+    float x[8];
     const int kb = 8 * (lane / 16);
 #pragma unroll
     for (int i = 0; i < 8; i++) {
         const int k = kb + i;
-        float x = (k < rows && col < R) ? A[(r0 + k) * R + col] : 0.0f;
-        v[i] = (__bf16)x;
+        const bool ok = k < rows && col < R;
+        const float y = A[ok ? (r0 + k) * R + col : r0 * R];
+        x[i] = ok ? y : 0.0f;
     }
+    bf16x8 v;
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = (__bf16)x[i];
     return v;
 }
 

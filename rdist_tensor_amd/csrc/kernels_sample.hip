@@ -235,18 +235,34 @@ __device__ __forceinline__ void outer_g(const uint16_t* pairs, const TN* x, int 
 
 // x^T G x of one packed node for the group's query: vector loads of W
 // consecutive elements per lane, lane partial in TN, group total, FP64 out.
-template <typename TN, int G, int NV>
+template <typename TN, int G, int NV, bool BATCH>
 __device__ __forceinline__ double quad_g(const TN* __restrict__ node, const TN (&xx)[NV * Vec<TN>::W], int P,
                                          int l) {
     constexpr int W = Vec<TN>::W;
     typedef typename Vec<TN>::type VT;
     const VT* nv = reinterpret_cast<const VT*>(node);
+    // no branch around the loads, so that they are in flight together: a
+    // vector past the packed triangle reads vector 0 of the node instead
+    // (finite; tree builders zero the stride's padding) and meets xx = 0
+    // (BATCH: every load before the first product, where the registers
+    // allow NV more vectors; otherwise each vector is used as it arrives)
     TN acc = 0;
+    if constexpr (BATCH) {
+        VT d[NV];
 #pragma unroll
-    for (int t = 0; t < NV; t++) {
-        const int v = t * G + l;
-        if (v * W < P) {
-            const VT d = nv[v];
+        for (int t = 0; t < NV; t++) {
+            const int v = t * G + l;
+            d[t] = nv[v * W < P ? v : 0];
+        }
+#pragma unroll
+        for (int t = 0; t < NV; t++)
+#pragma unroll
+            for (int i = 0; i < W; i++) acc += d[t][i] * xx[t * W + i];
+    } else {
+#pragma unroll
+        for (int t = 0; t < NV; t++) {
+            const int v = t * G + l;
+            const VT d = nv[v * W < P ? v : 0];
 #pragma unroll
             for (int i = 0; i < W; i++) acc += d[i] * xx[t * W + i];
         }
@@ -296,16 +312,19 @@ __device__ __forceinline__ void leaf_q2(const TN* __restrict__ blk, int rows, in
 #pragma unroll
             for (int i = 0; i < VL; i++) xv[i] = v < nvec ? x[v * VL + i] : (TN)0;
             TN d[kLeafBatch][VL];
+            // unconditional loads (row 0, vector 0 of the block in place of
+            // the missing ones, then zeroed), so that they are in flight together
 #pragma unroll
             for (int t = 0; t < kLeafBatch; t++) {
                 const int r = r0 + t * RPS + sr;
                 const bool ok = r < rows && v < nvec;
+                const int64_t at = ok ? (int64_t)r * R + v * VL : 0;
                 if constexpr (VL == 1) {
-                    d[t][0] = ok ? blk[(int64_t)r * R + v] : (TN)0;
+                    const TN e = blk[at];
+                    d[t][0] = ok ? e : (TN)0;
                 } else {
                     typedef typename Vec<TN>::type VT;
-                    VT e;
-                    if (ok) e = *reinterpret_cast<const VT*>(blk + (int64_t)r * R + v * VL);
+                    const VT e = *reinterpret_cast<const VT*>(blk + at);
 #pragma unroll
                     for (int i = 0; i < VL; i++) d[t][i] = ok ? e[i] : (TN)0;
                 }
@@ -382,31 +401,31 @@ __device__ __forceinline__ int64_t group_upper(const double* __restrict__ cum, i
 // (snodes, same stride S): the staged copy when it holds c, else the node in
 // global memory.  The nodes of the top levels are staged, so every group of
 // a wavefront takes the same branch at a given level.
-template <typename TN, int G, int NV>
+template <typename TN, int G, int NV, bool BATCH>
 __device__ __forceinline__ double quad_node(const TN* __restrict__ gnodes, const TN* snodes, int nst, int64_t c,
                                             int S, const TN (&xx)[NV * Vec<TN>::W], int P, int l) {
-    if (c < nst) return quad_g<TN, G, NV>(snodes + c * S, xx, P, l);
-    return quad_g<TN, G, NV>(gnodes + c * S, xx, P, l);
+    if (c < nst) return quad_g<TN, G, NV, BATCH>(snodes + c * S, xx, P, l);
+    return quad_g<TN, G, NV, BATCH>(gnodes + c * S, xx, P, l);
 }
 
 // Root-to-leaf descent of one heap tree (partition_tree.hpp:226-307): masses
 // and cutoffs in FP64, left if u <= low + m_left / m_root.
-template <typename TN, int G, int NV>
+template <typename TN, int G, int NV, bool BATCH>
 __device__ __forceinline__ int64_t descend_g(const TN* __restrict__ nodes, const TN* snodes, int nst,
                                              const TreeGeom& g, int S, int P, const TN (&xx)[NV * Vec<TN>::W], int l,
                                              double u, double& low, double& high) {
-    const double m = quad_node<TN, G, NV>(nodes, snodes, nst, 0, S, xx, P, l);
+    const double m = quad_node<TN, G, NV, BATCH>(nodes, snodes, nst, 0, S, xx, P, l);
     int64_t c = 0;
     low = 0.0;
     high = 1.0;
     for (uint32_t lvl = 0; lvl < g.lfill_level; lvl++) {
-        const double mL = quad_node<TN, G, NV>(nodes, snodes, nst, 2 * c + 1, S, xx, P, l);
+        const double mL = quad_node<TN, G, NV, BATCH>(nodes, snodes, nst, 2 * c + 1, S, xx, P, l);
         const double cutoff = low + mL / m;
         if (u <= cutoff) { c = 2 * c + 1; high = cutoff; }
         else { c = 2 * c + 2; low = cutoff; }
     }
     if (!g.is_leaf(c)) {
-        const double mL = quad_node<TN, G, NV>(nodes, snodes, nst, 2 * c + 1, S, xx, P, l);
+        const double mL = quad_node<TN, G, NV, BATCH>(nodes, snodes, nst, 2 * c + 1, S, xx, P, l);
         const double cutoff = low + mL / m;
         if (u <= cutoff) { c = 2 * c + 1; high = cutoff; }
         else { c = 2 * c + 2; low = cutoff; }
@@ -414,11 +433,45 @@ __device__ __forceinline__ int64_t descend_g(const TN* __restrict__ nodes, const
     return c;
 }
 
+// xs[b] = h[b] w[b] (FP64 product, rounded to TN) for the lane's entries b =
+// l + q G < R; every load of w is issued before the first product.
+template <typename TN, int G, int KB>
+__device__ __forceinline__ void query_of(TN* xs, const TN* hs, const double* __restrict__ w, int R, int l) {
+    double wv[KB];
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+        const int b = l + q * G;
+        wv[q] = w[b < R ? b : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+        const int b = l + q * G;
+        if (b < R) xs[b] = (TN)((double)hs[b] * wv[q]);
+    }
+}
+
+// hs[b] *= row[b] for the lane's entries, the loads issued together.
+template <typename TN, int G, int KB>
+__device__ __forceinline__ void hadamard_row(TN* hs, const TN* __restrict__ row, int R, int l) {
+    TN rv[KB];
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+        const int b = l + q * G;
+        rv[q] = row[b < R ? b : 0];
+    }
+#pragma unroll
+    for (int q = 0; q < KB; q++) {
+        const int b = l + q * G;
+        if (b < R) hs[b] *= rv[q];
+    }
+}
+
 // LDS of a draw workgroup: the per-sample state (DrawShared) and the nodes of
 // the top levels of the current mode's eigen tree and row tree.  A
-// 1024-thread workgroup has the CU to itself (150 KB); a 256-thread one
-// shares it with two others (48 KB).
-constexpr size_t draw_lds(int NT) { return NT >= 1024 ? 150 * 1024 : 48 * 1024; }
+// workgroup of 1024 threads (four waves per SIMD) or of 512 (two waves per
+// SIMD, up to 256 registers per lane) has the CU to itself (150 KB); a
+// 256-thread one shares it with two others (48 KB).
+constexpr size_t draw_lds(int NT) { return NT >= 512 ? 150 * 1024 : 48 * 1024; }
 constexpr int cx_stride(int R, int bytes) {
     return bytes == 8 ? ((R * (R + 1) / 2 + 15) & ~15) : ((R * (R + 1) / 2 + 31) & ~31);
 }
@@ -447,6 +500,10 @@ struct DrawCfg {
             : RMAX;
     typedef DrawShared<TN, SLOTS, RMAX, RP> Shared;
     static constexpr int NSTAGE = draw_nstage(sizeof(Shared), NT, STRIDE, (int)sizeof(TN));
+    // node loads of a level all in flight: four waves per SIMD (1024
+    // threads) leave 128 registers per lane, too few beside the query's
+    // NV W products unless those are few
+    static constexpr bool BATCH = NT <= 512 || NV * W <= 20;
 };
 
 // STS-CP KRPDrawSamples (sts_cp.hpp:228-286): one sample per group of G
@@ -464,6 +521,7 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
     constexpr int SPW = C::SPW;
     constexpr int RMAX = C::RMAX;
     constexpr int NST = C::NSTAGE;
+    constexpr int KB = (RMAX + G - 1) / G;   // entries b = l + q G of a lane
     __shared__ typename C::Shared sh;
     __shared__ __attribute__((aligned(16))) TN stage[2][NST > 0 ? NST * C::STRIDE : 1];
     __shared__ double s_ecdf[kMaxRank];   // first mode by prefix sums: components' cumulative totals
@@ -547,20 +605,20 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
             const double lo = jj > 0 ? lpt[jj - 1] : 0.0;
             frac = (t - base - lo) / (lpt[jj] - lo);
             eig_idx = (uint32_t)v;
-            for (int b = l; b < R; b += G) xs[b] = (TN)((double)hs[b] * md.w[v * R + b]);
+            query_of<TN, G, KB>(xs, hs, md.w + v * R, R, l);
             wave_lds_sync();
         } else {
             // eigen tree: query h, F = 1
             outer_g<TN, G, NV>(sh.pairs, hs, P, l, xx);
             double low, high;
-            int64_t c = descend_g<TN, G, NV>(eig_k, stage[0], ne, md.ge, S, P, xx, l, ue, low, high);
+            int64_t c = descend_g<TN, G, NV, C::BATCH>(eig_k, stage[0], ne, md.ge, S, P, xx, l, ue, low, high);
             v = md.ge.leaf_idx(c);
             eig_idx = (uint32_t)v;
-            for (int b = l; b < R; b += G) xs[b] = (TN)((double)hs[b] * md.w[v * R + b]);
+            query_of<TN, G, KB>(xs, hs, md.w + v * R, R, l);
             wave_lds_sync();
             // row tree: query sh = h * w_v, F = R
             outer_g<TN, G, NV>(sh.pairs, xs, P, l, xx);
-            c = descend_g<TN, G, NV>(nodes_k, stage[1], nr, md.g, S, P, xx, l, ur, low, high);
+            c = descend_g<TN, G, NV, C::BATCH>(nodes_k, stage[1], nr, md.g, S, P, xx, l, ur, low, high);
             leaf = md.g.leaf_idx(c);
             frac = (ur - low) / (high - low);
         }
@@ -580,17 +638,24 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
         if constexpr (sizeof(TN) == 4) dropped |= !(leaf_mass > 0.0);
         const int res = res0 < rows ? res0 : rows - 1;
         const int64_t row = leaf * F + res;
-        for (int b = l; b < R; b += G) hs[b] *= rows_k[row * R + b];
+        hadamard_row<TN, G, KB>(hs, rows_k + row * R, R, l);
         wave_lds_sync();
         if (l == 0) a.samples[s * N + k] = (uint32_t)row;
     }
     if (!active) return;
     // weight = R / (J * h^T pinv h), FP64
+    // (unrolled, unconditional loads: element 0 in place of those past P)
     double q = 0.0;
-    for (int e = l; e < P; e += G) {
-        const uint32_t pr = sh.pairs[e];
+    const int ksteps = (P + G - 1) / G;
+#pragma unroll 8
+    for (int t = 0; t < ksteps; t++) {
+        const int e = t * G + l;
+        const bool ok = e < P;
+        const double pv = a.pinvp[ok ? e : 0];
+        const uint32_t pr = sh.pairs[ok ? e : 0];
         const int ea = pr & 0xFF, eb = pr >> 8;
-        q += a.pinvp[e] * ((ea == eb ? 1.0 : 2.0) * ((double)hs[ea] * (double)hs[eb]));
+        const double term = pv * ((ea == eb ? 1.0 : 2.0) * ((double)hs[ea] * (double)hs[eb]));
+        q = ok ? q + term : q;
     }
     q = group_total<G>(q);
     if (l == 0) {
@@ -928,8 +993,9 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
                            dim3(NT), 0, s, a);                                                        \
     } while (0)
     // lanes per sample G and W-wide vectors per lane NV: G * NV * W >= R (R + 1) / 2;
-    // NT threads per workgroup (1024 where the registers of four waves per
-    // SIMD suffice: more samples share each staged copy of the top levels).
+    // NT threads per workgroup (1024: more samples share each staged copy of
+    // the top levels; 512: registers for every node load of a level in
+    // flight together).
     // Variant 0 of each rank class is the default; the others are the
     // alternatives the autotuner measures (tune_value).
     if (node_bytes == 8) {
@@ -937,6 +1003,7 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
             switch (tune_value(kTuneDrawF64R16)) {
                 case 1: DRAW_LAUNCH(double, 8, 9, 1024); break;
                 case 2: DRAW_LAUNCH(double, 32, 3, 1024); break;
+                case 3: DRAW_LAUNCH(double, 8, 9, 512); break;
                 default: DRAW_LAUNCH(double, 16, 5, 1024); break;
             }
         } else if (R <= 32) {
@@ -944,6 +1011,7 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
                 case 1: DRAW_LAUNCH(double, 16, 17, 1024); break;
                 case 2: DRAW_LAUNCH(double, 64, 5, 1024); break;
                 case 3: DRAW_LAUNCH(double, 32, 9, 256); break;
+                case 4: DRAW_LAUNCH(double, 16, 17, 512); break;
                 default: DRAW_LAUNCH(double, 32, 9, 1024); break;
             }
         } else if (R <= 45) {
@@ -956,6 +1024,7 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
             switch (tune_value(kTuneDrawF32R16)) {
                 case 1: DRAW_LAUNCH(float, 8, 5, 1024); break;
                 case 2: DRAW_LAUNCH(float, 32, 2, 1024); break;
+                case 3: DRAW_LAUNCH(float, 8, 5, 512); break;
                 default: DRAW_LAUNCH(float, 16, 3, 1024); break;
             }
         } else if (R <= 32) {
@@ -963,6 +1032,7 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
                 case 1: DRAW_LAUNCH(float, 16, 9, 1024); break;
                 case 2: DRAW_LAUNCH(float, 32, 5, 1024); break;
                 case 3: DRAW_LAUNCH(float, 8, 17, 256); break;
+                case 4: DRAW_LAUNCH(float, 8, 17, 512); break;
                 default: DRAW_LAUNCH(float, 8, 17, 1024); break;
             }
         } else if (R <= 45) {
@@ -970,6 +1040,7 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
         } else {
             switch (tune_value(kTuneDrawF32R64)) {
                 case 1: DRAW_LAUNCH(float, 32, 17, 1024); break;
+                case 2: DRAW_LAUNCH(float, 32, 17, 512); break;
                 default: DRAW_LAUNCH(float, 64, 9, 1024); break;
             }
         }

@@ -550,11 +550,11 @@ struct GroupRows {
     }
     // hits kb .. ke - 1 in steps of S, the loads of NB steps issued
     // together: meta(k, r, c, x) fetches hit k, pre_of(c) the handle of its
-    // design row (a dependent load may happen here), row_of(h, col0, d) the
-    // row's values at columns col0 .. col0 + VW - 1 (zero at columns >= R).
+    // design row (a dependent load may happen here), rows_of(h, col0, d) the
+    // NB rows' values at columns col0 .. col0 + VW - 1 (zero at columns >= R).
     // Each stage is issued for all NB steps before the next stage uses it.
-    template <int NB, typename Meta, typename Pre, typename RowOf>
-    __device__ void run(int64_t kb, int64_t ke, Meta meta, Pre pre_of, RowOf row_of) {
+    template <int NB, typename Meta, typename Pre, typename RowsOf>
+    __device__ void run(int64_t kb, int64_t ke, Meta meta, Pre pre_of, RowsOf rows_of) {
         for (int64_t k0 = kb; k0 < ke; k0 += (int64_t)NB * S) {
             uint32_t rr[NB], cc[NB];
             V xx[NB];
@@ -571,8 +571,7 @@ struct GroupRows {
             }
 #pragma unroll
             for (int b = 0; b < NB; b++) hh[b] = pre_of(cc[b]);
-#pragma unroll
-            for (int b = 0; b < NB; b++) row_of(hh[b], sub * VW, dd[b]);
+            rows_of(hh, sub * VW, dd);
 #pragma unroll
             for (int b = 0; b < NB; b++) {
                 const int64_t s0 = k0 + (int64_t)b * S;
@@ -630,11 +629,10 @@ struct GroupRows {
 template <typename V>
 constexpr int group_vw() { return sizeof(V) == 4 ? 4 : 2; }
 
-// Grouped steps whose loads are issued together: 8 for the FP32 exact
-// MTTKRP (every nonzero a hit, factor rows behind each), 4 otherwise (more
-// cost the sampled row pass its registers).
+// Grouped steps whose loads are issued together (4: more cost registers;
+// the exact MTTKRP issues each mode's factor rows of all 4 at once).
 template <typename V, bool kExact>
-constexpr int step_batch() { return (kExact && sizeof(V) == 4) ? 8 : 4; }
+constexpr int step_batch() { return 4; }
 
 // Sums each wavefront's hit and head counts per workgroup into
 // part[2 * blockIdx.x + {0, 1}] (no atomics: many workgroups adding to one
@@ -741,27 +739,46 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
         if constexpr (kExact) return ex.fkey[c];
         else return (uint64_t)c;
     };
-    auto row_of = [ex, design, R](uint64_t h, int col0, V (&d)[VW]) {
+    auto rows_of = [ex, design, R](const auto& h, int col0, auto& d) {
+        constexpr int NB = sizeof(h) / sizeof(h[0]);
+        constexpr int VW = group_vw<V>();
         if constexpr (kExact) {
-            // product of the fiber's factor rows in mode order
+            // product of each fiber's factor rows in mode order; per mode the
+            // loads of all NB rows are issued before the first product
+            // (unconditional: column 0 in place of columns >= R, whose
+            // entries are zeroed at the end)
 #pragma unroll
-            for (int v = 0; v < VW; v++) d[v] = (V)1;
+            for (int b = 0; b < NB; b++)
+#pragma unroll
+                for (int v = 0; v < VW; v++) d[b][v] = (V)1;
 #pragma unroll
             for (int k = 0; k < kMaxModes; k++) {
                 if (k < ex.N && ex.shift[k] >= 0) {
-                    const V* u = ex.U[k] + (int64_t)((h >> ex.shift[k]) & ex.mask[k]) * R;
+                    V u[NB][VW];
 #pragma unroll
-                    for (int v = 0; v < VW; v++)
-                        if (col0 + v < R) d[v] *= u[col0 + v];
+                    for (int b = 0; b < NB; b++) {
+                        const V* row = ex.U[k] + (int64_t)((h[b] >> ex.shift[k]) & ex.mask[k]) * R;
+#pragma unroll
+                        for (int v = 0; v < VW; v++) u[b][v] = row[col0 + v < R ? col0 + v : 0];
+                    }
+#pragma unroll
+                    for (int b = 0; b < NB; b++)
+#pragma unroll
+                        for (int v = 0; v < VW; v++) d[b][v] *= u[b][v];
                 }
             }
 #pragma unroll
-            for (int v = 0; v < VW; v++)
-                if (col0 + v >= R) d[v] = (V)0;
-        } else {
-            const V* u = design + (int64_t)h * R;
+            for (int b = 0; b < NB; b++)
 #pragma unroll
-            for (int v = 0; v < VW; v++) d[v] = col0 + v < R ? u[col0 + v] : (V)0;
+                for (int v = 0; v < VW; v++)
+                    if (col0 + v >= R) d[b][v] = (V)0;
+        } else {
+#pragma unroll
+            for (int b = 0; b < NB; b++) {
+                const V* u = design + (int64_t)h[b] * R;
+#pragma unroll
+                for (int v = 0; v < VW; v++) d[b][v] = col0 + v < R ? u[col0 + v] : (V)0;
+            }
         }
     };
     auto meta = [&](int64_t k, uint32_t& r, uint32_t& c, V& x) {
@@ -838,7 +855,7 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
             nh += (uint32_t)__popcll(mask);
         }
         wave_lds_sync();
-        if (nh > 0) st.template run<step_batch<V, kExact>()>(0, nh, meta, pre_of, row_of);
+        if (nh > 0) st.template run<step_batch<V, kExact>()>(0, nh, meta, pre_of, rows_of);
         st.finish(cflag, ckey);
         wave_lds_sync();
         whits += nh;
@@ -1052,10 +1069,15 @@ __global__ __launch_bounds__(kBitThreads) void hit_scan_reduce(const uint32_t* _
     const int64_t units = nuh < nu ? nuh : nu;
     unsigned long long whits = 0, wheads = 0;
     auto pre_of = [](uint32_t c) -> uint64_t { return (uint64_t)c; };
-    auto row_of = [design, R](uint64_t h, int col0, V (&d)[VW]) {
-        const V* u = design + (int64_t)h * R;
+    auto rows_of = [design, R](const auto& h, int col0, auto& d) {
+        constexpr int NB = sizeof(h) / sizeof(h[0]);
+        constexpr int VW = group_vw<V>();
 #pragma unroll
-        for (int v = 0; v < VW; v++) d[v] = col0 + v < R ? u[col0 + v] : (V)0;
+        for (int b = 0; b < NB; b++) {
+            const V* u = design + (int64_t)h[b] * R;
+#pragma unroll
+            for (int v = 0; v < VW; v++) d[b][v] = col0 + v < R ? u[col0 + v] : (V)0;
+        }
     };
     auto meta = [&](int64_t k, uint32_t& r, uint32_t& c, V& x) {
         const uint32_t p = hits[k];
@@ -1073,7 +1095,7 @@ __global__ __launch_bounds__(kBitThreads) void hit_scan_reduce(const uint32_t* _
         st.last_row = rrow[hits[e - 1]];
         st.first_cont = b > 0 && rrow[hits[b - 1]] == st.first_row;
         st.continues = e < H && rrow[hits[e]] == st.last_row;
-        st.template run<step_batch<V, false>()>(b, e, meta, pre_of, row_of);
+        st.template run<step_batch<V, false>()>(b, e, meta, pre_of, rows_of);
         st.finish(cflag, ckey);
         whits += (unsigned long long)(e - b);
         wheads += st.nheads;
@@ -1321,10 +1343,15 @@ __global__ __launch_bounds__(kBkThreads) void bk_reduce(const uint32_t* __restri
     const uint32_t r0 = base[blockIdx.x], nb = btot[blockIdx.x];
     unsigned long long whits = 0, wheads = 0;
     auto pre_of = [](uint32_t c) -> uint64_t { return (uint64_t)c; };
-    auto row_of = [design, R](uint64_t h, int col0, V (&d)[VW]) {
-        const V* u = design + (int64_t)h * R;
+    auto rows_of = [design, R](const auto& h, int col0, auto& d) {
+        constexpr int NB = sizeof(h) / sizeof(h[0]);
+        constexpr int VW = group_vw<V>();
 #pragma unroll
-        for (int v = 0; v < VW; v++) d[v] = col0 + v < R ? u[col0 + v] : (V)0;
+        for (int b = 0; b < NB; b++) {
+            const V* u = design + (int64_t)h[b] * R;
+#pragma unroll
+            for (int v = 0; v < VW; v++) d[b][v] = col0 + v < R ? u[col0 + v] : (V)0;
+        }
     };
     for (int64_t ws = pb; ws < pe; ws += kBkWin) {
         for (int i = threadIdx.x; i < kBkWin; i += kBkThreads) slot[i] = ~0u;
@@ -1363,7 +1390,7 @@ __global__ __launch_bounds__(kBkThreads) void bk_reduce(const uint32_t* __restri
                 c = h.c;
                 x = h.x;
             };
-            if (nh > 0) st.template run<step_batch<V, false>()>(0, nh, meta, pre_of, row_of);
+            if (nh > 0) st.template run<step_batch<V, false>()>(0, nh, meta, pre_of, rows_of);
             st.finish(cflag, ckey);
             whits += nh;
             wheads += st.nheads;

@@ -36,8 +36,8 @@ def rel_err(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-@pytest.mark.parametrize("name,variant,R", [("sts_draw.fp64.r16", v, 16) for v in range(3)] +
-                         [("sts_draw.fp64.r32", v, 32) for v in range(4)])
+@pytest.mark.parametrize("name,variant,R", [("sts_draw.fp64.r16", v, 16) for v in range(4)] +
+                         [("sts_draw.fp64.r32", v, 32) for v in range(5)])
 def test_fp64_draw_variants_bitexact(ctx, orc, knob, name, variant, R):
     from rdist_tensor_amd import abi
     knob(name, variant)
@@ -64,9 +64,9 @@ def test_fp64_draw_variants_bitexact(ctx, orc, knob, name, variant, R):
         assert rel_err(wg[ok], wo[ok]) < 1e-11
 
 
-@pytest.mark.parametrize("name,variant,R", [("sts_draw.fp32.r16", v, 16) for v in range(3)] +
-                         [("sts_draw.fp32.r32", v, 32) for v in range(4)] +
-                         [("sts_draw.fp32.r64", v, 64) for v in range(2)] +
+@pytest.mark.parametrize("name,variant,R", [("sts_draw.fp32.r16", v, 16) for v in range(4)] +
+                         [("sts_draw.fp32.r32", v, 32) for v in range(5)] +
+                         [("sts_draw.fp32.r64", v, 64) for v in range(3)] +
                          [("sts_first.fp32", v, R) for v in range(2) for R in (16, 32, 64)])
 def test_mixed_draw_variants_distribution(ctx, knob, name, variant, R):
     from rdist_tensor_amd import abi
