@@ -1419,7 +1419,7 @@ int ex_sampler_create(ex_ctx* ctx, ex_factors* f, int kind, int precision, uint6
             s->lam.ensure((size_t)(N + 1) * R * 8);
             s->eig.ensure((size_t)N * s->ge.node_count * S * 8);
             if (precision == EX_MIXED) s->eig32.ensure((size_t)N * s->ge.node_count * tri_stride32(R) * 4);
-            s->pinvp.ensure((size_t)S * 8);
+            s->pinvp.ensure((size_t)tri_pad64(R) * 8);
         }
         for (int k2 = 0; k2 < N; k2++) sampler_update_impl(s.get(), k2);
         EX_HIP(hipStreamSynchronize(ctx->c.stream));

@@ -75,6 +75,9 @@ __host__ __device__ inline int tri_index(int R, int a, int b) { return a * R - (
 __host__ __device__ inline int tri_stride(int R) { return (tri_size(R) + 15) & ~15; }
 // FP32 node stride: packed size rounded up to 32 elements (128 B).
 __host__ __device__ inline int tri_stride32(int R) { return (tri_size(R) + 31) & ~31; }
+// Packed size rounded up to a multiple of 64 (a weight table read G lanes at a
+// time, G <= 64, with no bound check).
+__host__ __device__ inline int tri_pad64(int R) { return (tri_size(R) + 63) & ~63; }
 
 // Heap-ordered segment tree over ceil(n/F) leaves (PartitionTree geometry,
 // partition_tree.hpp:82-115).
@@ -215,6 +218,8 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
 // totals, cdf[R L + v NT + q]; the last is v's total.  R <= 64; cdf holds
 // R (L + NT) doubles.
 void sts_first_cdf(hipStream_t s, const StsModeArgs& m, int R, double* cdf);
+// packed[e] = (1 on the diagonal, 2 off it) x M[a][b] for the packed upper
+// element e = (a, b), zeros up to tri_pad64(R) entries (the draw's weight table).
 void pack_upper(hipStream_t s, const double* M, int R, double* packed);
 // sparse
 void keys_from_coords(hipStream_t s, const uint32_t* coords, int64_t n, const KeyLayout& kl, uint64_t* keys,

@@ -430,13 +430,20 @@ __global__ void sum_final(const double* __restrict__ part, int nparts, double* _
     *out = s;
 }
 
+// packed[e] = w M[a][b] for the packed upper element e = (a, b), w = 1 on
+// the diagonal and 2 off it (h^T M h = sum_e packed[e] h_a h_b), zero for e
+// in [tri_size(R), tri_pad64(R)).
 __global__ void pack_upper_kernel(const double* __restrict__ M, int R, double* __restrict__ packed) {
     const int P = tri_size(R);
-    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+    for (int e = threadIdx.x; e < tri_pad64(R); e += blockDim.x) {
+        if (e >= P) {
+            packed[e] = 0.0;
+            continue;
+        }
         int a = 0;
         while (a + 1 < R && tri_index(R, a + 1, a + 1) <= e) a++;
         int b = a + (e - tri_index(R, a, a));
-        packed[e] = M[a * R + b];
+        packed[e] = (a == b ? 1.0 : 2.0) * M[a * R + b];
     }
 }
 

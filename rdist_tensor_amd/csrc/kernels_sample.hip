@@ -21,6 +21,18 @@ __device__ __forceinline__ void tri_ab(int R, int e, int& a, int& b) {
     b = a + (e - tri_index(R, a, a));
 }
 
+// The same (a, b) from the root of tri_index(R, a, a) = e, corrected by one
+// step either way (the FP32 estimate is within 1 of the row for R <= 64).
+__device__ __forceinline__ void tri_ab_direct(int R, int e, int& a, int& b) {
+    const float d = (float)(2 * R + 1);
+    int r = (int)((d - sqrtf(d * d - 8.0f * (float)e)) * 0.5f);
+    r = r < 0 ? 0 : (r > R - 1 ? R - 1 : r);
+    if (r + 1 < R && tri_index(R, r + 1, r + 1) <= e) r++;
+    if (tri_index(R, r, r) > e) r--;
+    a = r;
+    b = r + (e - tri_index(R, r, r));
+}
+
 // ---------------------------------------------------------------- row trees
 // Leaf Grams: one wavefront per leaf; the leaf's F = R rows are staged in LDS
 // and each lane produces its packed-upper elements, summing the rows' products
@@ -135,7 +147,8 @@ struct StsArgs {
     k::StsModeArgs m[kMaxModes];
     int N, j, R;
     int64_t J;
-    const double* pinvp;    // packed upper of M[0] = pinv of the Hadamard product (FP64)
+    const double* pinvp;    // M[0] = pinv of the Hadamard product, packed upper with off-diagonal
+                            // entries doubled and zeros up to tri_pad64(R) entries (FP64)
     uint64_t seed, sweep;
     const uint64_t* sweep_dev;   // when set, the sweep number is read here (graph replays)
     const double* draws_row;
@@ -192,7 +205,7 @@ template <> struct Vec<double> {
 // banks; with RP = RMAX (a power of two) every other group shares banks.
 template <typename TN, int SLOTS, int RMAX, int RP>
 struct DrawShared {
-    uint16_t pairs[RMAX * (RMAX + 1) / 2];
+    uint16_t pairs[(RMAX * (RMAX + 1) / 2 + 63) & ~63];   // (0, 0) past the triangle
     TN h[SLOTS][RP];
     TN x[SLOTS][RP];
     double q2[SLOTS][RP];
@@ -529,9 +542,9 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
     const int R = a.R, N = a.N, j = a.j;
     const int P = tri_size(R);
     const int S = sizeof(TN) == 8 ? tri_stride(R) : tri_stride32(R);
-    for (int e = threadIdx.x; e < P; e += blockDim.x) {
-        int ea, eb;
-        tri_ab(R, e, ea, eb);
+    for (int e = threadIdx.x; e < tri_pad64(R); e += blockDim.x) {
+        int ea = 0, eb = 0;
+        if (e < P) tri_ab_direct(R, e, ea, eb);
         sh.pairs[e] = (uint16_t)(ea | (eb << 8));
     }
     // the first non-target mode: prefix sums instead of descents when given
@@ -644,18 +657,15 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
     }
     if (!active) return;
     // weight = R / (J * h^T pinv h), FP64
-    // (unrolled, unconditional loads: element 0 in place of those past P)
+    // (the table's diagonal factors are folded in and its padding is zero,
+    // so the G-strided terms need no bound check)
     double q = 0.0;
     const int ksteps = (P + G - 1) / G;
 #pragma unroll 8
     for (int t = 0; t < ksteps; t++) {
         const int e = t * G + l;
-        const bool ok = e < P;
-        const double pv = a.pinvp[ok ? e : 0];
-        const uint32_t pr = sh.pairs[ok ? e : 0];
-        const int ea = pr & 0xFF, eb = pr >> 8;
-        const double term = pv * ((ea == eb ? 1.0 : 2.0) * ((double)hs[ea] * (double)hs[eb]));
-        q = ok ? q + term : q;
+        const uint32_t pr = sh.pairs[e];
+        q += a.pinvp[e] * ((double)hs[pr & 0xFF] * (double)hs[pr >> 8]);
     }
     q = group_total<G>(q);
     if (l == 0) {
