@@ -483,6 +483,7 @@ struct GroupRows {
     V* carry;
     uint32_t cur = 0, nheads = 0, f0 = 0, f1 = 0;
     bool active = false;
+    bool vst = false;   // rows written VW at a time
     V acc[VW];
 
     __device__ void init(int lane_, int R_, int64_t w_, V* Y_, V* carry_) {
@@ -493,6 +494,7 @@ struct GroupRows {
         carry = carry_;
         g = lane / G;
         sub = lane % G;
+        vst = R % VW == 0 && ((uintptr_t)Y % (sizeof(V) * VW)) == 0 && ((uintptr_t)carry % (sizeof(V) * VW)) == 0;
 #pragma unroll
         for (int v = 0; v < VW; v++) acc[v] = (V)0;
     }
@@ -501,6 +503,17 @@ struct GroupRows {
     // row running in from the left)
     __device__ void write_row(uint32_t r, const V (&x)[VW]) {
         V* dst = head_row(r) ? Y + (int64_t)r * R : carry + (2 * w) * R;
+        if constexpr (sizeof(V) == 4) {
+            if (vst) {
+                // one vector store (rows of Y and carry are VW-aligned)
+                typedef V VT __attribute__((ext_vector_type(VW)));
+                VT y;
+#pragma unroll
+                for (int v = 0; v < VW; v++) y[v] = x[v];
+                if (sub * VW < R) *reinterpret_cast<VT*>(dst + sub * VW) = y;
+                return;
+            }
+        }
 #pragma unroll
         for (int v = 0; v < VW; v++) {
             const int col = sub * VW + v;
@@ -629,6 +642,45 @@ struct GroupRows {
 template <typename V>
 constexpr int group_vw() { return sizeof(V) == 4 ? 4 : 2; }
 
+// Columns col0 .. col0 + VW - 1 of NB rows (rows[b] + h[b] R), zero at
+// columns >= R, read without a branch per row: one VW-wide load per row when
+// `vec` (R a multiple of VW, rows aligned to it), else VW element loads;
+// column 0 stands in for the columns past R.
+template <typename V, int VW, int NB>
+__device__ __forceinline__ void rows_cols(const V* __restrict__ base, const int64_t (&off)[NB], int col0, int R,
+                                          bool vec, V (&d)[NB][VW]) {
+    typedef V VT __attribute__((ext_vector_type(VW)));
+    if constexpr (sizeof(V) == 8) {
+        // FP64 (measured faster this way): element loads behind their bounds
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int v = 0; v < VW; v++) d[b][v] = col0 + v < R ? base[off[b] + col0 + v] : (V)0;
+    } else if (vec) {
+        VT x[NB];
+#pragma unroll
+        for (int b = 0; b < NB; b++) x[b] = *reinterpret_cast<const VT*>(base + off[b] + (col0 < R ? col0 : 0));
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int v = 0; v < VW; v++) d[b][v] = col0 < R ? x[b][v] : (V)0;
+    } else {
+#pragma unroll
+        for (int b = 0; b < NB; b++)
+#pragma unroll
+            for (int v = 0; v < VW; v++) {
+                const V x = base[off[b] + (col0 + v < R ? col0 + v : 0)];
+                d[b][v] = col0 + v < R ? x : (V)0;
+            }
+    }
+}
+
+// Rows of R elements at p can be read VW at a time.
+template <typename V, int VW>
+__device__ __forceinline__ bool rows_vec(const V* p, int R) {
+    return R % VW == 0 && ((uintptr_t)p % (sizeof(V) * VW)) == 0;
+}
+
 // Grouped steps whose loads are issued together (4: more cost registers;
 // the exact MTTKRP issues each mode's factor rows of all 4 at once).
 template <typename V, bool kExact>
@@ -727,9 +779,10 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
         const int64_t b2 = w2 * kScanChunk, e2 = b2 + kScanChunk;
 #pragma unroll
         for (int m = 0; m < M; m++) {
-            const int64_t i = b2 + 64 * m + lane;
-            nrow[m] = i < nnz ? rrow[i] : 0u;
-            nfib[m] = i < nnz ? rfib[i] : 0u;
+            // (the last entry in place of those past nnz, which no hit uses)
+            const int64_t i = b2 + 64 * m + lane < nnz ? b2 + 64 * m + lane : nnz - 1;
+            nrow[m] = rrow[i];
+            nfib[m] = rfib[i];
         }
         nprev = b2 > 0 ? rrow[b2 - 1] : 0u;
         nnext = e2 < nnz ? rrow[e2] : 0u;
@@ -745,8 +798,6 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
         if constexpr (kExact) {
             // product of each fiber's factor rows in mode order; per mode the
             // loads of all NB rows are issued before the first product
-            // (unconditional: column 0 in place of columns >= R, whose
-            // entries are zeroed at the end)
 #pragma unroll
             for (int b = 0; b < NB; b++)
 #pragma unroll
@@ -755,11 +806,18 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
             for (int k = 0; k < kMaxModes; k++) {
                 if (k < ex.N && ex.shift[k] >= 0) {
                     V u[NB][VW];
+                    int64_t off[NB];
 #pragma unroll
-                    for (int b = 0; b < NB; b++) {
-                        const V* row = ex.U[k] + (int64_t)((h[b] >> ex.shift[k]) & ex.mask[k]) * R;
+                    for (int b = 0; b < NB; b++) off[b] = (int64_t)((h[b] >> ex.shift[k]) & ex.mask[k]) * R;
+                    if constexpr (sizeof(V) == 4) {
+                        rows_cols<V, VW, NB>(ex.U[k], off, col0, R, rows_vec<V, VW>(ex.U[k], R), u);
+                    } else {
+                        // FP64: element loads, column 0 in place of columns
+                        // >= R (zeroed at the end)
 #pragma unroll
-                        for (int v = 0; v < VW; v++) u[b][v] = row[col0 + v < R ? col0 + v : 0];
+                        for (int b = 0; b < NB; b++)
+#pragma unroll
+                            for (int v = 0; v < VW; v++) u[b][v] = ex.U[k][off[b] + (col0 + v < R ? col0 + v : 0)];
                     }
 #pragma unroll
                     for (int b = 0; b < NB; b++)
@@ -773,12 +831,10 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
                 for (int v = 0; v < VW; v++)
                     if (col0 + v >= R) d[b][v] = (V)0;
         } else {
+            int64_t off[NB];
 #pragma unroll
-            for (int b = 0; b < NB; b++) {
-                const V* u = design + (int64_t)h[b] * R;
-#pragma unroll
-                for (int v = 0; v < VW; v++) d[b][v] = col0 + v < R ? u[col0 + v] : (V)0;
-            }
+            for (int b = 0; b < NB; b++) off[b] = (int64_t)h[b] * R;
+            rows_cols<V, VW, NB>(design, off, col0, R, rows_vec<V, VW>(design, R), d);
         }
     };
     auto meta = [&](int64_t k, uint32_t& r, uint32_t& c, V& x) {
@@ -806,7 +862,8 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
                 const int64_t i = b + 64 * m + lane;
                 hit[m] = i < e;
                 hc[m] = fr[m];
-                hx[m] = hit[m] ? rval[i] : (V)0;
+                const V x = rval[hit[m] ? i : b];
+                hx[m] = hit[m] ? x : (V)0;
             }
         } else {
             // filter probes first, then every map entry and value of the
@@ -819,11 +876,22 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
                 const uint32_t h1 = filter_h1(fr[m]), h2 = filter_h2(fr[m]);
                 pos[m] = i < e && ((lfilt[h1 >> 5] >> (h1 & 31)) & (lfilt[h2 >> 5] >> (h2 & 31)) & 1u);
             }
+            // FP32: every lane loads, a negative one the map entry of fiber 0
+            // and the chunk's first value (one shared line each), so that no
+            // load sits behind a branch (FP64 measured faster with the loads
+            // behind their lanes' probes)
 #pragma unroll
             for (int m = 0; m < M; m++) {
                 const int64_t i = b + 64 * m + lane;
-                cm[m] = pos[m] ? cmap[fr[m]] : 0ull;
-                hx[m] = pos[m] ? rval[i] : (V)0;
+                if constexpr (sizeof(V) == 4) {
+                    const uint64_t c = cmap[pos[m] ? fr[m] : 0u];
+                    const V x = rval[pos[m] ? i : b];
+                    cm[m] = pos[m] ? c : 0ull;
+                    hx[m] = pos[m] ? x : (V)0;
+                } else {
+                    cm[m] = pos[m] ? cmap[fr[m]] : 0ull;
+                    hx[m] = pos[m] ? rval[i] : (V)0;
+                }
             }
 #pragma unroll
             for (int m = 0; m < M; m++) {
@@ -1072,12 +1140,10 @@ __global__ __launch_bounds__(kBitThreads) void hit_scan_reduce(const uint32_t* _
     auto rows_of = [design, R](const auto& h, int col0, auto& d) {
         constexpr int NB = sizeof(h) / sizeof(h[0]);
         constexpr int VW = group_vw<V>();
+        int64_t off[NB];
 #pragma unroll
-        for (int b = 0; b < NB; b++) {
-            const V* u = design + (int64_t)h[b] * R;
-#pragma unroll
-            for (int v = 0; v < VW; v++) d[b][v] = col0 + v < R ? u[col0 + v] : (V)0;
-        }
+        for (int b = 0; b < NB; b++) off[b] = (int64_t)h[b] * R;
+        rows_cols<V, VW, NB>(design, off, col0, R, rows_vec<V, VW>(design, R), d);
     };
     auto meta = [&](int64_t k, uint32_t& r, uint32_t& c, V& x) {
         const uint32_t p = hits[k];
@@ -1346,12 +1412,10 @@ __global__ __launch_bounds__(kBkThreads) void bk_reduce(const uint32_t* __restri
     auto rows_of = [design, R](const auto& h, int col0, auto& d) {
         constexpr int NB = sizeof(h) / sizeof(h[0]);
         constexpr int VW = group_vw<V>();
+        int64_t off[NB];
 #pragma unroll
-        for (int b = 0; b < NB; b++) {
-            const V* u = design + (int64_t)h[b] * R;
-#pragma unroll
-            for (int v = 0; v < VW; v++) d[b][v] = col0 + v < R ? u[col0 + v] : (V)0;
-        }
+        for (int b = 0; b < NB; b++) off[b] = (int64_t)h[b] * R;
+        rows_cols<V, VW, NB>(design, off, col0, R, rows_vec<V, VW>(design, R), d);
     };
     for (int64_t ws = pb; ws < pe; ws += kBkWin) {
         for (int i = threadIdx.x; i < kBkWin; i += kBkThreads) slot[i] = ~0u;
