@@ -381,28 +381,49 @@ constexpr int kFilterWords = kFilterBits / 32;
 __device__ __forceinline__ uint32_t filter_h1(uint32_t f) { return f & (kFilterBits - 1); }
 __device__ __forceinline__ uint32_t filter_h2(uint32_t f) { return (f * 2654435761u) >> (32 - 19); }
 
+// Wavefront-cooperative 64-ary search of key in the ascending fkey[0, n):
+// returns its index or -1.
+__device__ __forceinline__ int64_t wave_find_key(const uint64_t* __restrict__ fkey, int64_t n, uint64_t key,
+                                                 int lane) {
+    int64_t lo = 0, hi = n;   // fkey[lo] <= key once a probe has succeeded
+    while (hi - lo > 64) {
+        const int64_t stride = (hi - lo + 63) / 64;
+        const int64_t at = lo + (int64_t)lane * stride;
+        const bool le = at < hi && fkey[at] <= key;
+        const int cnt = __popcll(__ballot(le));
+        if (cnt == 0) return -1;   // key < fkey[lo]
+        lo = lo + (int64_t)(cnt - 1) * stride;
+        const int64_t h2 = lo + stride;
+        hi = h2 < hi ? h2 : hi;
+    }
+    const int64_t at = lo + lane;
+    const uint64_t hitmask = __ballot(at < hi && fkey[at] == key);
+    return hitmask ? lo + __ffsll((unsigned long long)hitmask) - 1 : -1;
+}
+
 __global__ __launch_bounds__(kThreads) void mark_fibers(const uint64_t* __restrict__ kd, int64_t Jb,
                                                         const uint64_t* __restrict__ Ju_dev,
                                                         const uint64_t* __restrict__ fkey, int64_t nfib,
                                                         uint64_t* __restrict__ cmap, uint32_t* __restrict__ filt,
                                                         uint32_t* __restrict__ fsel) {
     const int64_t Ju = rows_of(Jb, Ju_dev);
-    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < Ju; c += (int64_t)gridDim.x * blockDim.x) {
-        const uint64_t key = kd[c];
-        int64_t lo = 0, hi = nfib;
-        while (lo < hi) {
-            const int64_t mid = lo + (hi - lo) / 2;
-            if (fkey[mid] < key) lo = mid + 1; else hi = mid;
+    const int lane = threadIdx.x % 64;
+    // one wavefront per sample: a 64-ary search (a few dependent probes
+    // instead of one per bit of nfib)
+    for (int64_t c = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / 64; c < Ju;
+         c += (int64_t)gridDim.x * blockDim.x / 64) {
+        const int64_t at = wave_find_key(fkey, nfib, kd[c], lane);
+        if (lane == 0) {
+            uint32_t f = ~0u;
+            if (at >= 0) {
+                f = (uint32_t)at;
+                const uint32_t h1 = filter_h1(f), h2 = filter_h2(f);
+                atomicOr(filt + (h1 >> 5), 1u << (h1 & 31));
+                atomicOr(filt + (h2 >> 5), 1u << (h2 & 31));
+                cmap[f] = ((uint64_t)1 << 32) | (uint32_t)c;
+            }
+            fsel[c] = f;
         }
-        uint32_t f = ~0u;
-        if (lo < nfib && fkey[lo] == key) {
-            f = (uint32_t)lo;
-            const uint32_t h1 = filter_h1(f), h2 = filter_h2(f);
-            atomicOr(filt + (h1 >> 5), 1u << (h1 & 31));
-            atomicOr(filt + (h2 >> 5), 1u << (h2 & 31));
-            cmap[f] = ((uint64_t)1 << 32) | (uint32_t)c;
-        }
-        fsel[c] = f;
     }
 }
 
@@ -445,6 +466,9 @@ __global__ __launch_bounds__(kThreads) void zero_fill(V* __restrict__ y, int64_t
 // partials of the hit and head counts go to `counts` (sum_counts adds them).
 constexpr int kScanChunk = 256;
 constexpr int kSegBatch = 32;  // carry entries whose loads are issued together
+// Carry entries per wavefront of a reduction level: short chunks keep each
+// wavefront's chain of dependent load batches short (two batches at most).
+constexpr int kSegChunk = 64;
 
 // The filter (64 KB) and the chunk's hit lists sit in LDS; one 1024-thread
 // workgroup per CU.
@@ -940,25 +964,6 @@ __global__ __launch_bounds__(kScanThreads) void row_scan_reduce(const uint32_t* 
 // proportional to the hits plus one byte per nonzero instead of every
 // nonzero's row and fiber number.
 
-// Wavefront-cooperative 64-ary search of key in the ascending fkey[0, n):
-// returns its index or -1.
-__device__ __forceinline__ int64_t wave_find_key(const uint64_t* __restrict__ fkey, int64_t n, uint64_t key,
-                                                 int lane) {
-    int64_t lo = 0, hi = n;   // fkey[lo] <= key once a probe has succeeded
-    while (hi - lo > 64) {
-        const int64_t stride = (hi - lo + 63) / 64;
-        const int64_t at = lo + (int64_t)lane * stride;
-        const bool le = at < hi && fkey[at] <= key;
-        const int cnt = __popcll(__ballot(le));
-        if (cnt == 0) return -1;   // key < fkey[lo]
-        lo = lo + (int64_t)(cnt - 1) * stride;
-        const int64_t h2 = lo + stride;
-        hi = h2 < hi ? h2 : hi;
-    }
-    const int64_t at = lo + lane;
-    const uint64_t hitmask = __ballot(at < hi && fkey[at] == key);
-    return hitmask ? lo + __ffsll((unsigned long long)hitmask) - 1 : -1;
-}
 
 // Nonzeros marked per expansion block.
 constexpr int kExpandBlock = 1024;
@@ -1479,12 +1484,12 @@ __global__ __launch_bounds__(kThreads) void bk_keyorder(const uint32_t* __restri
 }
 
 // One level of the carry reduction: entries (key, R-vector, flag) sorted by
-// key, kScanChunk entries per wavefront (four per lane), flagged entries
+// key, kSegChunk entries per wavefront (one per lane), flagged entries
 // summed in order per key.  A key whose flagged entries all lie in the
 // chunk and that does not cross the chunk boundary is final and written to
 // Y; a crossing key leaves its partial in the next level's slot 0 (running
 // in from the left) or slot 1 (running out to the right), with the same
-// layout as the row pass.  With n <= kScanChunk every key is final.
+// layout as the row pass.  With n <= kSegChunk every key is final.
 // Entries of carry level `level` (1 = the list the hit pass writes) when the
 // first list came from the device hit count H: 2 per unit of kHitUnit
 // hits, then 2 per chunk of the previous level while it spans more than one
@@ -1493,7 +1498,7 @@ __device__ __forceinline__ int64_t carry_count(const uint64_t* H_dev, int level,
     if (!H_dev) return n_host;
     int64_t n = 2 * (((int64_t)*H_dev + kHitUnit - 1) / kHitUnit);
     for (int l = 1; l < level; l++) {
-        const int64_t ch = (n + kScanChunk - 1) / kScanChunk;
+        const int64_t ch = (n + kSegChunk - 1) / kSegChunk;
         n = ch > 1 ? 2 * ch : 0;
     }
     return n < n_host ? n : n_host;
@@ -1507,12 +1512,12 @@ __global__ __launch_bounds__(kThreads) void seg_level(const uint32_t* __restrict
                                                       uint32_t* __restrict__ out_flag, uint32_t* __restrict__ out_key) {
     const int lane = threadIdx.x % 64;
     const int lr = lane < R ? lane : 0;
-    constexpr int M = kScanChunk / 64;
+    constexpr int M = kSegChunk / 64;
     const int64_t n = carry_count(H_dev, level, n_host);
-    const int64_t nch = (n + kScanChunk - 1) / kScanChunk;
+    const int64_t nch = (n + kSegChunk - 1) / kSegChunk;
     for (int64_t w = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64; w < nch;
          w += (int64_t)gridDim.x * (kThreads / 64)) {
-        const int64_t b = w * kScanChunk, e = b + kScanChunk < n ? b + kScanChunk : n;
+        const int64_t b = w * kSegChunk, e = b + kSegChunk < n ? b + kSegChunk : n;
         uint32_t kk[M];
         bool fl[M];
 #pragma unroll
@@ -1961,7 +1966,7 @@ template <typename V>
 void carry_reserve(DevBuf* tmp, int64_t nch, int R) {
     const int64_t n1 = 2 * nch;
     tmp[2].ensure(carry_list_bytes<V>(n1, R));
-    tmp[3].ensure(carry_list_bytes<V>(2 * ((n1 + kScanChunk - 1) / kScanChunk), R));
+    tmp[3].ensure(carry_list_bytes<V>(2 * ((n1 + kSegChunk - 1) / kSegChunk), R));
 }
 
 // Reduces the level-1 carry list (n1 entries in tmp[2]) level by level until
@@ -1975,7 +1980,7 @@ void carry_reduce(hipStream_t s, DevBuf* tmp, int64_t n1, int R, V* Y, const uin
     int cur = 2;
     int level = 1;
     while (true) {
-        const int64_t ch = (n + kScanChunk - 1) / kScanChunk;
+        const int64_t ch = (n + kSegChunk - 1) / kSegChunk;
         V* ovec = nullptr;
         uint32_t *oflag = nullptr, *okey = nullptr;
         if (ch > 1) carry_views(tmp[5 - cur], 2 * ch, R, ovec, oflag, okey);
@@ -1986,7 +1991,7 @@ void carry_reduce(hipStream_t s, DevBuf* tmp, int64_t n1, int R, V* Y, const uin
         n = 2 * ch;
         cur = 5 - cur;
         carry_views(tmp[cur], n, R, vec, flag, key);
-        const size_t need = carry_list_bytes<V>(2 * ((n + kScanChunk - 1) / kScanChunk), R);
+        const size_t need = carry_list_bytes<V>(2 * ((n + kSegChunk - 1) / kSegChunk), R);
         if (tmp[5 - cur].bytes < need) tmp[5 - cur].ensure(need);
     }
 }
@@ -2197,7 +2202,7 @@ void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t*
         V* vec;
         uint32_t *flag, *key;
         carry_views(tmp[2], 2 * nch, R, vec, flag, key);
-        hipLaunchKernelGGL(mark_fibers, dim3(grid_for(Ju, kThreads, 16384)), dim3(kThreads), 0, s, kd, Ju, Ju_dev,
+        hipLaunchKernelGGL(mark_fibers, dim3(grid_for(Ju, kThreads / 64, 65536)), dim3(kThreads), 0, s, kd, Ju, Ju_dev,
                            L.fkey.as<uint64_t>(), L.nfib, L.cmap.as<uint64_t>(), L.filt.as<uint32_t>(), fsel);
         const dim3 scan_grid(grid_for(nch, kScanThreads / 64, num_cus(s)));   // one workgroup per CU (LDS)
         unsigned long long* part = count_parts(L, counts, scan_grid.x);
