@@ -176,6 +176,7 @@ struct ex_ctx {
     // (the design Gram and its inverse run beside the sampled MTTKRP)
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_zfork = nullptr, ev_zjoin = nullptr;   // Y cleared on aux beside the draw
     // opt-in per-launch-group timing (ex_ctx_timing): events are recorded
     // around named launch groups on the context's stream and resolved after
     // the stream is synchronised
@@ -579,6 +580,16 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
         if (regions) EX_HIP(hipEventRecord(ev[q], st));
     };
     mark(0);
+    // Y is cleared on the second stream while the draw runs (the previous
+    // step's factor update, its last reader, is before this point on st)
+    const bool mixed = a->precision == EX_MIXED;
+    const size_t vb = mixed ? 4 : 8;   // bytes per design / MTTKRP value
+    a->Y.ensure((size_t)Ij * R * vb);
+    hipStream_t aux = a->ctx->aux;
+    EX_HIP(hipEventRecord(a->ctx->ev_zfork, st));
+    EX_HIP(hipStreamWaitEvent(aux, a->ctx->ev_zfork, 0));
+    k::zero_bytes(aux, a->Y.p, (size_t)Ij * R * vb);
+    EX_HIP(hipEventRecord(a->ctx->ev_zjoin, aux));
     // Region 1: draw samples (accumulator_stationary.hpp:148-153)
     s->samples.ensure((size_t)J * N * 4);
     s->weights.ensure((size_t)J * 8);
@@ -600,8 +611,6 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
         k::dedup(st, s->samples.as<uint32_t>(), s->weights.as<double>(), J, N, kl, a->sd.as<uint32_t>(),
                  a->wd.as<double>(), a->kd.as<uint64_t>(), a->tmp, cnt + 3, cnt + 2);
     const int64_t Ju = J;
-    const bool mixed = a->precision == EX_MIXED;
-    const size_t vb = mixed ? 4 : 8;   // bytes per design / MTTKRP value
     a->dg.ensure((size_t)J * R * vb);
     a->dm.ensure((size_t)J * R * vb);
     a->G.ensure((size_t)R * R * 8);
@@ -622,7 +631,6 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     // the design Gram and its inverse run on the second stream beside the
     // sampled MTTKRP (which needs only the design rows); the factor update
     // waits for them
-    hipStream_t aux = a->ctx->aux;
     EX_HIP(hipEventRecord(a->ctx->ev_fork, st));
     EX_HIP(hipStreamWaitEvent(aux, a->ctx->ev_fork, 0));
     Span s_dp(a->ctx, "design_gram_pinv", aux);
@@ -639,7 +647,6 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     EX_HIP(hipEventRecord(a->ctx->ev_join, aux));
     mark(2);
     // Region 4: sampled MTTKRP (:252-274)
-    a->Y.ensure((size_t)Ij * R * vb);
     ModeLookup& L = a->t->lk[j];
     if (mixed && !L.rval32.p) {
         L.rval32.ensure((size_t)(L.nnz > 0 ? L.nnz : 1) * 4);
@@ -651,13 +658,15 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
         scan_ev[0] = a->ctx->get_event();
         scan_ev[1] = a->ctx->get_event();
     }
-    // hits and touched rows accumulate on the device (a->cnt)
+    // hits and touched rows accumulate on the device (a->cnt); Y was
+    // cleared on aux
+    EX_HIP(hipStreamWaitEvent(st, a->ctx->ev_zjoin, 0));
     if (mixed)
         k::sampled_mttkrp<float>(st, L, L.rval32.as<float>(), a->kd.as<uint64_t>(), Ju, a->dm.as<float>(), R,
-                                 a->Y.as<float>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr, ju);
+                                 a->Y.as<float>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr, ju, false);
     else
         k::sampled_mttkrp<double>(st, L, L.rval.as<double>(), a->kd.as<uint64_t>(), Ju, a->dm.as<double>(), R,
-                                  a->Y.as<double>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr, ju);
+                                  a->Y.as<double>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr, ju, false);
     s_mt.end();
     if (a->ctx->timing) {
         a->ctx->pending.push_back({"row_scan", {scan_ev[0], scan_ev[1]}});
@@ -1038,6 +1047,8 @@ int ex_ctx_create(int device, ex_ctx** out) {
         EX_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
         EX_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
         EX_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+        EX_HIP(hipEventCreateWithFlags(&c->ev_zfork, hipEventDisableTiming));
+        EX_HIP(hipEventCreateWithFlags(&c->ev_zjoin, hipEventDisableTiming));
         *out = c.release();
     });
 }
@@ -1049,6 +1060,8 @@ int ex_ctx_destroy(ex_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->aux);
         (void)hipEventDestroy(ctx->ev_fork);
         (void)hipEventDestroy(ctx->ev_join);
+        (void)hipEventDestroy(ctx->ev_zfork);
+        (void)hipEventDestroy(ctx->ev_zjoin);
         (void)hipStreamDestroy(ctx->aux);
         (void)hipStreamDestroy(ctx->c.stream);
         delete ctx;

@@ -241,14 +241,18 @@ int64_t dedup_hash(hipStream_t s, const uint32_t* samples, const double* weights
                    uint64_t* Ju_dev = nullptr, uint64_t* Ju_total = nullptr);
 void design(hipStream_t s, const uint32_t* sd, const double* wd, int64_t Ju, int N, int j, int R,
             const double* const* U, double* d_gram, double* d_mttkrp, const uint64_t* Ju_dev = nullptr);
-// Y (I_j x R, zero-filled here) = sampled MTTKRP of the Ju deduplicated
-// samples with fiber keys kd and design rows `design`.  V = double (FP64
-// path) or float (mixed path): values (L.rval / L.rval32), design rows and Y.
-// counts (optional, device u64[2]): the hits and the touched rows are added.
+// Y (I_j x R, zero-filled here unless zero_y is false: then the caller has
+// cleared it) = sampled MTTKRP of the Ju deduplicated samples with fiber keys
+// kd and design rows `design`.  V = double (FP64 path) or float (mixed path):
+// values (L.rval / L.rval32), design rows and Y.  counts (optional, device
+// u64[2]): the hits and the touched rows are added.
 template <typename V>
 void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju, const V* design,
                     int R, V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts = nullptr,
-                    hipEvent_t* scan_events = nullptr, const uint64_t* Ju_dev = nullptr);
+                    hipEvent_t* scan_events = nullptr, const uint64_t* Ju_dev = nullptr, bool zero_y = true);
+// p[0, bytes) = 0 by a fill kernel (bytes a multiple of 4; no memset node
+// under graph capture).
+void zero_bytes(hipStream_t s, void* p, size_t bytes);
 // Y (I_j x R, zero-filled here) = exact MTTKRP: every nonzero x at (i_0..i_N)
 // adds x * prod_{k != mode} U_k[i_k] to row i_mode.  U: the N factors in V.
 template <typename V>

@@ -226,11 +226,23 @@ __global__ __launch_bounds__(kThreads) void dh_count(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ hfirst,
                                                      uint32_t* __restrict__ tcount) {
     __shared__ uint32_t wsum[kThreads / 64];
+    constexpr int Q = kDhTile / kThreads;
     const int64_t b = (int64_t)blockIdx.x * kDhTile;
-    uint32_t n = 0;
-    for (int q = 0; q < kDhTile / kThreads; q++) {
+    // every slot index, then every first index, loaded together (indices
+    // past J read the last sample's, J >= 1, and count nothing)
+    uint32_t sl[Q], hf[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
         const int64_t i = b + q * kThreads + threadIdx.x;
-        if (i < J) n += hfirst[slotof[i]] == ~(uint32_t)i ? 1u : 0u;
+        sl[q] = slotof[i < J ? i : J - 1];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) hf[q] = hfirst[sl[q]];
+    uint32_t n = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int64_t i = b + q * kThreads + threadIdx.x;
+        n += (i < J && hf[q] == ~(uint32_t)i) ? 1u : 0u;
     }
     for (int off = 32; off >= 1; off >>= 1) n += __shfl_xor(n, off, 64);
     if (threadIdx.x % 64 == 0) wsum[threadIdx.x / 64] = n;
@@ -258,19 +270,29 @@ __global__ __launch_bounds__(kThreads) void dh_scatter(const uint32_t* __restric
     __shared__ uint32_t wsum[kThreads / 64];
     __shared__ uint32_t base;
     const int lane = threadIdx.x % 64, wv = threadIdx.x / 64;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+        // the earlier tiles' counts, summed by the first wavefront (integer,
+        // so in any order)
         uint32_t o = 0;
-        for (unsigned t = 0; t < blockIdx.x; t++) o += tcount[t];
-        base = o;
+        for (unsigned t = threadIdx.x; t < blockIdx.x; t += 64) o += tcount[t];
+        for (int off = 32; off >= 1; off >>= 1) o += __shfl_xor(o, off, 64);
+        if (threadIdx.x == 0) base = o;
     }
-    // thread t takes samples b + PER t .. b + PER t + PER - 1 (in order)
+    // thread t takes samples b + PER t .. b + PER t + PER - 1 (in order);
+    // their slot and first indices are loaded together (past J: the last
+    // sample's, J >= 1)
     const int64_t i0 = (int64_t)blockIdx.x * kDhTile + (int64_t)threadIdx.x * PER;
     bool first[PER];
+    uint32_t sl[PER], hf[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) sl[q] = slotof[i0 + q < J ? i0 + q : J - 1];
+#pragma unroll
+    for (int q = 0; q < PER; q++) hf[q] = hfirst[sl[q]];
     uint32_t mine = 0;
 #pragma unroll
     for (int q = 0; q < PER; q++) {
         const int64_t i = i0 + q;
-        first[q] = i < J && hfirst[slotof[i]] == ~(uint32_t)i;
+        first[q] = i < J && hf[q] == ~(uint32_t)i;
         mine += first[q] ? 1u : 0u;
     }
     // exclusive scan of the per-thread counts over the workgroup
@@ -287,7 +309,7 @@ __global__ __launch_bounds__(kThreads) void dh_scatter(const uint32_t* __restric
     for (int q = 0; q < PER; q++) {
         if (!first[q]) continue;
         const int64_t i = i0 + q;
-        const uint32_t h = slotof[i];
+        const uint32_t h = sl[q];
         const uint32_t u = pre++;
         for (int k = 0; k < ka.N; k++) sd[(int64_t)u * ka.N + k] = samples[i * ka.N + k];
         kd[u] = (uint64_t)(hkey[h] - 1ull);
@@ -2125,8 +2147,9 @@ bool use_hit_list(const ModeLookup& L, int64_t Ju) {
 template <typename V>
 void sampled_mttkrp(hipStream_t s, ModeLookup& L, const V* vals, const uint64_t* kd, int64_t Ju, const V* design,
                     int R, V* Y, int64_t I_j, DevBuf* tmp, uint64_t* counts, hipEvent_t* scan_events,
-                    const uint64_t* Ju_dev) {
-    hipLaunchKernelGGL(zero_fill<V>, dim3(grid_for(I_j * R, kThreads, 8192)), dim3(kThreads), 0, s, Y, I_j * R);
+                    const uint64_t* Ju_dev, bool zero_y) {
+    if (zero_y)
+        hipLaunchKernelGGL(zero_fill<V>, dim3(grid_for(I_j * R, kThreads, 8192)), dim3(kThreads), 0, s, Y, I_j * R);
     if (scan_events) EX_HIP(hipEventRecord(scan_events[0], s));
     if (Ju > 0 && L.nfib > 0 && L.nnz > 0) {
         const int64_t nch = (L.nnz + kScanChunk - 1) / kScanChunk;
@@ -2261,9 +2284,16 @@ template void exact_mttkrp<float>(hipStream_t, ModeLookup&, const float*, const 
                                   int, float*, int64_t, DevBuf*, uint64_t*);
 
 template void sampled_mttkrp<double>(hipStream_t, ModeLookup&, const double*, const uint64_t*, int64_t, const double*,
-                                     int, double*, int64_t, DevBuf*, uint64_t*, hipEvent_t*, const uint64_t*);
+                                     int, double*, int64_t, DevBuf*, uint64_t*, hipEvent_t*, const uint64_t*, bool);
 template void sampled_mttkrp<float>(hipStream_t, ModeLookup&, const float*, const uint64_t*, int64_t, const float*,
-                                    int, float*, int64_t, DevBuf*, uint64_t*, hipEvent_t*, const uint64_t*);
+                                    int, float*, int64_t, DevBuf*, uint64_t*, hipEvent_t*, const uint64_t*, bool);
+
+void zero_bytes(hipStream_t s, void* p, size_t bytes) {
+    const int64_t n = (int64_t)(bytes / 4);
+    if (n == 0) return;
+    hipLaunchKernelGGL(zero_fill<uint32_t>, dim3(grid_for(n, kThreads, 8192)), dim3(kThreads), 0, s, (uint32_t*)p, n);
+    EX_HIP(hipGetLastError());
+}
 
 void fit_bmb(hipStream_t s, const uint32_t* coords, const double* vals, int64_t nnz, int N, const double* const* U,
              const double* sigma, int R, double* out, DevBuf& part) {
