@@ -51,8 +51,8 @@ thread_local std::string g_last_error;
 
 // Launch variants (common.hpp TuneKnob): names, variant counts, current values.
 const TuneInfo kTuneTable[kTuneKnobs] = {
-    {"sts_draw.fp64.r16", 4}, {"sts_draw.fp64.r32", 5}, {"sts_draw.fp32.r16", 4},
-    {"sts_draw.fp32.r32", 5}, {"sts_draw.fp32.r64", 3}, {"row_scan.fp64", 2},
+    {"sts_draw.fp64.r16", 4}, {"sts_draw.fp64.r32", 6}, {"sts_draw.fp32.r16", 4},
+    {"sts_draw.fp32.r32", 7}, {"sts_draw.fp32.r64", 4}, {"row_scan.fp64", 2},
     {"row_scan.fp32", 2}, {"sts_first.fp32", 2},
 };
 std::atomic<int> g_tune[kTuneKnobs];
@@ -1423,15 +1423,21 @@ int ex_sampler_create(ex_ctx* ctx, ex_factors* f, int kind, int precision, uint6
             const int S = tri_stride(R);
             const size_t node_bytes = precision == EX_MIXED ? (size_t)tri_stride32(R) * 4 : (size_t)S * 8;
             s->ge.init(R, 1);
+            // node arrays with zeroed slack after them (the draw's unchecked
+            // vector reads past the last node)
+            auto with_slack = [&](DevBuf& b, size_t bytes) {
+                b.ensure(bytes + kNodeSlackBytes);
+                k::zero_bytes(ctx->c.stream, static_cast<char*>(b.p) + bytes, kNodeSlackBytes);
+            };
             for (int k2 = 0; k2 < N; k2++) {
                 s->g[k2].init(f->dims[k2], R);
-                s->nodes[k2].ensure((size_t)s->g[k2].node_count * node_bytes);
+                with_slack(s->nodes[k2], (size_t)s->g[k2].node_count * node_bytes);
             }
             s->grams.ensure((size_t)N * R * R * 8);
             s->M.ensure((size_t)(N + 2) * R * R * 8);
             s->lam.ensure((size_t)(N + 1) * R * 8);
-            s->eig.ensure((size_t)N * s->ge.node_count * S * 8);
-            if (precision == EX_MIXED) s->eig32.ensure((size_t)N * s->ge.node_count * tri_stride32(R) * 4);
+            with_slack(s->eig, (size_t)N * s->ge.node_count * S * 8);
+            if (precision == EX_MIXED) with_slack(s->eig32, (size_t)N * s->ge.node_count * tri_stride32(R) * 4);
             s->pinvp.ensure((size_t)tri_pad64(R) * 8);
         }
         for (int k2 = 0; k2 < N; k2++) sampler_update_impl(s.get(), k2);

@@ -75,6 +75,10 @@ __host__ __device__ inline int tri_index(int R, int a, int b) { return a * R - (
 __host__ __device__ inline int tri_stride(int R) { return (tri_size(R) + 15) & ~15; }
 // FP32 node stride: packed size rounded up to 32 elements (128 B).
 __host__ __device__ inline int tri_stride32(int R) { return (tri_size(R) + 31) & ~31; }
+// Zeroed bytes after each STS-CP row-tree and eigen-tree node array: the draw
+// reads all NV W-wide vectors of a node's G lanes (up to 17 KB) without a
+// bound check, so the reads past the last node land here.
+constexpr size_t kNodeSlackBytes = 32768;
 // Packed size rounded up to a multiple of 64 (a weight table read G lanes at a
 // time, G <= 64, with no bound check).
 __host__ __device__ inline int tri_pad64(int R) { return (tri_size(R) + 63) & ~63; }

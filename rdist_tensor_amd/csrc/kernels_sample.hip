@@ -253,20 +253,20 @@ __device__ __forceinline__ double quad_g(const TN* __restrict__ node, const TN (
                                          int l) {
     constexpr int W = Vec<TN>::W;
     typedef typename Vec<TN>::type VT;
-    const VT* nv = reinterpret_cast<const VT*>(node);
-    // no branch around the loads, so that they are in flight together: a
-    // vector past the packed triangle reads vector 0 of the node instead
-    // (finite; tree builders zero the stride's padding) and meets xx = 0
+    const VT* nv = reinterpret_cast<const VT*>(node) + l;
+    // no branch and no bound on the loads, so that they are in flight
+    // together from one address with immediate offsets: a vector past the
+    // packed triangle lies in the stride's zero padding, the next node or the
+    // zeroed slack after the node array (or after the staged copies), all
+    // finite, and meets xx = 0
     // (BATCH: every load before the first product, where the registers
     // allow NV more vectors; otherwise each vector is used as it arrives)
+    (void)P;
     TN acc = 0;
     if constexpr (BATCH) {
         VT d[NV];
 #pragma unroll
-        for (int t = 0; t < NV; t++) {
-            const int v = t * G + l;
-            d[t] = nv[v * W < P ? v : 0];
-        }
+        for (int t = 0; t < NV; t++) d[t] = nv[t * G];
 #pragma unroll
         for (int t = 0; t < NV; t++)
 #pragma unroll
@@ -274,8 +274,7 @@ __device__ __forceinline__ double quad_g(const TN* __restrict__ node, const TN (
     } else {
 #pragma unroll
         for (int t = 0; t < NV; t++) {
-            const int v = t * G + l;
-            const VT d = nv[v * W < P ? v : 0];
+            const VT d = nv[t * G];
 #pragma unroll
             for (int i = 0; i < W; i++) acc += d[i] * xx[t * W + i];
         }
@@ -536,7 +535,13 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
     constexpr int NST = C::NSTAGE;
     constexpr int KB = (RMAX + G - 1) / G;   // entries b = l + q G of a lane
     __shared__ typename C::Shared sh;
-    __shared__ __attribute__((aligned(16))) TN stage[2][NST > 0 ? NST * C::STRIDE : 1];
+    // the staged nodes of both trees, then slack for the draw's unchecked
+    // reads past the last staged node; zeroed once below, so that whatever a
+    // read past a node meets is finite
+    constexpr int kStageLen = NST > 0 ? NST * C::STRIDE : 1;
+    constexpr int kStageAll = 2 * kStageLen + NV * G * W;
+    __shared__ __attribute__((aligned(16))) TN stage_all[kStageAll];
+    TN* const stage[2] = {stage_all, stage_all + kStageLen};
     __shared__ double s_ecdf[kMaxRank];   // first mode by prefix sums: components' cumulative totals
     typedef typename Vec<TN>::type VT;
     const int R = a.R, N = a.N, j = a.j;
@@ -547,6 +552,7 @@ __global__ __launch_bounds__(NT) void sts_draw_kernel(StsArgs a) {
         if (e < P) tri_ab_direct(R, e, ea, eb);
         sh.pairs[e] = (uint16_t)(ea | (eb << 8));
     }
+    for (int i = threadIdx.x; i < kStageAll; i += blockDim.x) stage_all[i] = (TN)0;
     // the first non-target mode: prefix sums instead of descents when given
     const int k0 = j == 0 ? 1 : 0;
     if (a.first_cdf != nullptr && threadIdx.x < 64) {
@@ -1022,6 +1028,7 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
                 case 2: DRAW_LAUNCH(double, 64, 5, 1024); break;
                 case 3: DRAW_LAUNCH(double, 32, 9, 256); break;
                 case 4: DRAW_LAUNCH(double, 16, 17, 512); break;
+                case 5: DRAW_LAUNCH(double, 64, 5, 256); break;
                 default: DRAW_LAUNCH(double, 32, 9, 1024); break;
             }
         } else if (R <= 45) {
@@ -1043,6 +1050,8 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
                 case 2: DRAW_LAUNCH(float, 32, 5, 1024); break;
                 case 3: DRAW_LAUNCH(float, 8, 17, 256); break;
                 case 4: DRAW_LAUNCH(float, 8, 17, 512); break;
+                case 5: DRAW_LAUNCH(float, 16, 9, 256); break;
+                case 6: DRAW_LAUNCH(float, 32, 5, 256); break;
                 default: DRAW_LAUNCH(float, 8, 17, 1024); break;
             }
         } else if (R <= 45) {
@@ -1051,6 +1060,7 @@ void sts_draw(hipStream_t s, int N, int j, int R, int64_t J, const StsModeArgs* 
             switch (tune_value(kTuneDrawF32R64)) {
                 case 1: DRAW_LAUNCH(float, 32, 17, 1024); break;
                 case 2: DRAW_LAUNCH(float, 32, 17, 512); break;
+                case 3: DRAW_LAUNCH(float, 64, 9, 256); break;
                 default: DRAW_LAUNCH(float, 64, 9, 1024); break;
             }
         }

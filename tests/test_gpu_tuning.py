@@ -37,7 +37,7 @@ def rel_err(a, b):
 
 
 @pytest.mark.parametrize("name,variant,R", [("sts_draw.fp64.r16", v, 16) for v in range(4)] +
-                         [("sts_draw.fp64.r32", v, 32) for v in range(5)])
+                         [("sts_draw.fp64.r32", v, 32) for v in range(6)])
 def test_fp64_draw_variants_bitexact(ctx, orc, knob, name, variant, R):
     from rdist_tensor_amd import abi
     knob(name, variant)
@@ -65,8 +65,8 @@ def test_fp64_draw_variants_bitexact(ctx, orc, knob, name, variant, R):
 
 
 @pytest.mark.parametrize("name,variant,R", [("sts_draw.fp32.r16", v, 16) for v in range(4)] +
-                         [("sts_draw.fp32.r32", v, 32) for v in range(5)] +
-                         [("sts_draw.fp32.r64", v, 64) for v in range(3)] +
+                         [("sts_draw.fp32.r32", v, 32) for v in range(7)] +
+                         [("sts_draw.fp32.r64", v, 64) for v in range(4)] +
                          [("sts_first.fp32", v, R) for v in range(2) for R in (16, 32, 64)])
 def test_mixed_draw_variants_distribution(ctx, knob, name, variant, R):
     from rdist_tensor_amd import abi
