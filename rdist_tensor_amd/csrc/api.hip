@@ -580,16 +580,21 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
         if (regions) EX_HIP(hipEventRecord(ev[q], st));
     };
     mark(0);
-    // Y is cleared on the second stream while the draw runs (the previous
-    // step's factor update, its last reader, is before this point on st)
+    // a large Y is cleared on the second stream while the draw runs (the
+    // previous step's factor update, its last reader, is before this point
+    // on st); a small one inline, where the fork and join would cost more
     const bool mixed = a->precision == EX_MIXED;
     const size_t vb = mixed ? 4 : 8;   // bytes per design / MTTKRP value
-    a->Y.ensure((size_t)Ij * R * vb);
+    const size_t ybytes = (size_t)Ij * R * vb;
+    a->Y.ensure(ybytes);
     hipStream_t aux = a->ctx->aux;
-    EX_HIP(hipEventRecord(a->ctx->ev_zfork, st));
-    EX_HIP(hipStreamWaitEvent(aux, a->ctx->ev_zfork, 0));
-    k::zero_bytes(aux, a->Y.p, (size_t)Ij * R * vb);
-    EX_HIP(hipEventRecord(a->ctx->ev_zjoin, aux));
+    const bool early_zero = ybytes >= ((size_t)8 << 20);
+    if (early_zero) {
+        EX_HIP(hipEventRecord(a->ctx->ev_zfork, st));
+        EX_HIP(hipStreamWaitEvent(aux, a->ctx->ev_zfork, 0));
+        k::zero_bytes(aux, a->Y.p, ybytes);
+        EX_HIP(hipEventRecord(a->ctx->ev_zjoin, aux));
+    }
     // Region 1: draw samples (accumulator_stationary.hpp:148-153)
     s->samples.ensure((size_t)J * N * 4);
     s->weights.ensure((size_t)J * 8);
@@ -658,15 +663,16 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
         scan_ev[0] = a->ctx->get_event();
         scan_ev[1] = a->ctx->get_event();
     }
-    // hits and touched rows accumulate on the device (a->cnt); Y was
-    // cleared on aux
-    EX_HIP(hipStreamWaitEvent(st, a->ctx->ev_zjoin, 0));
+    // hits and touched rows accumulate on the device (a->cnt)
+    if (early_zero) EX_HIP(hipStreamWaitEvent(st, a->ctx->ev_zjoin, 0));
     if (mixed)
         k::sampled_mttkrp<float>(st, L, L.rval32.as<float>(), a->kd.as<uint64_t>(), Ju, a->dm.as<float>(), R,
-                                 a->Y.as<float>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr, ju, false);
+                                 a->Y.as<float>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr, ju,
+                                 !early_zero);
     else
         k::sampled_mttkrp<double>(st, L, L.rval.as<double>(), a->kd.as<uint64_t>(), Ju, a->dm.as<double>(), R,
-                                  a->Y.as<double>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr, ju, false);
+                                  a->Y.as<double>(), Ij, a->tmp, cnt, a->ctx->timing ? scan_ev : nullptr, ju,
+                                  !early_zero);
     s_mt.end();
     if (a->ctx->timing) {
         a->ctx->pending.push_back({"row_scan", {scan_ev[0], scan_ev[1]}});
