@@ -177,6 +177,7 @@ struct ex_ctx {
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipEvent_t ev_zfork = nullptr, ev_zjoin = nullptr;   // Y cleared on aux beside the draw
+    hipEvent_t ev_pfork = nullptr, ev_pjoin = nullptr;   // mixed STS-CP: Gram and M beside the row tree
     // opt-in per-launch-group timing (ex_ctx_timing): events are recorded
     // around named launch groups on the context's stream and resolved after
     // the stream is synchronised
@@ -374,6 +375,10 @@ struct ex_sampler {
     // set when a batch advanced the factors: every mode's state is rebuilt
     // before the next draw or score read
     bool refresh = false;
+    // mixed STS-CP: the target mode whose M, eigen tree and packed weights
+    // were prepared by the last update (-1: none); any update or M write
+    // clears it
+    int m_ready = -1;
 };
 
 struct ex_als {
@@ -411,8 +416,23 @@ struct ex_als {
 
 namespace {
 
+// STS-CP draw set-up for target mode j, from the Grams alone
+// (sts_cp.hpp:80-188): M's slots, the eigen tree and the packed weights.
+void sts_prepare_m(ex_sampler* s, hipStream_t st, int j) {
+    const int N = s->f->N, R = s->f->R;
+    if (s->precision == EX_MIXED) {
+        k::sts_compute_m_mixed_launch(st, N, j, R, s->grams.as<double>(), s->M.as<double>());
+        k::eig_tree_build32(st, N, j, R, s->M.as<double>(), s->grams.as<double>(), s->eig32.as<float>());
+    } else {
+        k::sts_compute_m(st, N, j, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
+        k::eig_tree_build(st, N, j, R, s->M.as<double>(), s->grams.as<double>(), s->eig.as<double>());
+    }
+    k::pack_upper(st, s->M.as<double>(), R, s->pinvp.as<double>());
+}
+
 // Mixed precision: FP32 factor, BF16 MFMA Grams, FP64 inverse and scores.
-void sampler_update_mixed(ex_sampler* s, int j) {
+// next >= 0 (STS-CP): the draw set-up of target mode next is prepared here.
+void sampler_update_mixed(ex_sampler* s, int j, int next) {
     ex_factors* f = s->f;
     hipStream_t st = s->ctx->c.stream;
     const int R = f->R;
@@ -431,15 +451,25 @@ void sampler_update_mixed(ex_sampler* s, int j) {
         k::dagat32(st, U, I, R, s->gpinv.as<double>(), s->scores[j].as<double>());
         k::arls_cdf(st, s->scores[j].as<double>(), I, s->cdf[j].as<double>(), s->sums.as<double>() + j, s->part);
     } else {
-        // the root of the row tree is U_j^T U_j
+        // the row tree on st; beside it on the second stream U_j^T U_j (the
+        // tree's root, from the same BF16 products) and, for the next target
+        // mode, the draw set-up, which reads only the Grams
+        hipStream_t aux = s->ctx->aux;
+        EX_HIP(hipEventRecord(s->ctx->ev_pfork, st));
+        EX_HIP(hipStreamWaitEvent(aux, s->ctx->ev_pfork, 0));
         k::tree_build_f32(st, U, s->g[j], R, s->nodes[j].as<float>());
-        k::unpack_root<float>(st, s->nodes[j].as<float>(), R, s->grams.as<double>() + (int64_t)j * R * R);
+        k::gram_bf16(aux, U, I, R, s->grams.as<double>() + (int64_t)j * R * R, s->part);
+        if (next >= 0) sts_prepare_m(s, aux, next);
+        EX_HIP(hipEventRecord(s->ctx->ev_pjoin, aux));
+        EX_HIP(hipStreamWaitEvent(st, s->ctx->ev_pjoin, 0));
+        s->m_ready = next;
     }
     sp.end();
 }
 
-void sampler_update_impl(ex_sampler* s, int j) {
-    if (s->precision == EX_MIXED) return sampler_update_mixed(s, j);
+void sampler_update_impl(ex_sampler* s, int j, int next = -1) {
+    s->m_ready = -1;
+    if (s->precision == EX_MIXED) return sampler_update_mixed(s, j, next);
     ex_factors* f = s->f;
     hipStream_t st = s->ctx->c.stream;
     const int R = f->R;
@@ -492,11 +522,8 @@ void sampler_draw_impl(ex_sampler* s, int64_t J, int j, uint64_t sweep, const do
     } else {
         const bool mixed = s->precision == EX_MIXED;
         Span sm(s->ctx, "sts_compute_m");
-        if (mixed) k::sts_compute_m_mixed_launch(st, N, j, R, s->grams.as<double>(), s->M.as<double>());
-        else k::sts_compute_m(st, N, j, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
-        if (mixed) k::eig_tree_build32(st, N, j, R, s->M.as<double>(), s->grams.as<double>(), s->eig32.as<float>());
-        else k::eig_tree_build(st, N, j, R, s->M.as<double>(), s->grams.as<double>(), s->eig.as<double>());
-        k::pack_upper(st, s->M.as<double>(), R, s->pinvp.as<double>());
+        if (s->m_ready != j) sts_prepare_m(s, st, j);
+        s->m_ready = -1;
         sm.end();
         k::StsModeArgs m[kMaxModes];
         if (mixed) f->sync32_all();
@@ -707,7 +734,9 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     s_fu.end();
     mark(4);
     // Region 7: sampler update (:291-294)
-    sampler_update_impl(s, j);
+    // mixed STS-CP: the next mode's draw set-up runs beside this tree build
+    const int next = mixed && s->kind == EX_SAMPLER_STS_CP && j + 1 < N ? j + 1 : -1;
+    sampler_update_impl(s, j, next);
     mark(5);
     // region times are read back later (als_resolve_regions): no wait here
     if (!regions) return;
@@ -1055,6 +1084,8 @@ int ex_ctx_create(int device, ex_ctx** out) {
         EX_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
         EX_HIP(hipEventCreateWithFlags(&c->ev_zfork, hipEventDisableTiming));
         EX_HIP(hipEventCreateWithFlags(&c->ev_zjoin, hipEventDisableTiming));
+        EX_HIP(hipEventCreateWithFlags(&c->ev_pfork, hipEventDisableTiming));
+        EX_HIP(hipEventCreateWithFlags(&c->ev_pjoin, hipEventDisableTiming));
         *out = c.release();
     });
 }
@@ -1068,6 +1099,8 @@ int ex_ctx_destroy(ex_ctx* ctx) {
         (void)hipEventDestroy(ctx->ev_join);
         (void)hipEventDestroy(ctx->ev_zfork);
         (void)hipEventDestroy(ctx->ev_zjoin);
+        (void)hipEventDestroy(ctx->ev_pfork);
+        (void)hipEventDestroy(ctx->ev_pjoin);
         (void)hipStreamDestroy(ctx->aux);
         (void)hipStreamDestroy(ctx->c.stream);
         delete ctx;
@@ -1512,6 +1545,7 @@ int ex_sts_compute_m(ex_sampler* s, int mode, double* M) {
         EX_CHECK(mode >= 0 && mode < s->f->N, EX_ERR_ARG, "mode out of range");
         const int N = s->f->N, R = s->f->R;
         hipStream_t st = s->ctx->c.stream;
+        s->m_ready = -1;
         if (s->precision == EX_MIXED) k::sts_compute_m_mixed_launch(st, N, mode, R, s->grams.as<double>(), s->M.as<double>());
         else k::sts_compute_m(st, N, mode, R, s->grams.as<double>(), s->M.as<double>(), s->lam.as<double>());
         d2h(st, M, s->M.p, (size_t)(N + 2) * R * R * 8);
