@@ -159,6 +159,37 @@ def test_mixed_factor_copies_stay_consistent(ctx):
     assert np.isfinite(A.fit())
 
 
+@pytest.mark.parametrize("R", [16, 32])
+def test_mixed_prepared_draw_setup_equals_fresh(ctx, R):
+    """A mixed STS-CP step prepares the next mode's computeM, eigen trees and
+    packed weights on the second stream beside its row-tree rebuild.  A draw
+    of that mode from the prepared set-up equals, bit for bit, a draw after
+    the set-up is recomputed (ex_sts_compute_m clears the prepared mark), and
+    a draw of another mode recomputes its own set-up."""
+    from rdist_tensor_amd import abi
+    rng = np.random.default_rng(700 + R)
+    dims = [400, 300, 200, 150]
+    idx, vals = random_tensor(rng, dims, 30000)
+    U = normalized_factors(rng, dims, R)
+    T = abi.Tensor(ctx, idx, vals, dims)
+    F = set_factors(ctx, dims, R, U)
+    S = abi.Sampler(ctx, F, abi.EX_SAMPLER_STS_CP, precision=abi.EX_MIXED, seed=9)
+    A = abi.ALS(ctx, T, F, S, precision=abi.EX_MIXED)
+    A.init()
+    A.step(0, 2048, 1)                       # prepares mode 1's draw set-up
+    s_pre, w_pre = S.draw(4096, 1, sweep=5)
+    S.sts_m(1)                               # recomputes M, clears the mark
+    s_new, w_new = S.draw(4096, 1, sweep=5)  # recomputes the whole set-up
+    assert np.array_equal(s_pre, s_new)
+    assert np.array_equal(w_pre, w_new)
+    assert np.isfinite(w_pre).all() and (w_pre > 0).all()
+    A.step(1, 2048, 1)                       # prepares mode 2's
+    s_a, w_a = S.draw(4096, 3, sweep=6)      # another mode: its own set-up
+    s_b, w_b = S.draw(4096, 3, sweep=6)
+    assert np.array_equal(s_a, s_b)
+    assert np.array_equal(w_a, w_b)
+
+
 def test_four_mode_rank64_shape(ctx):
     """The 4-mode R = 64 shape of the at-scale STS-CP workload (Zipf(1.2) on
     modes 0-1, uniform on 2-3, 1 + Poisson(2) values, J = 131,072) with every
