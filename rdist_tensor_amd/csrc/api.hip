@@ -431,8 +431,11 @@ void sts_prepare_m(ex_sampler* s, hipStream_t st, int j) {
 }
 
 // Mixed precision: FP32 factor, BF16 MFMA Grams, FP64 inverse and scores.
-// next >= 0 (STS-CP): the draw set-up of target mode next is prepared here.
-void sampler_update_mixed(ex_sampler* s, int j, int next) {
+// STS-CP: next >= 0 asks for the draw set-up of target mode next.  With
+// gram_done the caller has written U_j^T U_j into grams[j] and, for next >= 0,
+// enqueued that set-up on the second stream ending with ev_pjoin; otherwise
+// both are enqueued here.
+void sampler_update_mixed(ex_sampler* s, int j, int next, bool gram_done) {
     ex_factors* f = s->f;
     hipStream_t st = s->ctx->c.stream;
     const int R = f->R;
@@ -455,21 +458,25 @@ void sampler_update_mixed(ex_sampler* s, int j, int next) {
         // tree's root, from the same BF16 products) and, for the next target
         // mode, the draw set-up, which reads only the Grams
         hipStream_t aux = s->ctx->aux;
-        EX_HIP(hipEventRecord(s->ctx->ev_pfork, st));
-        EX_HIP(hipStreamWaitEvent(aux, s->ctx->ev_pfork, 0));
+        if (!gram_done) {
+            EX_HIP(hipEventRecord(s->ctx->ev_pfork, st));
+            EX_HIP(hipStreamWaitEvent(aux, s->ctx->ev_pfork, 0));
+        }
         k::tree_build_f32(st, U, s->g[j], R, s->nodes[j].as<float>());
-        k::gram_bf16(aux, U, I, R, s->grams.as<double>() + (int64_t)j * R * R, s->part);
-        if (next >= 0) sts_prepare_m(s, aux, next);
-        EX_HIP(hipEventRecord(s->ctx->ev_pjoin, aux));
-        EX_HIP(hipStreamWaitEvent(st, s->ctx->ev_pjoin, 0));
+        if (!gram_done) {
+            k::gram_bf16(aux, U, I, R, s->grams.as<double>() + (int64_t)j * R * R, s->part);
+            if (next >= 0) sts_prepare_m(s, aux, next);
+            EX_HIP(hipEventRecord(s->ctx->ev_pjoin, aux));
+        }
+        if (!gram_done || next >= 0) EX_HIP(hipStreamWaitEvent(st, s->ctx->ev_pjoin, 0));
         s->m_ready = next;
     }
     sp.end();
 }
 
-void sampler_update_impl(ex_sampler* s, int j, int next = -1) {
+void sampler_update_impl(ex_sampler* s, int j, int next = -1, bool gram_done = false) {
     s->m_ready = -1;
-    if (s->precision == EX_MIXED) return sampler_update_mixed(s, j, next);
+    if (s->precision == EX_MIXED) return sampler_update_mixed(s, j, next, gram_done);
     ex_factors* f = s->f;
     hipStream_t st = s->ctx->c.stream;
     const int R = f->R;
@@ -713,6 +720,8 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     }
     mark(3);
     // Region 6: U_j = Y G+, renormalise (:280-289)
+    const bool sts_mixed = mixed && s->kind == EX_SAMPLER_STS_CP;
+    const int next = sts_mixed && j + 1 < N ? j + 1 : -1;
     EX_HIP(hipStreamWaitEvent(st, a->ctx->ev_join, 0));
     Span s_fu(a->ctx, "factor_update");
     if (mixed) {
@@ -721,7 +730,18 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
         a->GY.ensure((size_t)R * R * 8);
         a->Ms.ensure((size_t)R * R * 4);
         k::gram_f32in(st, a->Y.as<float>(), Ij, R, a->GY.as<double>(), a->part);
-        k::sigma_from_gram(st, a->GY.as<double>(), a->Gi.as<double>(), R, f->sigma.as<double>(), a->Ms.as<float>());
+        // STS-CP: U_j^T U_j from the same small matrices, so the next mode's
+        // draw set-up (which reads only the Grams) runs on the second stream
+        // beside the pass over Y and the row-tree rebuild
+        double* gj = sts_mixed ? s->grams.as<double>() + (int64_t)j * R * R : nullptr;
+        k::sigma_from_gram(st, a->GY.as<double>(), a->Gi.as<double>(), R, f->sigma.as<double>(), a->Ms.as<float>(),
+                           gj);
+        if (next >= 0) {
+            EX_HIP(hipEventRecord(a->ctx->ev_pfork, st));
+            EX_HIP(hipStreamWaitEvent(aux, a->ctx->ev_pfork, 0));
+            sts_prepare_m(s, aux, next);
+            EX_HIP(hipEventRecord(a->ctx->ev_pjoin, aux));
+        }
         k::dsymm32(st, a->Ms.as<float>(), a->Y.as<float>(), Ij, R, Uj);
         f->wrote32(j);
     } else {
@@ -734,9 +754,7 @@ void als_step_impl(ex_als* a, int j, int64_t J, uint64_t sweep) {
     s_fu.end();
     mark(4);
     // Region 7: sampler update (:291-294)
-    // mixed STS-CP: the next mode's draw set-up runs beside this tree build
-    const int next = mixed && s->kind == EX_SAMPLER_STS_CP && j + 1 < N ? j + 1 : -1;
-    sampler_update_impl(s, j, next);
+    sampler_update_impl(s, j, next, sts_mixed);
     mark(5);
     // region times are read back later (als_resolve_regions): no wait here
     if (!regions) return;

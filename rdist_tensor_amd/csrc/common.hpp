@@ -301,8 +301,10 @@ void tns_scan(const char* path, int64_t& nnz, int& N);
 void tns_read(const char* path, int64_t nnz, int N, uint64_t* coords, double* vals, uint64_t* mins, uint64_t* maxs);
 namespace k {
 void renormalize32(hipStream_t s, float* U, int64_t m, int R, double* sigma, DevBuf& part);
-// sigma_c = ||(Y Gi)[:, c]|| from GY = Y^T Y; Ms = Gi diag(1 / sigma) (FP32)
-void sigma_from_gram(hipStream_t s, const double* GY, const double* Gi, int R, double* sigma, float* Ms);
+// sigma_c = ||(Y Gi)[:, c]|| from GY = Y^T Y; Ms = Gi diag(1 / sigma) (FP32);
+// gu (optional, R x R): U^T U for U = Y Ms, as Ms^T GY Ms in FP64
+void sigma_from_gram(hipStream_t s, const double* GY, const double* Gi, int R, double* sigma, float* Ms,
+                     double* gu = nullptr);
 void dagat32(hipStream_t s, const float* A, int64_t m, int R, const double* G, double* res);
 // sort / scan primitives (rocPRIM)
 size_t sort_pairs_u64_u32(hipStream_t s, void* temp, size_t temp_bytes, const uint64_t* kin, uint64_t* kout,

@@ -359,7 +359,8 @@ __global__ __launch_bounds__(kThreads) void design_f32(const uint32_t* __restric
 // Ms = Gi diag(1 / sigma) in FP32, so that Y Ms is U with unit columns.
 __global__ __launch_bounds__(1024) void sigma_from_gram_kernel(const double* __restrict__ GY,
                                                               const double* __restrict__ Gi, int R,
-                                                              double* __restrict__ sigma, float* __restrict__ Ms) {
+                                                              double* __restrict__ sigma, float* __restrict__ Ms,
+                                                              double* __restrict__ gu) {
     __shared__ double gi[kMaxRank * kMaxRank];
     __shared__ double t[kMaxRank * kMaxRank];   // Gi[a][c] (GY Gi)[a][c], then sigma
     const int R2 = R * R;
@@ -383,7 +384,29 @@ __global__ __launch_bounds__(1024) void sigma_from_gram_kernel(const double* __r
     __syncthreads();
     if (threadIdx.x < R) t[threadIdx.x] = sg;
     __syncthreads();
-    for (int o = threadIdx.x; o < R2; o += blockDim.x) Ms[o] = (float)(gi[o] / t[o % R]);
+    for (int o = threadIdx.x; o < R2; o += blockDim.x) {
+        const float m = (float)(gi[o] / t[o % R]);
+        Ms[o] = m;
+        gi[o] = (double)m;   // each thread rewrites only the entries it read
+    }
+    if (!gu) return;
+    // U^T U = Ms^T (Y^T Y) Ms for U = Y Ms with the FP32 Ms above; entry (c, d)
+    // and (d, c) are the same sum, so the result is exactly symmetric
+    __syncthreads();
+    for (int o = threadIdx.x; o < R2; o += blockDim.x) {
+        const int a = o / R, d = o % R;
+        double acc = 0.0;
+        for (int b = 0; b < R; b++) acc += GY[a * R + b] * gi[b * R + d];
+        t[o] = acc;
+    }
+    __syncthreads();
+    for (int o = threadIdx.x; o < R2; o += blockDim.x) {
+        const int c0 = o / R, d0 = o % R;
+        const int c = c0 < d0 ? c0 : d0, d = c0 < d0 ? d0 : c0;
+        double acc = 0.0;
+        for (int a = 0; a < R; a++) acc += gi[a * R + c] * t[a * R + d];
+        gu[o] = acc;
+    }
 }
 
 // Column sums of squares of an FP32 matrix, accumulated in FP64.
@@ -554,8 +577,9 @@ void renormalize32(hipStream_t s, float* U, int64_t m, int R, double* sigma, Dev
     EX_HIP(hipGetLastError());
 }
 
-void sigma_from_gram(hipStream_t s, const double* GY, const double* Gi, int R, double* sigma, float* Ms) {
-    hipLaunchKernelGGL(sigma_from_gram_kernel, dim3(1), dim3(1024), 0, s, GY, Gi, R, sigma, Ms);
+void sigma_from_gram(hipStream_t s, const double* GY, const double* Gi, int R, double* sigma, float* Ms,
+                     double* gu) {
+    hipLaunchKernelGGL(sigma_from_gram_kernel, dim3(1), dim3(1024), 0, s, GY, Gi, R, sigma, Ms, gu);
     EX_HIP(hipGetLastError());
 }
 
