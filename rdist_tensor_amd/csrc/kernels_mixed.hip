@@ -382,7 +382,10 @@ __global__ __launch_bounds__(1024) void sigma_from_gram_kernel(const double* __r
         sigma[c] = sg;
     }
     __syncthreads();
-    if (threadIdx.x < R) t[threadIdx.x] = sg;
+    // a column whose squared norm rounds to zero or below (Y nearly rank
+    // deficient along it) keeps its values unscaled, with sigma = 0: its
+    // component carries no weight, where 1 / 0 would fill it with inf
+    if (threadIdx.x < R) t[threadIdx.x] = sg > 0.0 ? sg : 1.0;
     __syncthreads();
     for (int o = threadIdx.x; o < R2; o += blockDim.x) {
         const float m = (float)(gi[o] / t[o % R]);
